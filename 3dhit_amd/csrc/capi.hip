@@ -1,0 +1,863 @@
+// C ABI of the hit3d HIP library: argument checks, workspace layouts and the
+// launch sequences of the RHS / RK4 step (integrators.py:32-53,
+// navierstokes.py:62-97) and the diagnostics (diagnostics.py:45-114).
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hit3d.h"
+#include "misc.hpp"
+#include "passes.hpp"
+
+using h3::cpx;
+using h3::Geo;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define H3_HIP(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return fail(HIT3D_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Per-pass HIP-event timing for hit3d_profile_rk4: one event after every
+// launch on the stream, so consecutive events bracket one kernel each.
+enum PassId { kP1 = 0, kP2, kP3, kP4, kP5, kFinalize, kOther, kNumPass };
+
+struct PassTimer {
+  std::vector<hipEvent_t> ev;
+  std::vector<int> id;
+  hipStream_t s = nullptr;
+  bool ok = true;
+  void record(int pass) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) { ok = false; return; }
+    if (hipEventRecord(e, s) != hipSuccess) { ok = false; hipEventDestroy(e); return; }
+    ev.push_back(e);
+    id.push_back(pass);
+  }
+  ~PassTimer() {
+    for (hipEvent_t e : ev) hipEventDestroy(e);
+  }
+};
+
+inline void mark(PassTimer* t, int pass) {
+  if (t) t->record(pass);
+}
+
+bool pow2_ok(int64_t n) { return n >= 4 && n <= 1024 && (n & (n - 1)) == 0; }
+
+int check_grid(const hit3d_grid* gd) {
+  if (!gd) return fail(HIT3D_ERR_INVALID, "grid is NULL");
+  if (!pow2_ok(gd->nx) || !pow2_ok(gd->ny) || !pow2_ok(gd->nz))
+    return fail(HIT3D_ERR_INVALID, "grid sizes must be powers of two in [4, 1024], got " +
+                                       std::to_string(gd->nx) + "x" + std::to_string(gd->ny) +
+                                       "x" + std::to_string(gd->nz));
+  if (gd->precision != 4 && gd->precision != 8)
+    return fail(HIT3D_ERR_INVALID, "precision must be 4 (float32) or 8 (float64), got " +
+                                       std::to_string(gd->precision));
+  if (!(gd->lx > 0 && gd->ly > 0 && gd->lz > 0) || !std::isfinite(gd->lx) ||
+      !std::isfinite(gd->ly) || !std::isfinite(gd->lz))
+    return fail(HIT3D_ERR_INVALID, "box lengths must be positive and finite");
+  if (gd->reserved != 0) return fail(HIT3D_ERR_INVALID, "grid.reserved must be 0");
+  return HIT3D_OK;
+}
+
+int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+// pruned: keep only |n_i| <= N_i/3 (compact rows padded to 16 elements);
+// dense: every mode, compact layout == API layout.
+Geo make_geo(const hit3d_grid& gd, bool pruned) {
+  Geo g{};
+  g.nx = int(gd.nx);
+  g.ny = int(gd.ny);
+  g.nz = int(gd.nz);
+  g.nzh = g.nz / 2 + 1;
+  g.cx = g.nx / 3;
+  g.cy = g.ny / 3;
+  g.cz = g.nz / 3;
+  if (pruned) {
+    g.kx = 2 * g.cx + 1;
+    g.ky = 2 * g.cy + 1;
+    g.kz = g.cz + 1;
+    g.jp = int(round_up(int64_t(g.ky) * g.kz, 16));
+  } else {
+    g.kx = g.nx;
+    g.ky = g.ny;
+    g.kz = g.nzh;
+    g.jp = g.ny * g.nzh;
+  }
+  g.px = (g.kx + 1) / 2;
+  g.py = (g.ky + 1) / 2;
+  g.k0x = 2 * M_PI / gd.lx;
+  g.k0y = 2 * M_PI / gd.ly;
+  g.k0z = 2 * M_PI / gd.lz;
+  return g;
+}
+
+int64_t shell_bins(const hit3d_grid& gd) {
+  const int64_t n = std::max(gd.nx, std::max(gd.ny, gd.nz));
+  return int64_t(std::ceil(std::sqrt(3.0) * double(n) / 2.0)) + 1 + 1;
+}
+
+// Workspace carving: fixed order, 256-byte aligned slices.
+struct Carve {
+  size_t off = 0;
+  char* base = nullptr;
+  void* take(size_t bytes) {
+    off = (off + 255) & ~size_t(255);
+    void* p = base ? base + off : nullptr;
+    off += bytes;
+    return p;
+  }
+};
+
+struct StepBuffers {
+  void* u_c = nullptr;  // compact step-start state (pruned only)
+  void* us = nullptr;   // stage input / rhs output
+  void* acc = nullptr;  // RK4 accumulator
+  void* bufA = nullptr; // [5][nx][jp]
+  void* bufB = nullptr; // [6][nx][ny][kz]
+  double* partials = nullptr;
+  double* scalars = nullptr;
+};
+
+size_t cbytes(const hit3d_grid& gd) { return size_t(2) * gd.precision; }
+
+StepBuffers carve_step(const hit3d_grid& gd, const Geo& g, bool pruned, bool rk4, void* work,
+                       size_t* total) {
+  Carve c;
+  c.base = static_cast<char*>(work);
+  const size_t cb = cbytes(gd);
+  const size_t st = size_t(3) * g.kx * g.jp * cb;
+  StepBuffers b;
+  if (pruned) b.u_c = c.take(st);
+  b.us = c.take(st);
+  if (rk4) b.acc = c.take(st);
+  b.bufA = c.take(size_t(5) * g.nx * g.jp * cb);
+  b.bufB = c.take(size_t(6) * g.nx * g.ny * g.kz * cb);
+  const int nb = std::max(h3::x_stage_blocks(g.nx, g.ky * g.kz), h3::kReduceBlocks);
+  b.partials = static_cast<double*>(c.take(size_t(3) * nb * sizeof(double)));
+  b.scalars = static_cast<double*>(c.take(16 * sizeof(double)));
+  *total = c.off;
+  return b;
+}
+
+size_t reduce_bytes(const hit3d_grid& gd, double** partials, double** scalars, void* work) {
+  Carve c;
+  c.base = static_cast<char*>(work);
+  double* p = static_cast<double*>(c.take(size_t(3) * h3::kReduceBlocks * sizeof(double)));
+  double* s = static_cast<double*>(c.take(16 * sizeof(double)));
+  if (partials) *partials = p;
+  if (scalars) *scalars = s;
+  (void)gd;
+  return c.off;
+}
+
+size_t spectrum_bytes(const hit3d_grid& gd, double** partials, void* work) {
+  Carve c;
+  c.base = static_cast<char*>(work);
+  const Geo g = make_geo(gd, false);
+  double* p = static_cast<double*>(
+      c.take(size_t(h3::spectrum_blocks(g)) * shell_bins(gd) * sizeof(double)));
+  if (partials) *partials = p;
+  return c.off;
+}
+
+size_t transform_bytes(const hit3d_grid& gd, int64_t ncomp) {
+  return size_t(ncomp) * gd.nx * gd.ny * (gd.nz / 2 + 1) * cbytes(gd);
+}
+
+// ---- FP64 finalize: partial sums -> totals and the forcing coefficient ----
+
+struct FinalizeArgs {
+  int kind;        // HIT3D_FORCING_*
+  double eps;      // band injection
+  double nu, energy, tau;
+  double tiny;     // finfo(dtype).eps
+  double inv_ntot2;
+};
+
+__global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ partials, int nblocks,
+                                                  FinalizeArgs fa, double* __restrict__ scalars) {
+  __shared__ double sh[3][256];
+  h3::KSum s[3];
+  for (int i = threadIdx.x; i < nblocks; i += 256)
+    for (int c = 0; c < 3; ++c) s[c].add(partials[3 * i + c]);
+  for (int c = 0; c < 3; ++c) sh[c][threadIdx.x] = s[c].value();
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int c = 0; c < 3; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double twoE = sh[0][0] * fa.inv_ntot2, twoO = sh[1][0] * fa.inv_ntot2,
+                 twoB = sh[2][0] * fa.inv_ntot2;
+    double coef = 0.0;
+    if (fa.kind == HIT3D_FORCING_BAND) {
+      coef = twoB > fa.tiny ? fa.eps / twoB : 0.0;
+    } else if (fa.kind == HIT3D_FORCING_LUNDGREN) {
+      if (twoE > fa.tiny) {
+        double rate = fa.nu * twoO;
+        if (fa.tau > 0.0 && isfinite(fa.tau)) rate -= (0.5 * twoE - fa.energy) / fa.tau;
+        coef = rate / twoE;
+      }
+    }
+    scalars[0] = coef;
+    scalars[1] = twoE;
+    scalars[2] = twoO;
+    scalars[3] = twoB;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_spectrum_final(const double* __restrict__ partials,
+                                                        int nblocks, int nbins, double scale,
+                                                        double* __restrict__ out) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= nbins) return;
+  h3::KSum s;
+  for (int i = 0; i < nblocks; ++i) s.add(partials[int64_t(i) * nbins + b]);
+  out[b] = s.value() * scale;
+}
+
+__global__ void k_energy_out(const double* __restrict__ scalars, double nu,
+                             double* __restrict__ out) {
+  // [E, eps, Omega, E_band]
+  out[0] = 0.5 * scalars[1];
+  out[1] = nu * scalars[2];
+  out[2] = 0.5 * scalars[2];
+  out[3] = 0.5 * scalars[3];
+}
+
+double ntot2_inv(const hit3d_grid& gd) {
+  const double n = double(gd.nx) * double(gd.ny) * double(gd.nz);
+  return 1.0 / (n * n);
+}
+
+int check_forcing(const hit3d_forcing* f) {
+  if (!f) return HIT3D_OK;
+  if (f->reserved != 0) return fail(HIT3D_ERR_INVALID, "forcing.reserved must be 0");
+  if (f->kind < HIT3D_FORCING_NONE || f->kind > HIT3D_FORCING_LUNDGREN)
+    return fail(HIT3D_ERR_INVALID, "unknown forcing kind " + std::to_string(f->kind));
+  return HIT3D_OK;
+}
+
+FinalizeArgs finalize_args(const hit3d_grid& gd, const hit3d_forcing* f) {
+  FinalizeArgs fa{};
+  fa.kind = f ? f->kind : HIT3D_FORCING_NONE;
+  if (f) {
+    fa.eps = f->eps;
+    fa.nu = f->nu;
+    fa.energy = f->energy;
+    fa.tau = f->tau;
+  }
+  fa.tiny = gd.precision == 4 ? double(FLT_EPSILON) : DBL_EPSILON;
+  fa.inv_ntot2 = ntot2_inv(gd);
+  return fa;
+}
+
+bool needs_sums(const hit3d_forcing* f) {
+  return f && (f->kind == HIT3D_FORCING_BAND || f->kind == HIT3D_FORCING_LUNDGREN);
+}
+
+// P1-P4 of one RHS evaluation on the compact stage input `us`.
+template <class T>
+int rhs_passes(const hit3d_grid& gd, const Geo& g, const cpx<T>* us, cpx<T>* bufA, cpx<T>* bufB,
+               hipStream_t s, PassTimer* tm = nullptr) {
+  const int64_t st_fs = int64_t(g.kx) * g.jp;
+  const int64_t a_fs = int64_t(g.nx) * g.jp;
+  const int64_t b_fs = int64_t(g.nx) * g.ny * g.kz;
+  H3_HIP(h3::launch_x_inverse_curl<T>(g.nx, us, st_fs, bufA, a_fs, g, s));
+  mark(tm, kP1);
+  H3_HIP(h3::launch_y_inverse_curl<T>(g.ny, bufA, a_fs, bufB, b_fs, g, s));
+  mark(tm, kP2);
+  const double n = double(gd.nx) * double(gd.ny) * double(gd.nz);
+  H3_HIP(h3::launch_z_fused<T>(g.nz, bufB, b_fs, int64_t(g.nx) * g.ny, g.kz, T(1.0 / (n * n)), s));
+  mark(tm, kP3);
+  h3::StridedArgs<T> y{};
+  y.in = bufB;
+  y.out = bufA;
+  y.in_fs = b_fs;
+  y.out_fs = a_fs;
+  y.in_slab = int64_t(g.ny) * g.kz;
+  y.out_slab = g.jp;
+  y.B = g.kz;
+  y.J = g.nx * g.kz;
+  y.nf = 3;
+  y.in_k = g.ny;
+  y.in_p = (g.ny + 1) / 2;
+  y.out_k = g.ky;
+  y.out_p = g.py;
+  H3_HIP(h3::launch_strided<T>(g.ny, -1, y, s));
+  mark(tm, kP4);
+  return HIT3D_OK;
+}
+
+template <class T>
+h3::StageArgs<T> stage_args(const hit3d_grid& gd, const Geo& g, const hit3d_forcing* f, double nu,
+                            double dt, const cpx<T>* bufA, double* scalars) {
+  h3::StageArgs<T> a{};
+  a.nl = bufA;
+  a.nl_fs = int64_t(g.nx) * g.jp;
+  a.st_fs = int64_t(g.kx) * g.jp;
+  a.nu = T(nu);
+  a.dt = T(dt);
+  a.fterm = h3::kTermNone;
+  a.coef_const = T(0);
+  a.coef_dev = nullptr;
+  a.kmin2 = T(1);
+  a.kmax2 = T(0);
+  if (f) {
+    switch (f->kind) {
+      case HIT3D_FORCING_LINEAR:
+        a.fterm = h3::kTermUniform;
+        a.coef_const = T(f->a);
+        break;
+      case HIT3D_FORCING_BAND:
+        a.fterm = h3::kTermBand;
+        a.coef_dev = scalars;
+        a.kmin2 = T(f->kmin * f->kmin);
+        a.kmax2 = T(f->kmax * f->kmax);
+        break;
+      case HIT3D_FORCING_LUNDGREN:
+        a.fterm = h3::kTermUniform;
+        a.coef_dev = scalars;
+        break;
+      default:
+        break;
+    }
+  }
+  a.g = g;
+  (void)gd;
+  return a;
+}
+
+template <class T>
+int reduce_state(const hit3d_grid& gd, const Geo& g, const cpx<T>* st, const hit3d_forcing* f,
+                 double* partials, double* scalars, hipStream_t s) {
+  T kmin2 = T(1), kmax2 = T(0);
+  if (f && f->kind == HIT3D_FORCING_BAND) {
+    kmin2 = T(f->kmin * f->kmin);
+    kmax2 = T(f->kmax * f->kmax);
+  }
+  H3_HIP(h3::launch_quad_partials<T>(st, int64_t(g.kx) * g.jp, g, kmin2, kmax2, partials, s));
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, partials, h3::kReduceBlocks,
+                     finalize_args(gd, f), scalars);
+  H3_HIP(hipGetLastError());
+  return HIT3D_OK;
+}
+
+template <class T>
+int rhs_impl(const hit3d_grid& gd, const cpx<T>* u_hat, cpx<T>* du_hat, double nu,
+             const hit3d_forcing* f, bool pruned, void* work, hipStream_t s) {
+  const Geo g = make_geo(gd, pruned);
+  size_t total;
+  StepBuffers b = carve_step(gd, g, pruned, false, work, &total);
+  const int64_t st_fs = int64_t(g.kx) * g.jp;
+  const int64_t full_fs = int64_t(g.nx) * g.ny * g.nzh;
+  const cpx<T>* us = u_hat;
+  cpx<T>* out = du_hat;
+  if (pruned) {
+    H3_HIP(h3::launch_gather<T>(u_hat, full_fs, static_cast<cpx<T>*>(b.u_c), st_fs, 3, g, s));
+    us = static_cast<cpx<T>*>(b.u_c);
+    out = static_cast<cpx<T>*>(b.us);
+  }
+  if (needs_sums(f)) {
+    int rc = reduce_state<T>(gd, g, us, f, b.partials, b.scalars, s);
+    if (rc) return rc;
+  }
+  int rc = rhs_passes<T>(gd, g, us, static_cast<cpx<T>*>(b.bufA), static_cast<cpx<T>*>(b.bufB), s);
+  if (rc) return rc;
+  h3::StageArgs<T> a = stage_args<T>(gd, g, f, nu, 0.0, static_cast<cpx<T>*>(b.bufA), b.scalars);
+  a.us = us;
+  a.out = out;
+  H3_HIP(h3::launch_x_forward_stage<T>(g.nx, h3::kStageRhs, a, s, nullptr));
+  if (pruned) H3_HIP(h3::launch_scatter<T>(out, st_fs, du_hat, full_fs, 3, g, s));
+  return HIT3D_OK;
+}
+
+template <class T>
+int rk4_impl(const hit3d_grid& gd, cpx<T>* u_hat, double nu, double dt, int64_t nsteps,
+             const hit3d_forcing* f, bool pruned, double* diag, void* work, hipStream_t s,
+             PassTimer* tm = nullptr) {
+  const Geo g = make_geo(gd, pruned);
+  size_t total;
+  StepBuffers b = carve_step(gd, g, pruned, true, work, &total);
+  const int64_t st_fs = int64_t(g.kx) * g.jp;
+  const int64_t full_fs = int64_t(g.nx) * g.ny * g.nzh;
+  cpx<T>* u = u_hat;
+  cpx<T>* us = static_cast<cpx<T>*>(b.us);
+  cpx<T>* acc = static_cast<cpx<T>*>(b.acc);
+  cpx<T>* bufA = static_cast<cpx<T>*>(b.bufA);
+  cpx<T>* bufB = static_cast<cpx<T>*>(b.bufB);
+  if (pruned) {
+    u = static_cast<cpx<T>*>(b.u_c);
+    H3_HIP(h3::launch_gather<T>(u_hat, full_fs, u, st_fs, 3, g, s));
+  }
+  const bool sums = needs_sums(f);
+  if (sums) {
+    int rc = reduce_state<T>(gd, g, u, f, b.partials, b.scalars, s);
+    if (rc) return rc;
+  }
+  mark(tm, kOther);
+  h3::StageArgs<T> a = stage_args<T>(gd, g, f, nu, dt, bufA, b.scalars);
+  a.partials = sums ? b.partials : nullptr;
+  a.acc = acc;
+  a.u = u;
+  const FinalizeArgs fa = finalize_args(gd, f);
+  for (int64_t n = 0; n < nsteps; ++n) {
+    for (int stage = 1; stage <= 4; ++stage) {
+      const cpx<T>* us_in = stage == 1 ? u : us;
+      int rc = rhs_passes<T>(gd, g, us_in, bufA, bufB, s, tm);
+      if (rc) return rc;
+      a.us = us_in;
+      a.out = stage == 4 ? u : us;
+      int nb = 0;
+      H3_HIP(h3::launch_x_forward_stage<T>(g.nx, stage, a, s, &nb));
+      mark(tm, kP5);
+      if (sums) {
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, b.partials, nb, fa, b.scalars);
+        H3_HIP(hipGetLastError());
+        mark(tm, kFinalize);
+      }
+    }
+  }
+  if (diag) {
+    int rc = reduce_state<T>(gd, g, u, f, b.partials, b.scalars, s);
+    if (rc) return rc;
+    H3_HIP(hipMemcpyAsync(diag, b.scalars, 4 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
+  if (pruned) H3_HIP(h3::launch_scatter<T>(u, st_fs, u_hat, full_fs, 3, g, s));
+  return HIT3D_OK;
+}
+
+template <class T>
+int to_physical_impl(const hit3d_grid& gd, const cpx<T>* in, T* out, int nf, cpx<T>* work,
+                     hipStream_t s) {
+  const Geo g = make_geo(gd, false);
+  const int64_t fs = int64_t(g.nx) * g.ny * g.nzh;
+  h3::StridedArgs<T> x{};
+  x.in = in;
+  x.out = work;
+  x.in_fs = x.out_fs = fs;
+  x.in_slab = x.out_slab = 0;
+  x.B = g.ny * g.nzh;
+  x.J = x.B;
+  x.nf = nf;
+  x.in_k = x.out_k = g.nx;
+  x.in_p = x.out_p = (g.nx + 1) / 2;
+  H3_HIP(h3::launch_strided<T>(g.nx, +1, x, s));
+  h3::StridedArgs<T> y{};
+  y.in = work;
+  y.out = work;
+  y.in_fs = y.out_fs = fs;
+  y.in_slab = y.out_slab = int64_t(g.ny) * g.nzh;
+  y.B = g.nzh;
+  y.J = g.nx * g.nzh;
+  y.nf = nf;
+  y.in_k = y.out_k = g.ny;
+  y.in_p = y.out_p = (g.ny + 1) / 2;
+  H3_HIP(h3::launch_strided<T>(g.ny, +1, y, s));
+  const double n = double(gd.nx) * double(gd.ny) * double(gd.nz);
+  H3_HIP(h3::launch_z_c2r<T>(g.nz, work, fs, out, int64_t(g.nx) * g.ny * g.nz, nf,
+                             int64_t(g.nx) * g.ny, g.nzh, T(1.0 / n), s));
+  return HIT3D_OK;
+}
+
+template <class T>
+int to_spectral_impl(const hit3d_grid& gd, const T* in, cpx<T>* out, int nf, cpx<T>* work,
+                     hipStream_t s) {
+  const Geo g = make_geo(gd, false);
+  const int64_t fs = int64_t(g.nx) * g.ny * g.nzh;
+  H3_HIP(h3::launch_z_r2c<T>(g.nz, in, int64_t(g.nx) * g.ny * g.nz, work, fs, nf,
+                             int64_t(g.nx) * g.ny, g.nzh, s));
+  h3::StridedArgs<T> y{};
+  y.in = work;
+  y.out = work;
+  y.in_fs = y.out_fs = fs;
+  y.in_slab = y.out_slab = int64_t(g.ny) * g.nzh;
+  y.B = g.nzh;
+  y.J = g.nx * g.nzh;
+  y.nf = nf;
+  y.in_k = y.out_k = g.ny;
+  y.in_p = y.out_p = (g.ny + 1) / 2;
+  H3_HIP(h3::launch_strided<T>(g.ny, -1, y, s));
+  h3::StridedArgs<T> x{};
+  x.in = work;
+  x.out = out;
+  x.in_fs = x.out_fs = fs;
+  x.in_slab = x.out_slab = 0;
+  x.B = g.ny * g.nzh;
+  x.J = x.B;
+  x.nf = nf;
+  x.in_k = x.out_k = g.nx;
+  x.in_p = x.out_p = (g.nx + 1) / 2;
+  H3_HIP(h3::launch_strided<T>(g.nx, -1, x, s));
+  return HIT3D_OK;
+}
+
+int check_work(void* work, size_t have, size_t need) {
+  if (need > have)
+    return fail(HIT3D_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) +
+                                         " bytes, got " + std::to_string(have));
+  if (need > 0 && !work) return fail(HIT3D_ERR_INVALID, "workspace is NULL");
+  if (reinterpret_cast<uintptr_t>(work) % 256 != 0)
+    return fail(HIT3D_ERR_INVALID, "workspace must be 256-byte aligned");
+  return HIT3D_OK;
+}
+
+int check_ptr(const void* p, const char* name) {
+  if (!p) return fail(HIT3D_ERR_INVALID, std::string(name) + " is NULL");
+  if (reinterpret_cast<uintptr_t>(p) % 16 != 0)
+    return fail(HIT3D_ERR_INVALID, std::string(name) + " must be 16-byte aligned");
+  return HIT3D_OK;
+}
+
+#define H3_TRY(expr)       \
+  do {                     \
+    int rc_ = (expr);      \
+    if (rc_) return rc_;   \
+  } while (0)
+
+}  // namespace
+
+extern "C" {
+
+int hit3d_last_error(char* buf, size_t len) {
+  if (buf && len) {
+    const size_t n = std::min(len - 1, g_err.size());
+    std::memcpy(buf, g_err.data(), n);
+    buf[n] = '\0';
+  }
+  return int(g_err.size());
+}
+
+int hit3d_version(void) { return 1; }
+
+int hit3d_workspace_size(const hit3d_grid* grid, int32_t op, int64_t ncomp, int32_t dealiased,
+                         size_t* bytes) {
+  H3_TRY(check_grid(grid));
+  if (!bytes) return fail(HIT3D_ERR_INVALID, "bytes is NULL");
+  switch (op) {
+    case HIT3D_OP_TO_PHYSICAL:
+    case HIT3D_OP_TO_SPECTRAL:
+      if (ncomp < 1) return fail(HIT3D_ERR_INVALID, "ncomp must be >= 1");
+      *bytes = transform_bytes(*grid, ncomp);
+      return HIT3D_OK;
+    case HIT3D_OP_RHS:
+    case HIT3D_OP_RK4: {
+      const bool pruned = dealiased != 0;
+      const Geo g = make_geo(*grid, pruned);
+      carve_step(*grid, g, pruned, op == HIT3D_OP_RK4, nullptr, bytes);
+      return HIT3D_OK;
+    }
+    case HIT3D_OP_SPECTRUM:
+      *bytes = spectrum_bytes(*grid, nullptr, nullptr);
+      return HIT3D_OK;
+    case HIT3D_OP_REDUCE:
+      *bytes = reduce_bytes(*grid, nullptr, nullptr, nullptr);
+      return HIT3D_OK;
+    default:
+      return fail(HIT3D_ERR_INVALID, "unknown op " + std::to_string(op));
+  }
+}
+
+int hit3d_shell_bins(const hit3d_grid* grid, int64_t* nbins) {
+  H3_TRY(check_grid(grid));
+  if (!nbins) return fail(HIT3D_ERR_INVALID, "nbins is NULL");
+  *nbins = shell_bins(*grid);
+  return HIT3D_OK;
+}
+
+int hit3d_grid_tables(const hit3d_grid* grid, void* k2, void* inv_k2, uint8_t* mask,
+                      int32_t* shell, void* stream) {
+  H3_TRY(check_grid(grid));
+  const Geo g = make_geo(*grid, false);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (grid->precision == 4)
+    H3_HIP(h3::launch_tables<float>(g, static_cast<float*>(k2), static_cast<float*>(inv_k2), mask,
+                                    shell, s));
+  else
+    H3_HIP(h3::launch_tables<double>(g, static_cast<double*>(k2), static_cast<double*>(inv_k2),
+                                     mask, shell, s));
+  return HIT3D_OK;
+}
+
+int hit3d_to_physical(const hit3d_grid* grid, const void* u_hat, void* u, int64_t ncomp,
+                      void* work, size_t work_bytes, void* stream) {
+  H3_TRY(check_grid(grid));
+  H3_TRY(check_ptr(u_hat, "u_hat"));
+  H3_TRY(check_ptr(u, "u"));
+  if (ncomp < 1 || ncomp > 64) return fail(HIT3D_ERR_INVALID, "ncomp must be in [1, 64]");
+  H3_TRY(check_work(work, work_bytes, transform_bytes(*grid, ncomp)));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (grid->precision == 4)
+    return to_physical_impl<float>(*grid, static_cast<const cpx<float>*>(u_hat),
+                                   static_cast<float*>(u), int(ncomp),
+                                   static_cast<cpx<float>*>(work), s);
+  return to_physical_impl<double>(*grid, static_cast<const cpx<double>*>(u_hat),
+                                  static_cast<double*>(u), int(ncomp),
+                                  static_cast<cpx<double>*>(work), s);
+}
+
+int hit3d_to_spectral(const hit3d_grid* grid, const void* u, void* u_hat, int64_t ncomp,
+                      void* work, size_t work_bytes, void* stream) {
+  H3_TRY(check_grid(grid));
+  H3_TRY(check_ptr(u, "u"));
+  H3_TRY(check_ptr(u_hat, "u_hat"));
+  if (ncomp < 1 || ncomp > 64) return fail(HIT3D_ERR_INVALID, "ncomp must be in [1, 64]");
+  H3_TRY(check_work(work, work_bytes, transform_bytes(*grid, ncomp)));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (grid->precision == 4)
+    return to_spectral_impl<float>(*grid, static_cast<const float*>(u),
+                                   static_cast<cpx<float>*>(u_hat), int(ncomp),
+                                   static_cast<cpx<float>*>(work), s);
+  return to_spectral_impl<double>(*grid, static_cast<const double*>(u),
+                                  static_cast<cpx<double>*>(u_hat), int(ncomp),
+                                  static_cast<cpx<double>*>(work), s);
+}
+
+int hit3d_dealias_project(const hit3d_grid* grid, const void* in, void* out, int64_t ncomp,
+                          int32_t do_dealias, int32_t do_project, void* stream) {
+  H3_TRY(check_grid(grid));
+  H3_TRY(check_ptr(in, "in"));
+  H3_TRY(check_ptr(out, "out"));
+  if (ncomp < 1 || ncomp > 64) return fail(HIT3D_ERR_INVALID, "ncomp must be in [1, 64]");
+  if (do_project && ncomp != 3) return fail(HIT3D_ERR_INVALID, "project needs ncomp == 3");
+  const Geo g = make_geo(*grid, false);
+  const int64_t fs = int64_t(g.nx) * g.ny * g.nzh;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (grid->precision == 4)
+    H3_HIP(h3::launch_dealias_project<float>(static_cast<const cpx<float>*>(in),
+                                             static_cast<cpx<float>*>(out), fs, int(ncomp), g,
+                                             do_dealias, do_project, s));
+  else
+    H3_HIP(h3::launch_dealias_project<double>(static_cast<const cpx<double>*>(in),
+                                              static_cast<cpx<double>*>(out), fs, int(ncomp), g,
+                                              do_dealias, do_project, s));
+  return HIT3D_OK;
+}
+
+int hit3d_spectral_derivative(const hit3d_grid* grid, const void* in, void* out, int64_t ncomp,
+                              int32_t op, void* stream) {
+  H3_TRY(check_grid(grid));
+  H3_TRY(check_ptr(in, "in"));
+  H3_TRY(check_ptr(out, "out"));
+  if (ncomp < 1 || ncomp > 64) return fail(HIT3D_ERR_INVALID, "ncomp must be in [1, 64]");
+  if (op < 0 || op > 3) return fail(HIT3D_ERR_INVALID, "op must be 0..3");
+  const Geo g = make_geo(*grid, false);
+  const int64_t fs = int64_t(g.nx) * g.ny * g.nzh;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (grid->precision == 4)
+    H3_HIP(h3::launch_spectral_op<float>(static_cast<const cpx<float>*>(in),
+                                         static_cast<cpx<float>*>(out), fs, int(ncomp), g, op, s));
+  else
+    H3_HIP(h3::launch_spectral_op<double>(static_cast<const cpx<double>*>(in),
+                                          static_cast<cpx<double>*>(out), fs, int(ncomp), g, op,
+                                          s));
+  return HIT3D_OK;
+}
+
+int hit3d_rhs(const hit3d_grid* grid, const void* u_hat, void* du_hat, double nu,
+              const hit3d_forcing* forcing, double t, int32_t dealiased, void* work,
+              size_t work_bytes, void* stream) {
+  (void)t;  // no forcing of the reference depends on t
+  H3_TRY(check_grid(grid));
+  H3_TRY(check_forcing(forcing));
+  H3_TRY(check_ptr(u_hat, "u_hat"));
+  H3_TRY(check_ptr(du_hat, "du_hat"));
+  if (u_hat == du_hat) return fail(HIT3D_ERR_INVALID, "du_hat must not alias u_hat");
+  const bool pruned = dealiased != 0;
+  size_t need;
+  carve_step(*grid, make_geo(*grid, pruned), pruned, false, nullptr, &need);
+  H3_TRY(check_work(work, work_bytes, need));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (grid->precision == 4)
+    return rhs_impl<float>(*grid, static_cast<const cpx<float>*>(u_hat),
+                           static_cast<cpx<float>*>(du_hat), nu, forcing, pruned, work, s);
+  return rhs_impl<double>(*grid, static_cast<const cpx<double>*>(u_hat),
+                          static_cast<cpx<double>*>(du_hat), nu, forcing, pruned, work, s);
+}
+
+int hit3d_rk4_steps(const hit3d_grid* grid, void* u_hat, double nu, double dt, int64_t nsteps,
+                    double t0, const hit3d_forcing* forcing, int32_t dealiased, double* diag,
+                    void* work, size_t work_bytes, void* stream) {
+  (void)t0;
+  H3_TRY(check_grid(grid));
+  H3_TRY(check_forcing(forcing));
+  H3_TRY(check_ptr(u_hat, "u_hat"));
+  if (nsteps < 0) return fail(HIT3D_ERR_INVALID, "nsteps must be >= 0");
+  if (!std::isfinite(dt) || !std::isfinite(nu))
+    return fail(HIT3D_ERR_INVALID, "dt and nu must be finite");
+  const bool pruned = dealiased != 0;
+  size_t need;
+  carve_step(*grid, make_geo(*grid, pruned), pruned, true, nullptr, &need);
+  H3_TRY(check_work(work, work_bytes, need));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (grid->precision == 4)
+    return rk4_impl<float>(*grid, static_cast<cpx<float>*>(u_hat), nu, dt, nsteps, forcing,
+                           pruned, diag, work, s);
+  return rk4_impl<double>(*grid, static_cast<cpx<double>*>(u_hat), nu, dt, nsteps, forcing,
+                          pruned, diag, work, s);
+}
+
+int hit3d_shell_spectrum(const hit3d_grid* grid, const void* u_hat, int32_t component,
+                         double* out, void* work, size_t work_bytes, void* stream) {
+  H3_TRY(check_grid(grid));
+  H3_TRY(check_ptr(u_hat, "u_hat"));
+  H3_TRY(check_ptr(out, "out"));
+  if (component < -1 || component > 2) return fail(HIT3D_ERR_INVALID, "component must be -1..2");
+  H3_TRY(check_work(work, work_bytes, spectrum_bytes(*grid, nullptr, nullptr)));
+  double* partials;
+  spectrum_bytes(*grid, &partials, work);
+  const Geo g = make_geo(*grid, false);
+  const int64_t fs = int64_t(g.nx) * g.ny * g.nzh;
+  const int nbins = int(shell_bins(*grid));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int nb = 0;
+  if (grid->precision == 4)
+    H3_HIP(h3::launch_spectrum_partials<float>(static_cast<const cpx<float>*>(u_hat), fs,
+                                               component, g, nbins, partials, &nb, s));
+  else
+    H3_HIP(h3::launch_spectrum_partials<double>(static_cast<const cpx<double>*>(u_hat), fs,
+                                                component, g, nbins, partials, &nb, s));
+  hipLaunchKernelGGL(k_spectrum_final, dim3((nbins + 255) / 256), dim3(256), 0, s, partials, nb,
+                     nbins, 0.5 * ntot2_inv(*grid), out);
+  H3_HIP(hipGetLastError());
+  return HIT3D_OK;
+}
+
+int hit3d_energy_dissipation(const hit3d_grid* grid, const void* u_hat, double nu, double kmin,
+                             double kmax, double* out, void* work, size_t work_bytes,
+                             void* stream) {
+  H3_TRY(check_grid(grid));
+  H3_TRY(check_ptr(u_hat, "u_hat"));
+  H3_TRY(check_ptr(out, "out"));
+  H3_TRY(check_work(work, work_bytes, reduce_bytes(*grid, nullptr, nullptr, nullptr)));
+  double *partials, *scalars;
+  reduce_bytes(*grid, &partials, &scalars, work);
+  const Geo g = make_geo(*grid, false);
+  hit3d_forcing band{};
+  band.kind = HIT3D_FORCING_BAND;
+  band.kmin = kmin;
+  band.kmax = kmax;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (grid->precision == 4)
+    H3_TRY(reduce_state<float>(*grid, g, static_cast<const cpx<float>*>(u_hat), &band, partials,
+                               scalars, s));
+  else
+    H3_TRY(reduce_state<double>(*grid, g, static_cast<const cpx<double>*>(u_hat), &band, partials,
+                                scalars, s));
+  hipLaunchKernelGGL(k_energy_out, dim3(1), dim3(1), 0, s, scalars, nu, out);
+  H3_HIP(hipGetLastError());
+  return HIT3D_OK;
+}
+
+int hit3d_alias_energy(const hit3d_grid* grid, const void* u_hat, double* out, void* work,
+                       size_t work_bytes, void* stream) {
+  H3_TRY(check_grid(grid));
+  H3_TRY(check_ptr(u_hat, "u_hat"));
+  H3_TRY(check_ptr(out, "out"));
+  H3_TRY(check_work(work, work_bytes, reduce_bytes(*grid, nullptr, nullptr, nullptr)));
+  double *partials, *scalars;
+  reduce_bytes(*grid, &partials, &scalars, work);
+  const Geo g = make_geo(*grid, false);
+  const int64_t fs = int64_t(g.nx) * g.ny * g.nzh;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (grid->precision == 4)
+    H3_HIP(h3::launch_alias_partials<float>(static_cast<const cpx<float>*>(u_hat), fs, g, partials,
+                                            s));
+  else
+    H3_HIP(h3::launch_alias_partials<double>(static_cast<const cpx<double>*>(u_hat), fs, g,
+                                             partials, s));
+  FinalizeArgs fa{};
+  fa.kind = HIT3D_FORCING_NONE;
+  fa.inv_ntot2 = 1.0;
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, partials, h3::kReduceBlocks, fa,
+                     scalars);
+  H3_HIP(hipGetLastError());
+  H3_HIP(hipMemcpyAsync(out, scalars + 1, sizeof(double), hipMemcpyDeviceToDevice, s));
+  return HIT3D_OK;
+}
+
+int hit3d_uniform_noise(const hit3d_grid* grid, uint64_t seed, void* u, void* stream) {
+  H3_TRY(check_grid(grid));
+  H3_TRY(check_ptr(u, "u"));
+  const int64_t n = 3 * grid->nx * grid->ny * grid->nz;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (grid->precision == 4)
+    H3_HIP(h3::launch_philox_uniform<float>(static_cast<float*>(u), n, seed, s));
+  else
+    H3_HIP(h3::launch_philox_uniform<double>(static_cast<double*>(u), n, seed, s));
+  return HIT3D_OK;
+}
+
+int hit3d_shell_scale(const hit3d_grid* grid, void* u_hat, int64_t ncomp, const double* factors,
+                      int64_t nbins, void* stream) {
+  H3_TRY(check_grid(grid));
+  H3_TRY(check_ptr(u_hat, "u_hat"));
+  if (!factors) return fail(HIT3D_ERR_INVALID, "factors is NULL");
+  if (ncomp < 1 || ncomp > 64) return fail(HIT3D_ERR_INVALID, "ncomp must be in [1, 64]");
+  if (nbins < 1 || nbins > shell_bins(*grid))
+    return fail(HIT3D_ERR_INVALID, "nbins must be in [1, hit3d_shell_bins]");
+  const Geo g = make_geo(*grid, false);
+  const int64_t fs = int64_t(g.nx) * g.ny * g.nzh;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (grid->precision == 4)
+    H3_HIP(h3::launch_shell_scale<float>(static_cast<cpx<float>*>(u_hat), fs, int(ncomp), g,
+                                         factors, int(nbins), s));
+  else
+    H3_HIP(h3::launch_shell_scale<double>(static_cast<cpx<double>*>(u_hat), fs, int(ncomp), g,
+                                          factors, int(nbins), s));
+  return HIT3D_OK;
+}
+
+int hit3d_profile_rk4(const hit3d_grid* grid, void* u_hat, double nu, double dt, int64_t nsteps,
+                      const hit3d_forcing* forcing, int32_t dealiased, double* ms, int64_t* counts,
+                      void* work, size_t work_bytes, void* stream) {
+  H3_TRY(check_grid(grid));
+  H3_TRY(check_forcing(forcing));
+  H3_TRY(check_ptr(u_hat, "u_hat"));
+  if (!ms || !counts) return fail(HIT3D_ERR_INVALID, "ms and counts must not be NULL");
+  if (nsteps < 1) return fail(HIT3D_ERR_INVALID, "nsteps must be >= 1");
+  const bool pruned = dealiased != 0;
+  size_t need;
+  carve_step(*grid, make_geo(*grid, pruned), pruned, true, nullptr, &need);
+  H3_TRY(check_work(work, work_bytes, need));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  PassTimer tm;
+  tm.s = s;
+  tm.record(-1);
+  int rc = grid->precision == 4
+               ? rk4_impl<float>(*grid, static_cast<cpx<float>*>(u_hat), nu, dt, nsteps, forcing,
+                                 pruned, nullptr, work, s, &tm)
+               : rk4_impl<double>(*grid, static_cast<cpx<double>*>(u_hat), nu, dt, nsteps,
+                                  forcing, pruned, nullptr, work, s, &tm);
+  if (rc) return rc;
+  if (!tm.ok) return fail(HIT3D_ERR_HIP, "event creation/record failed");
+  H3_HIP(hipEventSynchronize(tm.ev.back()));
+  for (int i = 0; i < kNumPass; ++i) {
+    ms[i] = 0.0;
+    counts[i] = 0;
+  }
+  for (size_t i = 1; i < tm.ev.size(); ++i) {
+    float t = 0.f;
+    H3_HIP(hipEventElapsedTime(&t, tm.ev[i - 1], tm.ev[i]));
+    ms[tm.id[i]] += t;
+    counts[tm.id[i]] += 1;
+  }
+  return HIT3D_OK;
+}
+
+}  // extern "C"

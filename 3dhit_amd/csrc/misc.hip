@@ -1,0 +1,429 @@
+// Non-FFT kernels. Full-layout kernels take a dense Geo (kx = nx, ky = ny,
+// kz = nz/2+1, jp = ny*kz), under which the compact layout is the API layout.
+#include "misc.hpp"
+
+namespace h3 {
+
+constexpr int kBlock = 256;
+
+static inline unsigned nblocks_for(int64_t n, int per = kBlock) {
+  return unsigned((n + per - 1) / per);
+}
+
+// Deterministic block sum of three doubles (blockDim == 256).
+__device__ __forceinline__ void block_sum3m(double a, double b, double c, double* dst) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off);
+    b += __shfl_xor(b, off);
+    c += __shfl_xor(c, off);
+  }
+  __shared__ double sh[3][kBlock / 64];
+  const int w = threadIdx.x / 64;
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][w] = a;
+    sh[1][w] = b;
+    sh[2][w] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < 3; ++i) dst[i] = ((sh[i][0] + sh[i][1]) + sh[i][2]) + sh[i][3];
+  }
+}
+
+// ---- full <-> compact ------------------------------------------------------
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_gather(const cpx<T>* __restrict__ full, int64_t full_fs,
+                                                   cpx<T>* __restrict__ comp, int64_t comp_fs,
+                                                   int nf, Geo g) {
+  const int64_t J = int64_t(g.ky) * g.kz;
+  const int64_t n = int64_t(g.kx) * J;
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= n) return;
+  const int ix = int(idx / J);
+  const int j = int(idx - int64_t(ix) * J);
+  const int iy = j / g.kz, iz = j - iy * g.kz;
+  const int x = row_of_compact(ix, g.nx, g.kx, g.px), y = row_of_compact(iy, g.ny, g.ky, g.py);
+  const int64_t src = (int64_t(x) * g.ny + y) * g.nzh + iz;
+  const int64_t dst = int64_t(ix) * g.jp + j;
+  for (int f = 0; f < nf; ++f) comp[f * comp_fs + dst] = full[f * full_fs + src];
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_scatter(const cpx<T>* __restrict__ comp, int64_t comp_fs,
+                                                    cpx<T>* __restrict__ full, int64_t full_fs,
+                                                    int nf, Geo g) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= n) return;
+  const int iz = int(idx % g.nzh);
+  const int64_t xy = idx / g.nzh;
+  const int y = int(xy % g.ny), x = int(xy / g.ny);
+  const int ix = compact_of_row(x, g.nx, g.kx, g.px), iy = compact_of_row(y, g.ny, g.ky, g.py);
+  const bool kept = ix >= 0 && iy >= 0 && iz < g.kz;
+  const int64_t src = int64_t(ix) * g.jp + int64_t(iy) * g.kz + iz;
+  for (int f = 0; f < nf; ++f) full[f * full_fs + idx] = kept ? comp[f * comp_fs + src] : czero<T>();
+}
+
+// ---- FP64 quadratic sums ---------------------------------------------------
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_quad_partials(const cpx<T>* __restrict__ st, int64_t fs,
+                                                          Geo g, T kmin2, T kmax2,
+                                                          double* __restrict__ partials) {
+  const int64_t J = int64_t(g.ky) * g.kz;
+  const int64_t n = int64_t(g.kx) * J;
+  double sE = 0.0, sO = 0.0, sB = 0.0;
+  for (int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x; idx < n;
+       idx += int64_t(gridDim.x) * kBlock) {
+    const int ix = int(idx / J);
+    const int j = int(idx - int64_t(ix) * J);
+    const int iy = j / g.kz, iz = j - iy * g.kz;
+    const int64_t o = int64_t(ix) * g.jp + j;
+    double e = 0.0;
+    for (int f = 0; f < 3; ++f) {
+      const cpx<T> v = st[f * fs + o];
+      e += double(v.x) * v.x + double(v.y) * v.y;
+    }
+    const T k2 = k2_of(wavenumber<T>(g.k0x, freq_of(ix, g.kx, g.px)),
+                       wavenumber<T>(g.k0y, freq_of(iy, g.ky, g.py)), wavenumber<T>(g.k0z, iz));
+    const double we = hermitian_weight(iz, g.nz) * e;
+    sE += we;
+    sO += we * double(k2);
+    if (kmin2 <= k2 && k2 <= kmax2) sB += we;
+  }
+  block_sum3m(sE, sO, sB, partials + 3 * blockIdx.x);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_alias_partials(const cpx<T>* __restrict__ full,
+                                                           int64_t fs, Geo g,
+                                                           double* __restrict__ partials) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  double s = 0.0;
+  for (int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x; idx < n;
+       idx += int64_t(gridDim.x) * kBlock) {
+    const int iz = int(idx % g.nzh);
+    const int64_t xy = idx / g.nzh;
+    const int y = int(xy % g.ny), x = int(xy / g.ny);
+    const int nx = x < (g.nx + 1) / 2 ? x : x - g.nx;
+    const int ny = y < (g.ny + 1) / 2 ? y : y - g.ny;
+    if (abs(nx) <= g.cx && abs(ny) <= g.cy && iz <= g.cz) continue;
+    for (int f = 0; f < 3; ++f) {
+      const cpx<T> v = full[f * fs + idx];
+      s += double(v.x) * v.x + double(v.y) * v.y;
+    }
+  }
+  block_sum3m(s, 0.0, 0.0, partials + 3 * blockIdx.x);
+}
+
+// ---- shell spectrum ----------------------------------------------------------
+// Each wave walks whole kz lines. Along a line the shell index is
+// non-decreasing in kz, so a segmented suffix sum across the 64 lanes gives
+// each shell's run total in its head lane, which adds it to the wave's own
+// LDS histogram: one writer per bin at a time, in a fixed order, so the
+// result is bit-reproducible. Waves then merge in order into per-block rows.
+
+constexpr int kSpecWaves = kBlock / 64;
+
+int spectrum_blocks(const Geo& g) {
+  const int64_t lines = int64_t(g.kx) * g.ky;
+  const int64_t b = (lines + kSpecWaves - 1) / kSpecWaves;
+  return int(b < kReduceBlocks ? b : kReduceBlocks);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_spectrum_partials(const cpx<T>* __restrict__ st,
+                                                              int64_t fs, int component, Geo g,
+                                                              int nbins,
+                                                              double* __restrict__ partials) {
+  extern __shared__ double hist[];  // [kSpecWaves][nbins][2] (sum, compensation)
+  const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
+  for (int i = threadIdx.x; i < kSpecWaves * nbins * 2; i += kBlock) hist[i] = 0.0;
+  __syncthreads();
+  double* h = hist + wave * nbins * 2;
+  const int64_t lines = int64_t(g.kx) * g.ky;
+  const int f0 = component < 0 ? 0 : component, f1 = component < 0 ? 3 : component + 1;
+  for (int64_t l = int64_t(blockIdx.x) * kSpecWaves + wave; l < lines;
+       l += int64_t(gridDim.x) * kSpecWaves) {
+    const int ix = int(l / g.ky), iy = int(l - int64_t(ix) * g.ky);
+    const T kxv = wavenumber<T>(g.k0x, freq_of(ix, g.kx, g.px));
+    const T kyv = wavenumber<T>(g.k0y, freq_of(iy, g.ky, g.py));
+    const int64_t base = int64_t(ix) * g.jp + int64_t(iy) * g.kz;
+    for (int z0 = 0; z0 < g.kz; z0 += 64) {
+      const int iz = z0 + lane;
+      double v = 0.0;
+      int key = 0x7fffffff;
+      if (iz < g.kz) {
+        for (int f = f0; f < f1; ++f) {
+          const cpx<T> u = st[f * fs + base + iz];
+          v += double(u.x) * u.x + double(u.y) * u.y;
+        }
+        v *= hermitian_weight(iz, g.nz);
+        key = shell_of<T>(k2_of(kxv, kyv, wavenumber<T>(g.k0z, iz)), g.k0x);
+      }
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const double ov = __shfl_down(v, off);
+        const int ok = __shfl_down(key, off);
+        if (lane + off < 64 && ok == key) v += ov;
+      }
+      const int prev = __shfl_up(key, 1);
+      const bool head = lane == 0 || prev != key;
+      if (head && key < nbins) {
+        // Neumaier update of bin `key`
+        const double s = h[2 * key], t = s + v;
+        h[2 * key + 1] += fabs(s) >= fabs(v) ? (s - t) + v : (v - t) + s;
+        h[2 * key] = t;
+      }
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nbins; b += kBlock) {
+    KSum acc;
+    for (int w = 0; w < kSpecWaves; ++w) {
+      acc.add(hist[(w * nbins + b) * 2]);
+      acc.add(hist[(w * nbins + b) * 2 + 1]);
+    }
+    partials[int64_t(blockIdx.x) * nbins + b] = acc.value();
+  }
+}
+
+// ---- Philox4x64-10 uniform noise -------------------------------------------
+// value i = word i%4 of Philox4x64-10(counter = (i/4 + 1, 0, 0, 0), key = (seed, 0)),
+// mapped to U[-0.5, 0.5): (x >> 40) 2^-24 - 0.5 (float) or (x >> 11) 2^-53 - 0.5 (double).
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_philox_uniform(T* __restrict__ out, int64_t n,
+                                                           uint64_t seed) {
+  const int64_t b = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (4 * b >= n) return;
+  uint64_t c0 = uint64_t(b) + 1, c1 = 0, c2 = 0, c3 = 0, k0 = seed, k1 = 0;
+  constexpr uint64_t M0 = 0xD2E7470EE14C6C93ull, M1 = 0xCA5A826395121157ull;
+  constexpr uint64_t W0 = 0x9E3779B97F4A7C15ull, W1 = 0xBB67AE8584CAA73Bull;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t lo0 = M0 * c0, hi0 = __umul64hi(M0, c0);
+    const uint64_t lo1 = M1 * c2, hi1 = __umul64hi(M1, c2);
+    const uint64_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    k0 += W0;
+    k1 += W1;
+  }
+  const uint64_t words[4] = {c0, c1, c2, c3};
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    const int64_t i = 4 * b + w;
+    if (i < n) {
+      if constexpr (sizeof(T) == 4) {
+        out[i] = float(uint32_t(words[w] >> 40)) * 0x1p-24f - 0.5f;
+      } else {
+        out[i] = double(words[w] >> 11) * 0x1p-53 - 0.5;
+      }
+    }
+  }
+}
+
+// ---- elementwise spectral operators (full layout) ----------------------------
+
+struct ModeIdx {
+  int nx, ny, nz;
+};
+__device__ __forceinline__ ModeIdx mode_of(int64_t idx, const Geo& g) {
+  const int iz = int(idx % g.nzh);
+  const int64_t xy = idx / g.nzh;
+  const int y = int(xy % g.ny), x = int(xy / g.ny);
+  return {x < (g.nx + 1) / 2 ? x : x - g.nx, y < (g.ny + 1) / 2 ? y : y - g.ny, iz};
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_dealias_project(const cpx<T>* in, cpx<T>* out,
+                                                            int64_t fs, int nf, Geo g,
+                                                            int do_dealias, int do_project) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= n) return;
+  const ModeIdx m = mode_of(idx, g);
+  const bool keep = !do_dealias || in_box(m.nx, m.ny, m.nz, g);
+  if (do_project) {  // nf == 3 (checked by the caller)
+    cpx<T> u0 = keep ? in[idx] : czero<T>(), u1 = keep ? in[fs + idx] : czero<T>(),
+           u2 = keep ? in[2 * fs + idx] : czero<T>();
+    const T kx = wavenumber<T>(g.k0x, m.nx), ky = wavenumber<T>(g.k0y, m.ny),
+            kz = wavenumber<T>(g.k0z, m.nz);
+    const T ik2 = inv_k2_of(k2_of(kx, ky, kz));
+    const cpx<T> div = ik2 * (kx * u0 + ky * u1 + kz * u2);
+    out[idx] = u0 - kx * div;
+    out[fs + idx] = u1 - ky * div;
+    out[2 * fs + idx] = u2 - kz * div;
+  } else {
+    for (int f = 0; f < nf; ++f) out[f * fs + idx] = keep ? in[f * fs + idx] : czero<T>();
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_shell_scale(cpx<T>* st, int64_t fs, int nf, Geo g,
+                                                        const double* __restrict__ c, int nbins) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= n) return;
+  const ModeIdx m = mode_of(idx, g);
+  const T k2 = k2_of(wavenumber<T>(g.k0x, m.nx), wavenumber<T>(g.k0y, m.ny),
+                     wavenumber<T>(g.k0z, m.nz));
+  const int s = shell_of<T>(k2, g.k0x);
+  const T f = s < nbins ? T(c[s]) : T(1);
+  for (int i = 0; i < nf; ++i) st[i * fs + idx] = f * st[i * fs + idx];
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_spectral_op(const cpx<T>* in, cpx<T>* out, int64_t fs,
+                                                        int nf, Geo g, int op) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= n) return;
+  const ModeIdx m = mode_of(idx, g);
+  const T kx = wavenumber<T>(g.k0x, m.nx), ky = wavenumber<T>(g.k0y, m.ny),
+          kz = wavenumber<T>(g.k0z, m.nz);
+  for (int f = 0; f < nf; ++f) {
+    const cpx<T> u = in[f * fs + idx];
+    cpx<T> r;
+    switch (op) {
+      case kOpDdx: r = mul_i(kx * u); break;
+      case kOpDdy: r = mul_i(ky * u); break;
+      case kOpDdz: r = mul_i(kz * u); break;
+      default: r = -k2_of(kx, ky, kz) * u; break;
+    }
+    out[f * fs + idx] = r;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_tables(Geo g, T* k2o, T* ik2o, uint8_t* masko,
+                                                   int32_t* shello) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= n) return;
+  const ModeIdx m = mode_of(idx, g);
+  const T k2 = k2_of(wavenumber<T>(g.k0x, m.nx), wavenumber<T>(g.k0y, m.ny),
+                     wavenumber<T>(g.k0z, m.nz));
+  if (k2o) k2o[idx] = k2;
+  if (ik2o) ik2o[idx] = inv_k2_of(k2);
+  if (masko) masko[idx] = in_box(m.nx, m.ny, m.nz, g) ? 1 : 0;
+  if (shello) shello[idx] = shell_of<T>(k2, g.k0x);
+}
+
+// ---- launchers -------------------------------------------------------------
+
+template <class T>
+hipError_t launch_gather(const cpx<T>* full, int64_t full_fs, cpx<T>* comp, int64_t comp_fs,
+                         int nf, const Geo& g, hipStream_t s) {
+  const int64_t n = int64_t(g.kx) * g.ky * g.kz;
+  hipLaunchKernelGGL((k_gather<T>), dim3(nblocks_for(n)), dim3(kBlock), 0, s, full, full_fs, comp,
+                     comp_fs, nf, g);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_scatter(const cpx<T>* comp, int64_t comp_fs, cpx<T>* full, int64_t full_fs,
+                          int nf, const Geo& g, hipStream_t s) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  hipLaunchKernelGGL((k_scatter<T>), dim3(nblocks_for(n)), dim3(kBlock), 0, s, comp, comp_fs, full,
+                     full_fs, nf, g);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_quad_partials(const cpx<T>* st, int64_t fs, const Geo& g, T kmin2, T kmax2,
+                                double* partials, hipStream_t s) {
+  hipLaunchKernelGGL((k_quad_partials<T>), dim3(kReduceBlocks), dim3(kBlock), 0, s, st, fs, g,
+                     kmin2, kmax2, partials);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_alias_partials(const cpx<T>* full, int64_t fs, const Geo& g, double* partials,
+                                 hipStream_t s) {
+  hipLaunchKernelGGL((k_alias_partials<T>), dim3(kReduceBlocks), dim3(kBlock), 0, s, full, fs, g,
+                     partials);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_spectrum_partials(const cpx<T>* st, int64_t fs, int component, const Geo& g,
+                                    int nbins, double* partials, int* nblocks, hipStream_t s) {
+  const int nb = spectrum_blocks(g);
+  *nblocks = nb;
+  const size_t shmem = size_t(kSpecWaves) * nbins * 2 * sizeof(double);
+  hipLaunchKernelGGL((k_spectrum_partials<T>), dim3(nb), dim3(kBlock), shmem, s, st, fs, component,
+                     g, nbins, partials);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_philox_uniform(T* out, int64_t n, uint64_t seed, hipStream_t s) {
+  const int64_t blocks4 = (n + 3) / 4;
+  hipLaunchKernelGGL((k_philox_uniform<T>), dim3(nblocks_for(blocks4)), dim3(kBlock), 0, s, out, n,
+                     seed);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_dealias_project(const cpx<T>* in, cpx<T>* out, int64_t fs, int nf,
+                                  const Geo& g, int do_dealias, int do_project, hipStream_t s) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  hipLaunchKernelGGL((k_dealias_project<T>), dim3(nblocks_for(n)), dim3(kBlock), 0, s, in, out, fs,
+                     nf, g, do_dealias, do_project);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_shell_scale(cpx<T>* st, int64_t fs, int nf, const Geo& g, const double* c,
+                              int nbins, hipStream_t s) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  hipLaunchKernelGGL((k_shell_scale<T>), dim3(nblocks_for(n)), dim3(kBlock), 0, s, st, fs, nf, g,
+                     c, nbins);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_spectral_op(const cpx<T>* in, cpx<T>* out, int64_t fs, int nf, const Geo& g,
+                              int op, hipStream_t s) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  hipLaunchKernelGGL((k_spectral_op<T>), dim3(nblocks_for(n)), dim3(kBlock), 0, s, in, out, fs, nf,
+                     g, op);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_tables(const Geo& g, T* k2, T* inv_k2, uint8_t* mask, int32_t* shell,
+                         hipStream_t s) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  hipLaunchKernelGGL((k_tables<T>), dim3(nblocks_for(n)), dim3(kBlock), 0, s, g, k2, inv_k2, mask,
+                     shell);
+  return hipGetLastError();
+}
+
+#define H3_INSTANTIATE_MISC(T)                                                                     \
+  template hipError_t launch_gather<T>(const cpx<T>*, int64_t, cpx<T>*, int64_t, int, const Geo&, \
+                                       hipStream_t);                                               \
+  template hipError_t launch_scatter<T>(const cpx<T>*, int64_t, cpx<T>*, int64_t, int,            \
+                                        const Geo&, hipStream_t);                                  \
+  template hipError_t launch_quad_partials<T>(const cpx<T>*, int64_t, const Geo&, T, T, double*,  \
+                                              hipStream_t);                                        \
+  template hipError_t launch_alias_partials<T>(const cpx<T>*, int64_t, const Geo&, double*,       \
+                                               hipStream_t);                                       \
+  template hipError_t launch_spectrum_partials<T>(const cpx<T>*, int64_t, int, const Geo&, int,   \
+                                                  double*, int*, hipStream_t);                     \
+  template hipError_t launch_philox_uniform<T>(T*, int64_t, uint64_t, hipStream_t);               \
+  template hipError_t launch_dealias_project<T>(const cpx<T>*, cpx<T>*, int64_t, int, const Geo&, \
+                                                int, int, hipStream_t);                            \
+  template hipError_t launch_shell_scale<T>(cpx<T>*, int64_t, int, const Geo&, const double*, int, \
+                                            hipStream_t);                                          \
+  template hipError_t launch_spectral_op<T>(const cpx<T>*, cpx<T>*, int64_t, int, const Geo&, int, \
+                                            hipStream_t);                                          \
+  template hipError_t launch_tables<T>(const Geo&, T*, T*, uint8_t*, int32_t*, hipStream_t);
+
+H3_INSTANTIATE_MISC(float)
+H3_INSTANTIATE_MISC(double)
+
+}  // namespace h3

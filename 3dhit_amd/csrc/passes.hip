@@ -1,0 +1,572 @@
+// Pencil-pass kernels of the pseudo-spectral RK4 step (gfx950).
+//
+// One RHS evaluation = five passes over HBM (navierstokes.py:69-95):
+//   P1 x-inverse of u and of the x-dependent curl parts      [3][Kx][Jp] -> [5][Nx][Jp]
+//   P2 y-inverse, forming w0 = i(ky u2 - kz u1) on the fly   [5][Nx][Jp] -> [6][Nx][Ny][Kz]
+//   P3 z c2r of (u_j + i w_j), u x w, z r2c                  [6][..][Kz] -> [3][..][Kz] in place
+//   P4 y-forward                                             [3][Nx][Ny][Kz] -> [3][Nx][Jp]
+//   P5 x-forward + forcing + dealias + projection + viscous + RK4 stage, FP64 sums
+// The standalone to_physical / to_spectral use the plain strided pass and the
+// z c2r / r2c kernels.
+#include "passes.hpp"
+
+namespace h3 {
+
+// ---------------------------------------------------------------------------
+// Plain strided pass (any axis, either direction)
+// ---------------------------------------------------------------------------
+template <int N, int DIR, class T>
+__global__ __launch_bounds__(kThreads) void k_strided(StridedArgs<T> a) {
+  using S = StridedShape<N>;
+  constexpr int R = S::R, Q = S::Q, TC = S::TC;
+  __shared__ cpx<T> lds[S::LDS];
+  const int c = threadIdx.x % TC, q = threadIdx.x / TC;
+  const int j = blockIdx.x * TC + c;
+  const bool active = j < a.J;
+  const int jj = active ? j : 0;
+  const int ao = jj / a.B, bo = jj - ao * a.B;
+  const cpx<T>* in0 = a.in + ao * a.in_slab + bo;
+  cpx<T>* out0 = a.out + ao * a.out_slab + bo;
+  LdsStrided<N, TC, T> L{lds + c};
+  int irow[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) irow[r] = active ? compact_of_row(q + Q * r, N, a.in_k, a.in_p) : -1;
+  for (int f = 0; f < a.nf; ++f) {
+    cpx<T> v[R];
+    const cpx<T>* src = in0 + f * a.in_fs;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = irow[r] >= 0 ? src[int64_t(irow[r]) * a.B] : czero<T>();
+    pencil_fft<N, DIR>(v, q, L);
+    cpx<T>* dst = out0 + f * a.out_fs;
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      const int o = compact_of_row(out_index<N>(q, s), N, a.out_k, a.out_p);
+      if (active && o >= 0) dst[int64_t(o) * a.B] = v[s];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// P1: x-inverse of u0, u1, u2 and of the x-dependent vorticity parts
+//   w1 = i(kz u0 - kx u2), w2 = i(kx u1 - ky u0)   (navierstokes.py:73-77)
+// out fields: [u0, u1, u2, w1, w2]
+// ---------------------------------------------------------------------------
+template <int N, class T>
+__global__ __launch_bounds__(kThreads) void k_x_inverse_curl(const cpx<T>* __restrict__ us,
+                                                            int64_t us_fs,
+                                                            cpx<T>* __restrict__ out,
+                                                            int64_t out_fs, Geo g) {
+  using S = StridedShape<N>;
+  constexpr int R = S::R, Q = S::Q, TC = S::TC;
+  __shared__ cpx<T> lds[S::LDS];
+  const int c = threadIdx.x % TC, q = threadIdx.x / TC;
+  const int j = blockIdx.x * TC + c;
+  const bool active = j < g.ky * g.kz;
+  const int jj = active ? j : 0;
+  const int iy = jj / g.kz, iz = jj - iy * g.kz;
+  const T kyv = wavenumber<T>(g.k0y, freq_of(iy, g.ky, g.py));
+  const T kzv = wavenumber<T>(g.k0z, iz);
+  LdsStrided<N, TC, T> L{lds + c};
+  int irow[R];
+  T kxv[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    irow[r] = active ? compact_of_row(q + Q * r, N, g.kx, g.px) : -1;
+    kxv[r] = wavenumber<T>(g.k0x, irow[r] >= 0 ? freq_of(irow[r], g.kx, g.px) : 0);
+  }
+  const cpx<T>* u0 = us + jj;
+  const cpx<T>* u1 = u0 + us_fs;
+  const cpx<T>* u2 = u1 + us_fs;
+  auto load = [&](const cpx<T>* p, cpx<T>* v) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = irow[r] >= 0 ? p[int64_t(irow[r]) * g.jp] : czero<T>();
+  };
+  auto store = [&](int f, const cpx<T>* v) {
+    if (!active) return;
+    cpx<T>* dst = out + f * out_fs + j;
+#pragma unroll
+    for (int s = 0; s < R; ++s) dst[int64_t(out_index<N>(q, s)) * g.jp] = v[s];
+  };
+  cpx<T> a[R], b[R], w[R];
+  load(u0, a);
+  load(u2, b);
+#pragma unroll
+  for (int r = 0; r < R; ++r) w[r] = mul_i(kzv * a[r] - kxv[r] * b[r]);
+  pencil_fft<N, +1>(w, q, L);
+  store(3, w);
+  pencil_fft<N, +1>(b, q, L);
+  store(2, b);
+  load(u1, b);
+#pragma unroll
+  for (int r = 0; r < R; ++r) w[r] = mul_i(kxv[r] * b[r] - kyv * a[r]);
+  pencil_fft<N, +1>(w, q, L);
+  store(4, w);
+  pencil_fft<N, +1>(b, q, L);
+  store(1, b);
+  pencil_fft<N, +1>(a, q, L);
+  store(0, a);
+}
+
+// ---------------------------------------------------------------------------
+// P2: y-inverse; forms w0 = i(ky u2 - kz u1) before its transform.
+// in  [5][Nx][jp] fields (u0, u1, u2, w1, w2), rows iy (stride Kz)
+// out [6][Nx][Ny][Kz] fields (u0, u1, u2, w0, w1, w2)
+// ---------------------------------------------------------------------------
+template <int N, class T>
+__global__ __launch_bounds__(kThreads) void k_y_inverse_curl(const cpx<T>* __restrict__ in,
+                                                            int64_t in_fs,
+                                                            cpx<T>* __restrict__ out,
+                                                            int64_t out_fs, Geo g) {
+  using S = StridedShape<N>;
+  constexpr int R = S::R, Q = S::Q, TC = S::TC;
+  __shared__ cpx<T> lds[S::LDS];
+  const int c = threadIdx.x % TC, q = threadIdx.x / TC;
+  const int j = blockIdx.x * TC + c;
+  const bool active = j < g.nx * g.kz;
+  const int jj = active ? j : 0;
+  const int x = jj / g.kz, iz = jj - x * g.kz;
+  const T kzv = wavenumber<T>(g.k0z, iz);
+  LdsStrided<N, TC, T> L{lds + c};
+  int irow[R];
+  T kyv[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    irow[r] = active ? compact_of_row(q + Q * r, N, g.ky, g.py) : -1;
+    kyv[r] = wavenumber<T>(g.k0y, irow[r] >= 0 ? freq_of(irow[r], g.ky, g.py) : 0);
+  }
+  const cpx<T>* src = in + int64_t(x) * g.jp + iz;
+  cpx<T>* dst0 = out + int64_t(x) * g.ny * g.kz + iz;
+  auto load = [&](int f, cpx<T>* v) {
+    const cpx<T>* p = src + f * in_fs;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = irow[r] >= 0 ? p[irow[r] * g.kz] : czero<T>();
+  };
+  auto store = [&](int f, const cpx<T>* v) {
+    if (!active) return;
+    cpx<T>* p = dst0 + f * out_fs;
+#pragma unroll
+    for (int s = 0; s < R; ++s) p[out_index<N>(q, s) * g.kz] = v[s];
+  };
+  cpx<T> a[R], b[R], w[R];
+  load(1, a);
+  load(2, b);
+#pragma unroll
+  for (int r = 0; r < R; ++r) w[r] = mul_i(kyv[r] * b[r] - kzv * a[r]);
+  pencil_fft<N, +1>(w, q, L);
+  store(3, w);
+  pencil_fft<N, +1>(b, q, L);
+  store(2, b);
+  pencil_fft<N, +1>(a, q, L);
+  store(1, a);
+  load(0, a);
+  pencil_fft<N, +1>(a, q, L);
+  store(0, a);
+  load(3, a);
+  pencil_fft<N, +1>(a, q, L);
+  store(4, a);
+  load(4, a);
+  pencil_fft<N, +1>(a, q, L);
+  store(5, a);
+}
+
+// ---------------------------------------------------------------------------
+// z-axis helpers: Hermitian extension of two half spectra U, W into the
+// spectrum of u + i w (c2r semantics: only Re of the kz = 0 and Nyquist bins).
+// ---------------------------------------------------------------------------
+template <int N, class T>
+__device__ __forceinline__ cpx<T> herm_pair(const cpx<T>* U, const cpx<T>* W, int k, int kz) {
+  const bool mirror = 2 * k > N;
+  const int src = mirror ? N - k : k;
+  if (src >= kz) return czero<T>();
+  cpx<T> u = U[src];
+  cpx<T> w = W ? W[src] : czero<T>();
+  if (src == 0 || 2 * src == N) {
+    u.y = T(0);
+    w.y = T(0);
+  }
+  // u + i w, or conj(u) + i conj(w) on the mirrored half
+  return mirror ? cpx<T>{u.x + w.y, w.x - u.y} : cpx<T>{u.x - w.y, u.y + w.x};
+}
+
+// Separate A = FFT(a + i b) into FFT(a)[k], FFT(b)[k] using A[N-k].
+template <class T>
+__device__ __forceinline__ void split_pair(cpx<T> ak, cpx<T> bk, cpx<T>& X, cpx<T>& Y) {
+  const T h = T(0.5);
+  X = {h * (ak.x + bk.x), h * (ak.y - bk.y)};
+  Y = {h * (ak.y + bk.y), h * (bk.x - ak.x)};
+}
+
+// ---------------------------------------------------------------------------
+// P3: per z pencil, c2r of (u_j + i w_j) for j = 0..2, u x w scaled by
+// `scale` (= 1/Ntot^2, a power of two), then r2c; output (nl0, nl1, nl2)
+// overwrites fields 0..2 of the same pencils.
+// ---------------------------------------------------------------------------
+template <int N, class T>
+__global__ __launch_bounds__(kThreads) void k_z_fused(cpx<T>* buf, int64_t fs, int64_t npencils,
+                                                     int kz, T scale) {
+  using B = PencilBlock<N>;
+  constexpr int R = B::R, Q = B::Q, PW = B::PW;
+  __shared__ cpx<T> lds[B::LDS];
+  const int q = threadIdx.x % Q, p = threadIdx.x / Q;
+  const int64_t pp = int64_t(blockIdx.x) * PW + p;
+  const bool active = pp < npencils;
+  const int64_t base = (active ? pp : 0) * kz;
+  cpx<T>* lp = lds + p * lds_pencil_elems<N>();
+  LdsPencil<N, T> L{lp};
+  const int kzl = active ? kz : 0;  // inactive pencils load nothing
+  cpx<T> z0[R], z1[R], z2[R];
+  cpx<T>* zz[3] = {z0, z1, z2};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      zz[c][r] = herm_pair<N, T>(buf + c * fs + base, buf + (3 + c) * fs + base, q + Q * r, kzl);
+    pencil_fft<N, +1>(zz[c], q, L);
+  }
+  // physical space: z_c = u_c + i w_c at position out_index(q, s)
+  cpx<T> A[R], Bv[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+    const T u0 = z0[s].x, u1 = z1[s].x, u2 = z2[s].x;
+    const T w0 = z0[s].y, w1 = z1[s].y, w2 = z2[s].y;
+    const int r = out_slot_as_input<N>(s);
+    A[r] = {(u1 * w2 - u2 * w1) * scale, (u2 * w0 - u0 * w2) * scale};
+    Bv[r] = {(u0 * w1 - u1 * w0) * scale, T(0)};
+  }
+  pencil_fft<N, -1>(Bv, q, L);
+  pencil_fft<N, -1>(A, q, L);
+  if (active) {
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      const int k = out_index<N>(q, s);
+      if (k < kz) buf[2 * fs + base + k] = Bv[s];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < R; ++s) lp[out_index<N>(q, s)] = A[s];
+  __syncthreads();
+  if (active) {
+    for (int k = q; k < kz; k += Q) {
+      cpx<T> X, Y;
+      split_pair(lp[k], lp[(N - k) & (N - 1)], X, Y);
+      buf[0 * fs + base + k] = X;
+      buf[1 * fs + base + k] = Y;
+    }
+  }
+}
+
+// Standalone z c2r of nf fields, two per complex transform.
+template <int N, class T>
+__global__ __launch_bounds__(kThreads) void k_z_c2r(const cpx<T>* __restrict__ in, int64_t in_fs,
+                                                   T* __restrict__ out, int64_t out_fs, int nf,
+                                                   int64_t npencils, int kz, T scale) {
+  using B = PencilBlock<N>;
+  constexpr int R = B::R, Q = B::Q, PW = B::PW;
+  __shared__ cpx<T> lds[B::LDS];
+  const int q = threadIdx.x % Q, p = threadIdx.x / Q;
+  const int64_t pp = int64_t(blockIdx.x) * PW + p;
+  const bool active = pp < npencils;
+  const int64_t ib = (active ? pp : 0) * kz, ob = (active ? pp : 0) * N;
+  LdsPencil<N, T> L{lds + p * lds_pencil_elems<N>()};
+  const int kzl = active ? kz : 0;
+  for (int f = 0; f < nf; f += 2) {
+    const bool two = f + 1 < nf;
+    cpx<T> z[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      z[r] = herm_pair<N, T>(in + f * in_fs + ib, two ? in + (f + 1) * in_fs + ib : nullptr,
+                             q + Q * r, kzl);
+    pencil_fft<N, +1>(z, q, L);
+    if (active) {
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        const int n = out_index<N>(q, s);
+        out[f * out_fs + ob + n] = z[s].x * scale;
+        if (two) out[(f + 1) * out_fs + ob + n] = z[s].y * scale;
+      }
+    }
+  }
+}
+
+// Standalone z r2c of nf real fields, two per complex transform.
+template <int N, class T>
+__global__ __launch_bounds__(kThreads) void k_z_r2c(const T* __restrict__ in, int64_t in_fs,
+                                                   cpx<T>* __restrict__ out, int64_t out_fs,
+                                                   int nf, int64_t npencils, int kz) {
+  using B = PencilBlock<N>;
+  constexpr int R = B::R, Q = B::Q, PW = B::PW;
+  __shared__ cpx<T> lds[B::LDS];
+  const int q = threadIdx.x % Q, p = threadIdx.x / Q;
+  const int64_t pp = int64_t(blockIdx.x) * PW + p;
+  const bool active = pp < npencils;
+  const int64_t ib = (active ? pp : 0) * N, ob = (active ? pp : 0) * kz;
+  cpx<T>* lp = lds + p * lds_pencil_elems<N>();
+  LdsPencil<N, T> L{lp};
+  for (int f = 0; f < nf; f += 2) {
+    const bool two = f + 1 < nf;
+    cpx<T> z[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int n = q + Q * r;
+      z[r] = {active ? in[f * in_fs + ib + n] : T(0),
+              (active && two) ? in[(f + 1) * in_fs + ib + n] : T(0)};
+    }
+    pencil_fft<N, -1>(z, q, L);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < R; ++s) lp[out_index<N>(q, s)] = z[s];
+    __syncthreads();
+    if (active) {
+      for (int k = q; k < kz; k += Q) {
+        cpx<T> X, Y;
+        split_pair(lp[k], lp[(N - k) & (N - 1)], X, Y);
+        out[f * out_fs + ob + k] = X;
+        if (two) out[(f + 1) * out_fs + ob + k] = Y;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic block reduction of three doubles (blockDim == 256):
+// xor-butterfly inside each wave, then the four wave sums in fixed order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void block_sum3(double a, double b, double c, double* dst) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off);
+    b += __shfl_xor(b, off);
+    c += __shfl_xor(c, off);
+  }
+  __shared__ double sh[3][kThreads / 64];
+  const int w = threadIdx.x / 64;
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][w] = a;
+    sh[1][w] = b;
+    sh[2][w] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double t = sh[i][0];
+#pragma unroll
+      for (int k = 1; k < kThreads / 64; ++k) t += sh[i][k];
+      dst[i] = t;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// P5: x-forward of (nl0, nl1, nl2), then per retained mode
+//   du = P[mask (NL + f)] - nu k2 us          (navierstokes.py:89-95)
+// and the RK4 stage update (integrators.py:34-38).
+// ---------------------------------------------------------------------------
+template <int N, int MODE, class T>
+__global__ __launch_bounds__(kThreads) void k_x_forward_stage(StageArgs<T> a) {
+  using S = StridedShape<N>;
+  constexpr int R = S::R, Q = S::Q, TC = S::TC;
+  __shared__ cpx<T> lds[S::LDS];
+  const Geo& g = a.g;
+  const int c = threadIdx.x % TC, q = threadIdx.x / TC;
+  const int j = blockIdx.x * TC + c;
+  const bool active = j < g.ky * g.kz;
+  const int jj = active ? j : 0;
+  const int iy = jj / g.kz, iz = jj - iy * g.kz;
+  const int ny = freq_of(iy, g.ky, g.py);
+  const T kyv = wavenumber<T>(g.k0y, ny);
+  const T kzv = wavenumber<T>(g.k0z, iz);
+  LdsStrided<N, TC, T> L{lds + c};
+  cpx<T> v0[R], v1[R], v2[R];
+  cpx<T>* vv[3] = {v0, v1, v2};
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    const cpx<T>* src = a.nl + f * a.nl_fs + jj;
+#pragma unroll
+    for (int r = 0; r < R; ++r) vv[f][r] = active ? src[int64_t(q + Q * r) * g.jp] : czero<T>();
+    pencil_fft<N, -1>(vv[f], q, L);
+  }
+  T coef = a.coef_const;
+  if (a.fterm != kTermNone && a.coef_dev) coef = T(*a.coef_dev);
+  const int wgt = hermitian_weight(iz, g.nz);
+  double sE = 0.0, sO = 0.0, sB = 0.0;
+  const T hdt = T(0.5) * a.dt, sdt = a.dt / T(6), tdt = a.dt / T(3);
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+    const int ix = compact_of_row(out_index<N>(q, s), N, g.kx, g.px);
+    if (!active || ix < 0) continue;
+    const int nx = freq_of(ix, g.kx, g.px);
+    const T kxv = wavenumber<T>(g.k0x, nx);
+    const T k2 = k2_of(kxv, kyv, kzv);
+    const T ik2 = inv_k2_of(k2);
+    const bool keep = in_box(nx, ny, iz, g);
+    const int64_t o = int64_t(ix) * g.jp + j;
+    const cpx<T> s0 = a.us[o], s1 = a.us[o + a.st_fs], s2 = a.us[o + 2 * a.st_fs];
+    T fc = T(0);
+    if (a.fterm == kTermUniform) fc = coef;
+    if (a.fterm == kTermBand) fc = (a.kmin2 <= k2 && k2 <= a.kmax2) ? coef : T(0);
+    cpx<T> d0 = keep ? v0[s] + fc * s0 : czero<T>();
+    cpx<T> d1 = keep ? v1[s] + fc * s1 : czero<T>();
+    cpx<T> d2 = keep ? v2[s] + fc * s2 : czero<T>();
+    const cpx<T> div = ik2 * (kxv * d0 + kyv * d1 + kzv * d2);
+    const T nk2 = a.nu * k2;
+    d0 = d0 - kxv * div - nk2 * s0;
+    d1 = d1 - kyv * div - nk2 * s1;
+    d2 = d2 - kzv * div - nk2 * s2;
+    cpx<T> o0, o1, o2;
+    if constexpr (MODE == kStageRhs) {
+      o0 = d0; o1 = d1; o2 = d2;
+    } else if constexpr (MODE == kStage4) {
+      o0 = a.acc[o] + sdt * d0;
+      o1 = a.acc[o + a.st_fs] + sdt * d1;
+      o2 = a.acc[o + 2 * a.st_fs] + sdt * d2;
+    } else {
+      const cpx<T> b0 = a.u[o], b1 = a.u[o + a.st_fs], b2 = a.u[o + 2 * a.st_fs];
+      const T h = MODE == kStage3 ? a.dt : hdt;
+      o0 = b0 + h * d0; o1 = b1 + h * d1; o2 = b2 + h * d2;
+      if constexpr (MODE == kStage1) {
+        a.acc[o] = b0 + sdt * d0;
+        a.acc[o + a.st_fs] = b1 + sdt * d1;
+        a.acc[o + 2 * a.st_fs] = b2 + sdt * d2;
+      } else {
+        a.acc[o] = a.acc[o] + tdt * d0;
+        a.acc[o + a.st_fs] = a.acc[o + a.st_fs] + tdt * d1;
+        a.acc[o + 2 * a.st_fs] = a.acc[o + 2 * a.st_fs] + tdt * d2;
+      }
+    }
+    a.out[o] = o0;
+    a.out[o + a.st_fs] = o1;
+    a.out[o + 2 * a.st_fs] = o2;
+    if (a.partials) {
+      const double e = double(o0.x) * o0.x + double(o0.y) * o0.y + double(o1.x) * o1.x +
+                       double(o1.y) * o1.y + double(o2.x) * o2.x + double(o2.y) * o2.y;
+      const double we = wgt * e;
+      sE += we;
+      sO += we * double(k2);
+      if (a.kmin2 <= k2 && k2 <= a.kmax2) sB += we;
+    }
+  }
+  if (a.partials) block_sum3(sE, sO, sB, a.partials + 3 * blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers: runtime N -> template instance
+// ---------------------------------------------------------------------------
+#define H3_DISPATCH_N(N, CALL)                 \
+  switch (N) {                                 \
+    case 4: { constexpr int NN = 4; CALL; } break;       \
+    case 8: { constexpr int NN = 8; CALL; } break;       \
+    case 16: { constexpr int NN = 16; CALL; } break;     \
+    case 32: { constexpr int NN = 32; CALL; } break;     \
+    case 64: { constexpr int NN = 64; CALL; } break;     \
+    case 128: { constexpr int NN = 128; CALL; } break;   \
+    case 256: { constexpr int NN = 256; CALL; } break;   \
+    case 512: { constexpr int NN = 512; CALL; } break;   \
+    case 1024: { constexpr int NN = 1024; CALL; } break; \
+    default: return hipErrorInvalidValue;      \
+  }
+
+static inline unsigned ceil_div(int64_t a, int64_t b) { return unsigned((a + b - 1) / b); }
+
+template <class T>
+hipError_t launch_strided(int N, int dir, const StridedArgs<T>& a, hipStream_t s) {
+  if (a.J <= 0 || a.nf <= 0) return hipSuccess;
+  H3_DISPATCH_N(N, {
+    const unsigned nb = ceil_div(a.J, StridedShape<NN>::TC);
+    if (dir < 0) hipLaunchKernelGGL((k_strided<NN, -1, T>), dim3(nb), dim3(kThreads), 0, s, a);
+    else hipLaunchKernelGGL((k_strided<NN, +1, T>), dim3(nb), dim3(kThreads), 0, s, a);
+  });
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_x_inverse_curl(int N, const cpx<T>* us, int64_t us_fs, cpx<T>* out,
+                                 int64_t out_fs, const Geo& g, hipStream_t s) {
+  H3_DISPATCH_N(N, {
+    const unsigned nb = ceil_div(int64_t(g.ky) * g.kz, StridedShape<NN>::TC);
+    hipLaunchKernelGGL((k_x_inverse_curl<NN, T>), dim3(nb), dim3(kThreads), 0, s, us, us_fs, out,
+                       out_fs, g);
+  });
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_y_inverse_curl(int N, const cpx<T>* in, int64_t in_fs, cpx<T>* out,
+                                 int64_t out_fs, const Geo& g, hipStream_t s) {
+  H3_DISPATCH_N(N, {
+    const unsigned nb = ceil_div(int64_t(g.nx) * g.kz, StridedShape<NN>::TC);
+    hipLaunchKernelGGL((k_y_inverse_curl<NN, T>), dim3(nb), dim3(kThreads), 0, s, in, in_fs, out,
+                       out_fs, g);
+  });
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_z_fused(int N, cpx<T>* buf, int64_t fs, int64_t npencils, int kz, T scale,
+                          hipStream_t s) {
+  H3_DISPATCH_N(N, {
+    const unsigned nb = ceil_div(npencils, PencilBlock<NN>::PW);
+    hipLaunchKernelGGL((k_z_fused<NN, T>), dim3(nb), dim3(kThreads), 0, s, buf, fs, npencils, kz,
+                       scale);
+  });
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_z_c2r(int N, const cpx<T>* in, int64_t in_fs, T* out, int64_t out_fs, int nf,
+                        int64_t npencils, int kz, T scale, hipStream_t s) {
+  H3_DISPATCH_N(N, {
+    const unsigned nb = ceil_div(npencils, PencilBlock<NN>::PW);
+    hipLaunchKernelGGL((k_z_c2r<NN, T>), dim3(nb), dim3(kThreads), 0, s, in, in_fs, out, out_fs,
+                       nf, npencils, kz, scale);
+  });
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_z_r2c(int N, const T* in, int64_t in_fs, cpx<T>* out, int64_t out_fs, int nf,
+                        int64_t npencils, int kz, hipStream_t s) {
+  H3_DISPATCH_N(N, {
+    const unsigned nb = ceil_div(npencils, PencilBlock<NN>::PW);
+    hipLaunchKernelGGL((k_z_r2c<NN, T>), dim3(nb), dim3(kThreads), 0, s, in, in_fs, out, out_fs,
+                       nf, npencils, kz);
+  });
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_x_forward_stage(int N, int mode, const StageArgs<T>& a, hipStream_t s,
+                                  int* nblocks_out) {
+  const int64_t J = int64_t(a.g.ky) * a.g.kz;
+  H3_DISPATCH_N(N, {
+    const unsigned nb = ceil_div(J, StridedShape<NN>::TC);
+    if (nblocks_out) *nblocks_out = int(nb);
+    switch (mode) {
+      case kStageRhs: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageRhs, T>), dim3(nb), dim3(kThreads), 0, s, a); break;
+      case kStage1: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage1, T>), dim3(nb), dim3(kThreads), 0, s, a); break;
+      case kStage2: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage2, T>), dim3(nb), dim3(kThreads), 0, s, a); break;
+      case kStage3: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage3, T>), dim3(nb), dim3(kThreads), 0, s, a); break;
+      case kStage4: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage4, T>), dim3(nb), dim3(kThreads), 0, s, a); break;
+      default: return hipErrorInvalidValue;
+    }
+  });
+  return hipGetLastError();
+}
+
+#define H3_INSTANTIATE(T)                                                                        \
+  template hipError_t launch_strided<T>(int, int, const StridedArgs<T>&, hipStream_t);          \
+  template hipError_t launch_x_inverse_curl<T>(int, const cpx<T>*, int64_t, cpx<T>*, int64_t,   \
+                                               const Geo&, hipStream_t);                         \
+  template hipError_t launch_y_inverse_curl<T>(int, const cpx<T>*, int64_t, cpx<T>*, int64_t,   \
+                                               const Geo&, hipStream_t);                         \
+  template hipError_t launch_z_fused<T>(int, cpx<T>*, int64_t, int64_t, int, T, hipStream_t);   \
+  template hipError_t launch_z_c2r<T>(int, const cpx<T>*, int64_t, T*, int64_t, int, int64_t,   \
+                                      int, T, hipStream_t);                                      \
+  template hipError_t launch_z_r2c<T>(int, const T*, int64_t, cpx<T>*, int64_t, int, int64_t,   \
+                                      int, hipStream_t);                                         \
+  template hipError_t launch_x_forward_stage<T>(int, int, const StageArgs<T>&, hipStream_t, int*);
+
+H3_INSTANTIATE(H3_REAL)
+
+}  // namespace h3

@@ -1,0 +1,191 @@
+"""Throughput of the RK4 pseudo-spectral step on one MI355X.
+
+Workload (default): 256^3 forced DNS, float32 state, nu = 1/1000, dt = 1e-3,
+constant-energy Lundgren forcing (A from FP64 energy/enstrophy sums at every
+stage, holding E = 0.5), seeded solenoidal initial field with spectrum
+C k^4 exp(-2 (k/4)^2), E = 0.5. A "step" is one classic RK4 step (four RHS
+evaluations, 5 pencil-pass kernels + 1 FP64 finalize each) with the state
+resident in HBM. Spectra are timed separately, not inside the steps.
+
+Prints one JSON line:
+  value          RK4 steps/s (whole job, one GPU)
+  roofline       dominant kernel: algorithmic bytes per launch / mean launch
+                 time (HIP events on the launch stream), against 8 TB/s
+  cpu_baseline   the float32 NumPy/SciPy restatement (oracle/) of the same step
+                 on this host's cores, on a bounded sample of steps
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "3dhit_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def pass_bytes(N, elem=8):
+    """Algorithmic HBM bytes of each pass of the pruned pipeline (one launch),
+    from the compact extents: c = Kx Ky Kz retained modes, a = N Ky Kz,
+    b = N N Kz (Kx = Ky = 2(N/3)+1, Kz = N/3+1)."""
+    cut = N // 3
+    K, Kz = 2 * cut + 1, cut + 1
+    c, a, b = K * K * Kz, N * K * Kz, N * N * Kz
+    per = {
+        "x_inverse_curl": 3 * c + 5 * a,
+        "y_inverse_curl": 5 * a + 6 * b,
+        "z_c2r_cross_r2c": 9 * b,
+        "y_forward": 3 * b + 3 * a,
+        # stage-averaged: 3a transform input + (3+9+9+6 reads, 6+6+6+3 writes)c / 4
+        "x_forward_stage": 3 * a + 12 * c,
+    }
+    step = 4 * sum(per.values())
+    return {k: v * elem for k, v in per.items()}, step * elem
+
+
+def cpu_baseline(u0, N, nu, dt, budget_s=15.0):
+    """Oracle (float32 NumPy/SciPy, all host cores) on the same initial field:
+    one warm-up step, then steps until `budget_s` (at least two)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import hit3d_oracle as O
+
+    og = O.make_grid(N, dtype=np.float32)
+    rhs = O.make_ns_rhs(og)
+    params = O.NSParams(nu=nu, forcing=O.LundgrenForcing(nu=nu))
+    u = u0.astype(np.complex64)
+    u = O.rk4_step(rhs, u, params, dt, 0.0)
+    n, t0 = 0, time.perf_counter()
+    while n < 2 or time.perf_counter() - t0 < budget_s:
+        u = O.rk4_step(rhs, u, params, dt, 0.0)
+        n += 1
+    el = time.perf_counter() - t0
+    return {"value": n / el, "unit": "RK4 steps/s", "cores": O._WORKERS, "kind": "port",
+            "sample": f"{N}^3 float32 Lundgren step of oracle/hit3d_oracle.py (scipy.fft "
+                      f"workers={O._WORKERS}), 1 warm-up + {n} timed steps, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=10)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU slab "
+              "decomposition is a design, not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import hit3d as h3
+    from hit3d import _lib
+
+    N, nu, dt = args.n, 1.0 / 1000, 1e-3
+    g = h3.make_grid(N, dtype=torch.float32)
+    u0 = h3.random_field(0, g, h3.config_spectrum)
+    assert h3.is_dealiased(u0, g)
+    E0 = h3.energy(u0, g)
+    forcing = h3.LundgrenForcing(nu=nu)
+    fdesc = forcing.desc()
+    ws = g.workspace(_lib.OP_RK4, 3, 1)
+    u = u0.clone()
+    stream = h3.grid.stream()
+
+    def steps(n):
+        _lib.call("hit3d_rk4_steps", g.desc, u.data_ptr(), nu, dt, n, 0.0, ctypes.byref(fdesc),
+                  1, None, ws.data_ptr(), ws.numel(), stream)
+
+    steps(args.warmup)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    w0 = time.perf_counter()
+    ev0.record()
+    steps(args.steps)
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    ms_total = ev0.elapsed_time(ev1)
+    ms_step = ms_total / args.steps
+    sps = 1000.0 / ms_step
+    E1 = h3.energy(u, g)
+
+    # per-pass HIP-event profile of the same workload (separate run)
+    ms = (ctypes.c_double * 7)()
+    cnt = (ctypes.c_int64 * 7)()
+    up = u.clone()
+    _lib.call("hit3d_profile_rk4", g.desc, up.data_ptr(), nu, dt, args.profile_steps,
+              ctypes.byref(fdesc), 1, ms, cnt, ws.data_ptr(), ws.numel(), stream)
+    per_bytes, step_bytes = pass_bytes(N)
+    passes = {}
+    for i, name in enumerate(_lib.PASS_NAMES):
+        if cnt[i] == 0:
+            continue
+        avg = ms[i] / cnt[i]
+        d = {"avg_ms": round(avg, 5), "launches": int(cnt[i]),
+             "share": round(ms[i] / sum(ms[j] for j in range(7)), 4)}
+        if name in per_bytes:
+            d["bytes"] = per_bytes[name]
+            d["gbs"] = round(per_bytes[name] / (avg * 1e-3) / 1e9, 1)
+        passes[name] = d
+    dom = max((k for k in passes if k in per_bytes), key=lambda k: passes[k]["avg_ms"] * passes[k]["launches"])
+    achieved = passes[dom]["gbs"]
+
+    # spectrum cost (not part of the step)
+    spec = h3.diagnostics.shell_spectrum(u, g)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        spec = h3.diagnostics.shell_spectrum(u, g)
+    e1.record()
+    torch.cuda.synchronize()
+    spec_ms = e0.elapsed_time(e1) / 10
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(u0.cpu().numpy(), N, nu, dt)
+
+    out = {
+        "metric": "RK4 steps/s at 256³/1024³ fp32, % HBM peak; learned-LES speedup at ≤5% E(k)",
+        "value": round(sps, 3),
+        "unit": "RK4 steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{N}^3 Lundgren-forced DNS (constant energy 0.5, FP64 sums per stage), "
+                        f"nu=1/1000, dt=1e-3, seeded k^4 exp(-2(k/4)^2) field, pruned 2/3 pipeline",
+            "N": N, "nu": nu, "dt": dt, "reductions": "f64",
+        },
+        "grid_point_steps_per_s": round(sps * N**3, 1),
+        "step_bytes": step_bytes,
+        "hbm_frac_step": round(step_bytes * sps / (HBM_PEAK_GBS * 1e9), 4),
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "passes": passes,
+        "spectrum_ms": round(spec_ms, 4),
+        "energy": {"start": E0, "end": E1},
+        "wall_s_timed": round(wall, 4),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,165 @@
+/*
+ * hit3d C ABI — MI355X (gfx950) kernels for hit3d's pseudo-spectral
+ * Navier–Stokes step.
+ *
+ * Plain C: device pointers, sizes, a HIP stream passed as `void*` (NULL =
+ * the default stream). No call allocates device memory: every entry point
+ * that needs scratch takes a caller-owned workspace whose size comes from
+ * hit3d_workspace_size(). All calls are asynchronous on `stream` and never
+ * synchronise the host. On failure a call returns a non-zero status and
+ * hit3d_last_error() (per host thread) holds the message; no C++ exception
+ * crosses this boundary.
+ *
+ * Arrays (C order, interleaved complex):
+ *   spectral state  complex[ncomp][nx][ny][nz/2+1]    (grid.py:7-11 layout)
+ *   physical field  real[ncomp][nx][ny][nz]
+ * with real = float (precision 4, complex64) or double (precision 8,
+ * complex128). Grid sizes are powers of two, 4 <= n <= 1024 per axis.
+ *
+ * Each entry point names the reference symbol it replaces (file:line in
+ * nfrank22-gh/3DHIT src/hit3d/).
+ */
+#ifndef HIT3D_H_
+#define HIT3D_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HIT3D_OK 0
+#define HIT3D_ERR_INVALID 1     /* bad shape, size, pointer or parameter */
+#define HIT3D_ERR_HIP 2         /* HIP launch/runtime error */
+#define HIT3D_ERR_WORKSPACE 3   /* workspace smaller than required */
+
+/* Box description: replaces the static fields of `Grid` (grid.py:41-86). */
+typedef struct hit3d_grid {
+  int64_t nx, ny, nz;   /* grid points per axis */
+  double lx, ly, lz;    /* box lengths (2*pi for the unit-wavenumber box) */
+  int32_t precision;    /* 4: float32 state, 8: float64 state */
+  int32_t reserved;     /* must be 0 */
+} hit3d_grid;
+
+/* Forcing plug-in selector (forcing.py:55-163, plus the constant-energy rule). */
+#define HIT3D_FORCING_NONE 0      /* NoForcing      forcing.py:55-71   */
+#define HIT3D_FORCING_LINEAR 1    /* LinearForcing  forcing.py:74-106: f = a u */
+#define HIT3D_FORCING_BAND 2      /* BandForcing    forcing.py:109-163 */
+#define HIT3D_FORCING_LUNDGREN 3  /* constant-energy f = A(u) u,
+                                     A = (2 nu Omega - (E - energy)/tau) / (2E) */
+typedef struct hit3d_forcing {
+  int32_t kind;
+  int32_t reserved;       /* must be 0 */
+  double a;               /* LINEAR: coefficient A */
+  double eps, kmin, kmax; /* BAND: injection rate and shell limits */
+  double nu, energy, tau; /* LUNDGREN: viscosity in A(u), target E, relaxation
+                             time (tau <= 0 or non-finite: no relaxation term) */
+} hit3d_forcing;
+
+/* Operation selectors for hit3d_workspace_size. */
+#define HIT3D_OP_TO_PHYSICAL 0
+#define HIT3D_OP_TO_SPECTRAL 1
+#define HIT3D_OP_RHS 2
+#define HIT3D_OP_RK4 3
+#define HIT3D_OP_SPECTRUM 4
+#define HIT3D_OP_REDUCE 5
+
+/* Message of the last failed call on this host thread (always NUL-terminated);
+   returns the full message length. */
+int hit3d_last_error(char* buf, size_t len);
+
+/* Library ABI version (1). */
+int hit3d_version(void);
+
+/* Bytes of workspace `op` needs. `ncomp` applies to the transforms; `dealiased`
+   selects the pruned (1) or dense (0) step layout for RHS/RK4. */
+int hit3d_workspace_size(const hit3d_grid* grid, int32_t op, int64_t ncomp, int32_t dealiased,
+                         size_t* bytes);
+
+/* Number of shell bins written by hit3d_shell_spectrum: smax + 1 with
+   smax = ceil(sqrt(3) max(n) / 2) + 1 (diagnostics.py:78-80). */
+int hit3d_shell_bins(const hit3d_grid* grid, int64_t* nbins);
+
+/* Per-mode tables as the kernels compute them, full spectral grid
+   [nx][ny][nz/2+1]: k2 and inv_k2 in the state precision, the 2/3 dealias
+   mask (uint8), the shell index floor(sqrt(k2)/k0 + 1/2) (int32). Any output
+   may be NULL. Replaces make_grid's k2/inv_k2/dealias (grid.py:109-120) and
+   _shells (diagnostics.py:76-82). */
+int hit3d_grid_tables(const hit3d_grid* grid, void* k2, void* inv_k2, uint8_t* mask,
+                      int32_t* shell, void* stream);
+
+/* irfftn over axes (1,2,3), 1/N normalisation: to_physical (grid.py:181-183). */
+int hit3d_to_physical(const hit3d_grid* grid, const void* u_hat, void* u, int64_t ncomp,
+                      void* work, size_t work_bytes, void* stream);
+
+/* rfftn over axes (1,2,3), unnormalised: to_spectral (grid.py:186-188). */
+int hit3d_to_spectral(const hit3d_grid* grid, const void* u, void* u_hat, int64_t ncomp,
+                      void* work, size_t work_bytes, void* stream);
+
+/* out = project(dealias(in)) (grid.py:166-178); either step may be skipped.
+   do_project needs ncomp == 3. In-place allowed. */
+int hit3d_dealias_project(const hit3d_grid* grid, const void* in, void* out, int64_t ncomp,
+                          int32_t do_dealias, int32_t do_project, void* stream);
+
+/* i kx u, i ky u, i kz u, -k2 u for op = 0..3 (grid.py:146-163). In-place allowed. */
+int hit3d_spectral_derivative(const hit3d_grid* grid, const void* in, void* out, int64_t ncomp,
+                              int32_t op, void* stream);
+
+/* du = rhs(u_hat, params, t) of make_ns_rhs (navierstokes.py:62-97):
+   P[dealias(F(u x w) + f)] - nu k2 u. `dealiased` = 1 promises that u_hat has
+   no content outside the 2/3 box (|n_i| <= n_i/3; the k = 0 mode may be
+   non-zero), which selects the pruned transforms; 0 runs dense transforms. */
+int hit3d_rhs(const hit3d_grid* grid, const void* u_hat, void* du_hat, double nu,
+              const hit3d_forcing* forcing, double t, int32_t dealiased, void* work,
+              size_t work_bytes, void* stream);
+
+/* In place, nsteps classic RK4 steps of size dt from t0: rollout
+   (integrators.py:41-53) over rk4_step (integrators.py:32-38). If `diag` is
+   not NULL it receives 4 doubles (device memory) for the final state:
+   [forcing coefficient, 2E, 2 Omega, 2 E_band] (FP64 sums, compensated). */
+int hit3d_rk4_steps(const hit3d_grid* grid, void* u_hat, double nu, double dt, int64_t nsteps,
+                    double t0, const hit3d_forcing* forcing, int32_t dealiased, double* diag,
+                    void* work, size_t work_bytes, void* stream);
+
+/* FP64 shell spectrum (diagnostics.py:85-114): out[s] (device, nbins doubles)
+   = sum over modes of shell s of w |u_c|^2 / (2 Ntot^2), summed over
+   components (component = -1) or for one component (0..2). Bin 0 holds the
+   k = 0 mode. Deterministic (fixed order), Neumaier-compensated. */
+int hit3d_shell_spectrum(const hit3d_grid* grid, const void* u_hat, int32_t component,
+                         double* out, void* work, size_t work_bytes, void* stream);
+
+/* FP64 reductions (diagnostics.py:56-70, forcing.py:48-52): out (device, 4
+   doubles) = [E, eps = 2 nu Omega, Omega, E_band], E_band over
+   kmin^2 <= k2 <= kmax^2. */
+int hit3d_energy_dissipation(const hit3d_grid* grid, const void* u_hat, double nu, double kmin,
+                             double kmax, double* out, void* work, size_t work_bytes,
+                             void* stream);
+
+/* out (device, 1 double) = sum of |u_hat|^2 over the modes outside the 2/3
+   box: zero means the state can take the pruned (`dealiased` = 1) path. */
+int hit3d_alias_energy(const hit3d_grid* grid, const void* u_hat, double* out, void* work,
+                       size_t work_bytes, void* stream);
+
+/* U[-0.5, 0.5) noise real[3][nx][ny][nz] from Philox4x64-10 with key (seed, 0):
+   the draw of random_field (fields.py:45-46) with a counter-based generator. */
+int hit3d_uniform_noise(const hit3d_grid* grid, uint64_t seed, void* u, void* stream);
+
+/* u_hat[c][mode] *= factors[shell(mode)] for shell < nbins (fields.py:58-68). */
+int hit3d_shell_scale(const hit3d_grid* grid, void* u_hat, int64_t ncomp, const double* factors,
+                      int64_t nbins, void* stream);
+
+/* Measurement form of hit3d_rk4_steps (no reference counterpart): runs the
+   same nsteps with a HIP event after every launch on `stream`, waits for the
+   last one, and returns per kernel class the summed milliseconds `ms[7]` and
+   launch counts `counts[7]` (host arrays) for: 0 x-inverse + curl,
+   1 y-inverse, 2 z c2r/cross/r2c, 3 y-forward, 4 x-forward + stage,
+   5 FP64 finalize, 6 set-up. Synchronises the host. */
+int hit3d_profile_rk4(const hit3d_grid* grid, void* u_hat, double nu, double dt, int64_t nsteps,
+                      const hit3d_forcing* forcing, int32_t dealiased, double* ms, int64_t* counts,
+                      void* work, size_t work_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HIT3D_H_ */

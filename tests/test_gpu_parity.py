@@ -1,0 +1,301 @@
+"""Parity of the HIP path (called through the C ABI via the hit3d package)
+with the CPU oracle on the same inputs.
+
+Bars: tables / masks / shell indices / noise bit-exact; float64 kernels at
+round-off (<= 1e-11 relative); float32 kernels at the tolerance written in
+each test (state after 100 steps <= 1e-5 relative L2 against float64)."""
+
+import numpy as np
+import pytest
+import torch
+
+import hit3d_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+h3 = pytest.importorskip("hit3d")
+
+DT = {np.float32: torch.float32, np.float64: torch.float64}
+
+
+def _grids(shape, dtype):
+    return O.make_grid(*shape, dtype=dtype), h3.make_grid(*shape, dtype=DT[dtype])
+
+
+def rel(a, b):
+    a = a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+    b = b.detach().cpu().numpy() if isinstance(b, torch.Tensor) else np.asarray(b)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(np.asarray(b).ravel()), 1e-300))
+
+
+def dev(x, grid):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(grid.device)
+
+
+def random_spectral(seed, og, dealiased=True, ncomp=3):
+    rng = np.random.default_rng(seed)
+    u = rng.standard_normal((ncomp, og.Nx, og.Ny, og.Nz)).astype(og.dtype)
+    uh = O.to_spectral(u, og)
+    return O.dealias(uh, og) if dealiased else uh
+
+
+# --- tables: bit-exact ----------------------------------------------------------
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", [(16, 16, 16), (64, 64, 64), (16, 8, 4), (32, 16, 64), (256, 256, 256)])
+def test_tables_bit_exact(shape, dtype):
+    og, g = _grids(shape, dtype)
+    assert np.array_equal(g.k2.cpu().numpy(), og.k2)
+    assert np.array_equal(g.inv_k2.cpu().numpy(), og.inv_k2)
+    assert np.array_equal(g.dealias.cpu().numpy(), og.dealias)
+    shell, _, _ = O.shells(og)
+    assert np.array_equal(g.shell.cpu().numpy(), shell)
+    assert np.array_equal(g.kx.cpu().numpy(), og.kx) and np.array_equal(g.kz.cpu().numpy(), og.kz)
+    assert np.array_equal(g.weights.cpu().numpy(), og.weights)
+
+
+def test_tables_bit_exact_non_unit_box():
+    og = O.make_grid(32, 16, 8, Lx=3.0, Ly=1.7, Lz=5.5, dtype=np.float32)
+    g = h3.make_grid(32, 16, 8, Lx=3.0, Ly=1.7, Lz=5.5, dtype=torch.float32)
+    assert np.array_equal(g.k2.cpu().numpy(), og.k2)
+    assert np.array_equal(g.inv_k2.cpu().numpy(), og.inv_k2)
+    assert np.array_equal(g.shell.cpu().numpy(), O.shells(og)[0])
+
+
+# --- noise: bit-exact Philox ----------------------------------------------------------
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_uniform_noise_bit_exact(dtype):
+    og, g = _grids((16, 8, 32), dtype)
+    ref = O.philox_uniform(123, (3, 16, 8, 32), dtype)
+    assert np.array_equal(h3.uniform_noise(123, g).cpu().numpy(), ref)
+
+
+# --- transforms -----------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-13), (np.float32, 2e-6)])
+@pytest.mark.parametrize("shape", [(4, 4, 4), (8, 8, 8), (16, 8, 4), (32, 64, 16), (64, 64, 64),
+                                   (128, 128, 128), (256, 32, 512), (1024, 8, 8), (8, 1024, 4),
+                                   (4, 8, 1024)])
+def test_transforms_match_numpy(shape, dtype, tol):
+    og, g = _grids(shape, dtype)
+    u = np.random.default_rng(0).standard_normal((3,) + shape).astype(dtype)
+    uh = h3.to_spectral(dev(u, g), g)
+    assert rel(uh, O.to_spectral(u, og)) < tol
+    # irfftn of a generic (non-Hermitian) half spectrum follows c2r semantics
+    rng = np.random.default_rng(1)
+    s = (2,) + shape[:2] + (shape[2] // 2 + 1,)
+    vh = (rng.standard_normal(s) + 1j * rng.standard_normal(s)).astype(O.complex_dtype(og))
+    assert rel(h3.to_physical(dev(vh, g), g), O.to_physical(vh, og)) < tol
+    assert rel(h3.to_physical(uh, g), u) < tol
+
+
+def test_transform_single_component_and_odd_count():
+    og, g = _grids((16, 16, 16), np.float64)
+    u = np.random.default_rng(3).standard_normal((16, 16, 16))
+    assert rel(h3.to_spectral(dev(u, g), g), O.to_spectral(u[None], og)[0]) < 1e-13
+    u5 = np.random.default_rng(4).standard_normal((5, 16, 16, 16))
+    uh5 = h3.to_spectral(dev(u5, g), g)
+    assert rel(uh5, O.to_spectral(u5, og)) < 1e-13
+    assert rel(h3.to_physical(uh5, g), u5) < 1e-13
+
+
+# --- elementwise operators ----------------------------------------------------------------
+
+
+def test_dealias_project_derivatives():
+    og, g = _grids((32, 16, 32), np.float64)
+    uh = random_spectral(5, og, dealiased=False)
+    d = dev(uh, g)
+    assert rel(h3.dealias(d, g), O.dealias(uh, og)) == 0.0
+    assert rel(h3.project(d, g), O.project(uh, og)) < 1e-15
+    for f, of in ((h3.ddx, O.ddx), (h3.ddy, O.ddy), (h3.ddz, O.ddz), (h3.laplacian, O.laplacian)):
+        assert rel(f(d, g), of(uh, og)) < 1e-15
+
+
+# --- RHS ---------------------------------------------------------------------------------------
+
+FORCINGS = [
+    (lambda: O.NoForcing(), lambda: h3.NoForcing()),
+    (lambda: O.LinearForcing(A=0.15), lambda: h3.LinearForcing(A=0.15)),
+    (lambda: O.BandForcing(eps=0.3, kmin=1.0, kmax=3.0), lambda: h3.BandForcing(eps=0.3, kmin=1.0, kmax=3.0)),
+    (lambda: O.LundgrenForcing(nu=0.02, energy=0.5, tau=0.3), lambda: h3.LundgrenForcing(nu=0.02, energy=0.5, tau=0.3)),
+]
+
+
+@pytest.mark.parametrize("fi", range(len(FORCINGS)))
+@pytest.mark.parametrize("dealiased", [True, False])
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 3e-6)])
+def test_rhs_matches_oracle(fi, dealiased, dtype, tol):
+    og, g = _grids((32, 32, 32), dtype)
+    uh = O.project(random_spectral(7 + fi, og, dealiased), og)
+    of, gf = FORCINGS[fi]
+    ref = O.make_ns_rhs(og)(uh.astype(np.complex128) if dtype == np.float64 else uh,
+                            O.NSParams(nu=0.02, forcing=of()), 0.0)
+    du = h3.make_ns_rhs(g)(dev(uh, g), h3.NSParams(nu=0.02, forcing=gf()), 0.0)
+    assert h3.is_dealiased(dev(uh, g), g) == dealiased
+    assert rel(du, ref) < tol
+
+
+@pytest.mark.parametrize("shape", [(16, 8, 4), (8, 32, 16), (64, 16, 32)])
+def test_rhs_anisotropic(shape):
+    og, g = _grids(shape, np.float64)
+    uh = O.project(random_spectral(11, og), og)
+    ref = O.make_ns_rhs(og)(uh, O.NSParams(nu=0.01), 0.0)
+    assert rel(h3.make_ns_rhs(g)(dev(uh, g), h3.NSParams(nu=0.01), 0.0), ref) < 1e-12
+
+
+def test_beltrami_rhs_is_pure_viscous_decay():
+    """reference tests/test_navierstokes.py:10-19 on the GPU (float64, dense path)."""
+    og, g = _grids((32, 32, 32), np.float64)
+    u0 = O.beltrami(og)
+    du = h3.make_ns_rhs(g)(dev(u0, g), h3.NSParams(nu=0.05), 0.0)
+    assert np.allclose(du.cpu().numpy(), -0.05 * u0, atol=1e-10)
+
+
+# --- RK4 ------------------------------------------------------------------------------------------
+
+
+def test_beltrami_decay_matches_analytic():
+    """reference tests/test_navierstokes.py:22-32 (float64)."""
+    og, g = _grids((32, 32, 32), np.float64)
+    u0 = O.beltrami(og)
+    u = h3.rollout(h3.make_ns_rhs(g), dev(u0, g), h3.NSParams(nu=0.05), 1e-2, 100)
+    assert rel(u, np.exp(-0.05 * 1.0) * u0) < 1e-10
+
+
+@pytest.mark.parametrize("fi", range(len(FORCINGS)))
+def test_rollout_float64_matches_oracle(fi):
+    og, g = _grids((32, 32, 32), np.float64)
+    u0 = O.random_field(3, og, O.config_spectrum)
+    of, gf = FORCINGS[fi]
+    ref = O.rollout(O.make_ns_rhs(og), u0, O.NSParams(nu=0.02, forcing=of()), 2e-3, 10)
+    u = h3.rollout(h3.make_ns_rhs(g), dev(u0, g), h3.NSParams(nu=0.02, forcing=gf()), 2e-3, 10)
+    assert rel(u, ref) < 1e-11
+
+
+def test_rk4_step_dense_path_matches_oracle():
+    og, g = _grids((16, 16, 16), np.float64)
+    u0 = O.project(random_spectral(21, og, dealiased=False), og)
+    ref = O.rk4_step(O.make_ns_rhs(og), u0, O.NSParams(nu=0.03), 1e-3, 0.0)
+    u = h3.rk4_step(h3.make_ns_rhs(g), dev(u0, g), h3.NSParams(nu=0.03), 1e-3, 0.0)
+    assert rel(u, ref) < 1e-12
+
+
+def test_decaying_64_float32_100_steps_within_1e5():
+    """The decaying 64^3 case (nu = 1/400, dt = 2e-3, 100 steps): the float32
+    GPU state against the float64 oracle from the same initial field."""
+    og, g = _grids((64, 64, 64), np.float64)
+    g32 = h3.make_grid(64, dtype=torch.float32)
+    u0 = h3.random_field(0, g32, h3.config_spectrum)
+    u0_64 = u0.cpu().numpy().astype(np.complex128)
+    ref = O.rollout(O.make_ns_rhs(og), u0_64, O.NSParams(nu=1 / 400), 2e-3, 100)
+    u = h3.rollout(h3.make_ns_rhs(g32), u0, h3.NSParams(nu=1 / 400), 2e-3, 100)
+    err = rel(u.to(torch.complex128), ref)
+    print(f"decaying 64^3 fp32 vs fp64 oracle after 100 steps: rel L2 = {err:.3e}")
+    assert err < 1e-5
+
+
+def test_rollout_invariants_and_inviscid_energy():
+    """reference tests/test_navierstokes.py:35-54 (float64)."""
+    og, g = _grids((32, 32, 32), np.float64)
+    u0 = O.project(random_spectral(4, og), og)
+    rhs = h3.make_ns_rhs(g)
+    u = h3.rollout(rhs, dev(u0, g), h3.NSParams(nu=0.02), 1e-3, 10)
+    un = u.cpu().numpy()
+    div = og.kx * un[0] + og.ky * un[1] + og.kz * un[2]
+    assert np.max(np.abs(div)) < 1e-8
+    assert np.array_equal(un * og.dealias, un)
+    u = h3.rollout(rhs, dev(u0, g), h3.NSParams(nu=0.0), 1e-3, 20)
+    assert np.isclose(h3.energy(u, g), O.energy(u0, og), rtol=1e-8)
+
+
+def test_lundgren_holds_energy_float32():
+    g = h3.make_grid(64, dtype=torch.float32)
+    u0 = h3.random_field(0, g, h3.config_spectrum)
+    assert abs(h3.energy(u0, g) - 0.5) < 1e-6
+    u = h3.rollout(h3.make_ns_rhs(g), u0, h3.NSParams(nu=1e-3, forcing=h3.LundgrenForcing(nu=1e-3)), 1e-3, 50)
+    assert abs(h3.energy(u, g) - 0.5) < 2e-5
+
+
+def test_rollout_is_pure_and_zero_steps():
+    og, g = _grids((16, 16, 16), np.float64)
+    u0 = dev(O.random_field(1, og), g)
+    keep = u0.clone()
+    h3.rollout(h3.make_ns_rhs(g), u0, h3.NSParams(nu=0.02), 1e-3, 3)
+    assert torch.equal(u0, keep)
+    assert torch.equal(h3.rollout(h3.make_ns_rhs(g), u0, h3.NSParams(nu=0.02), 1e-3, 0), keep)
+
+
+# --- diagnostics ------------------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-13), (np.float32, 1e-6)])
+def test_diagnostics_match_oracle(dtype, tol):
+    og, g = _grids((64, 32, 32), dtype)
+    uh = random_spectral(9, og, dealiased=False)
+    d = dev(uh, g)
+    uh64 = uh.astype(np.complex128)
+    og64 = O.make_grid(64, 32, 32)
+    assert abs(h3.energy(d, g) / O.energy(uh64, og64) - 1) < tol
+    assert abs(h3.enstrophy(d, g) / O.enstrophy(uh64, og64) - 1) < tol
+    assert abs(h3.dissipation(d, g, 0.01) / O.dissipation(uh64, og64, 0.01) - 1) < tol
+    ks, Es = h3.energy_spectrum(d, g)
+    oks, oEs = O.energy_spectrum(uh64, og64)
+    assert np.array_equal(ks, oks)
+    assert np.max(np.abs(Es - oEs)) < tol * np.max(oEs)
+    _, E1, E2, E3 = h3.component_spectra(d, g)
+    _, oE1, oE2, oE3 = O.component_spectra(uh64, og64)
+    for a, b in ((E1, oE1), (E2, oE2), (E3, oE3)):
+        assert np.max(np.abs(a - b)) < tol * np.max(b)
+
+
+def test_spectrum_deterministic_and_sums_to_energy():
+    g = h3.make_grid(128, dtype=torch.float32)
+    u = h3.random_field(5, g, h3.config_spectrum)
+    _, E1 = h3.energy_spectrum(u, g)
+    _, E2 = h3.energy_spectrum(u, g)
+    assert np.array_equal(E1, E2)
+    assert abs(E1.sum() - h3.energy(u, g)) < 1e-12
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 1e-6)])
+def test_random_field_matches_oracle(dtype, tol):
+    og, g = _grids((32, 32, 32), dtype)
+    ref = O.random_field(2, og)
+    u = h3.random_field(2, g)
+    assert rel(u, ref) < tol
+    ks, Es = h3.energy_spectrum(u, g)
+    target = O.default_spectrum(ks)
+    assert np.allclose(Es[1:10], target[1:10], rtol=max(tol, 1e-8) * 10)
+
+
+def test_injection_and_budget_match_oracle():
+    og, g = _grids((32, 32, 32), np.float64)
+    uh = O.random_field(0, og)
+    d = dev(uh, g)
+    for of, gf in FORCINGS:
+        assert np.isclose(gf().injection(d, g), float(of().injection(uh, og)), rtol=1e-12, atol=0)
+    b = h3.energy_budget(d, g, 0.05, h3.BandForcing(eps=0.3, kmin=1.0, kmax=3.0))
+    assert b["P"] == 0.3 and np.isclose(b["E"], O.energy(uh, og), rtol=1e-13)
+    assert h3.BandForcing(eps=0.3, kmin=1.0, kmax=3.0).injection(torch.zeros_like(d), g) == 0.0
+
+
+# --- error behaviour -------------------------------------------------------------------------------
+
+
+def test_bad_inputs_raise():
+    g = h3.make_grid(16, dtype=torch.float32)
+    rhs = h3.make_ns_rhs(g)
+    with pytest.raises(ValueError):
+        rhs(torch.zeros((3, 16, 16, 8), dtype=torch.complex64, device="cuda"), h3.NSParams(nu=0.1), 0.0)
+    with pytest.raises(ValueError):
+        rhs(torch.zeros((3, 16, 16, 9), dtype=torch.complex128, device="cuda"), h3.NSParams(nu=0.1), 0.0)
+    with pytest.raises(ValueError):
+        h3.make_grid(12)
+    with pytest.raises(ValueError):
+        h3.rollout(rhs, torch.zeros((3, 16, 16, 9), dtype=torch.complex64, device="cuda"),
+                   h3.NSParams(nu=0.1), 1e-3, -1)
