@@ -134,6 +134,39 @@ struct PencilShape {
   static_assert(R * Q == N && (M == 1 || M == 2), "N must be a power of two");
 };
 
+// The inter-step twiddles w_N^{-q k1} (forward sign) of thread q, k1 < R,
+// loaded once when a thread runs several transforms.
+template <int N, class T>
+__device__ __forceinline__ void load_twiddles(cpx<T>* tw, int q) {
+#pragma unroll
+  for (int k1 = 0; k1 < PencilShape<N>::R; ++k1) tw[k1] = twiddle<N, -1, T>(q * k1);
+}
+
+template <int N, int DIR, class T, class Lds>
+__device__ __forceinline__ void pencil_fft_tw(cpx<T>* v, int q, Lds lds, const cpx<T>* tw) {
+  constexpr int R = PencilShape<N>::R, Q = PencilShape<N>::Q, M = PencilShape<N>::M;
+  dft<R, DIR>(v);
+#pragma unroll
+  for (int k1 = 1; k1 < R; ++k1) {
+    cpx<T> w = tw[k1];
+    if constexpr (DIR > 0) w.y = -w.y;
+    v[k1] = cmul(v[k1], w);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k1 = 0; k1 < R; ++k1) *lds(k1, q) = v[k1];
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    cpx<T> u[Q];
+#pragma unroll
+    for (int qq = 0; qq < Q; ++qq) u[qq] = *lds(q + Q * m, qq);
+    dft<Q, DIR>(u);
+#pragma unroll
+    for (int k2 = 0; k2 < Q; ++k2) v[m * Q + k2] = u[k2];
+  }
+}
+
 template <int N, int DIR, class T, class Lds>
 __device__ __forceinline__ void pencil_fft(cpx<T>* v, int q, Lds lds) {
   constexpr int R = PencilShape<N>::R, Q = PencilShape<N>::Q, M = PencilShape<N>::M;
@@ -153,6 +186,17 @@ __device__ __forceinline__ void pencil_fft(cpx<T>* v, int q, Lds lds) {
 #pragma unroll
     for (int k2 = 0; k2 < Q; ++k2) v[m * Q + k2] = u[k2];
   }
+}
+
+// Strided passes tile columns across workgroups, and neighbouring tiles share
+// the cache lines at their edges. Workgroups are dealt round-robin to the 8
+// XCDs (each with its own L2), so map the workgroups one XCD receives to a
+// contiguous run of tiles: a shared line is then fetched once, into one L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  constexpr int X = 8;
+  const int q = nb / X, r = nb % X;
+  const int x = b % X, i = b / X;
+  return x * q + min(x, r) + i;
 }
 
 // Output slot s = m*Q + k2 of pencil_fft holds frequency/position
@@ -223,14 +267,14 @@ __device__ __forceinline__ T wavenumber(double k0, int n) {
   return T(k0 * double(n));
 }
 
-// k2 = kx**2 + ky**2 + kz**2 in dtype, each operation rounded (no FMA).
+// k2 = kx**2 + ky**2 + kz**2 in dtype, each operation rounded: contraction
+// into FMA is switched off for this scope (an inlined x*x + y*y would
+// otherwise fuse and round once, unlike the reference's NumPy tables).
 template <class T>
 __device__ __forceinline__ T k2_of(T kx, T ky, T kz) {
-  if constexpr (sizeof(T) == 4) {
-    return __fadd_rn(__fadd_rn(__fmul_rn(kx, kx), __fmul_rn(ky, ky)), __fmul_rn(kz, kz));
-  } else {
-    return __dadd_rn(__dadd_rn(__dmul_rn(kx, kx), __dmul_rn(ky, ky)), __dmul_rn(kz, kz));
-  }
+#pragma clang fp contract(off)
+  const T a = kx * kx, b = ky * ky, c = kz * kz;
+  return (a + b) + c;
 }
 
 // where(k2 == 0, 0, 1/k2), correctly rounded

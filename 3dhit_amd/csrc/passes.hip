@@ -21,13 +21,15 @@ __global__ __launch_bounds__(kThreads) void k_strided(StridedArgs<T> a) {
   constexpr int R = S::R, Q = S::Q, TC = S::TC;
   __shared__ cpx<T> lds[S::LDS];
   const int c = threadIdx.x % TC, q = threadIdx.x / TC;
-  const int j = blockIdx.x * TC + c;
+  const int j = xcd_tile(blockIdx.x, gridDim.x) * TC + c;
   const bool active = j < a.J;
   const int jj = active ? j : 0;
   const int ao = jj / a.B, bo = jj - ao * a.B;
   const cpx<T>* in0 = a.in + ao * a.in_slab + bo;
   cpx<T>* out0 = a.out + ao * a.out_slab + bo;
   LdsStrided<N, TC, T> L{lds + c};
+  cpx<T> tw[R];
+  load_twiddles<N>(tw, q);
   int irow[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) irow[r] = active ? compact_of_row(q + Q * r, N, a.in_k, a.in_p) : -1;
@@ -36,7 +38,7 @@ __global__ __launch_bounds__(kThreads) void k_strided(StridedArgs<T> a) {
     const cpx<T>* src = in0 + f * a.in_fs;
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = irow[r] >= 0 ? src[int64_t(irow[r]) * a.B] : czero<T>();
-    pencil_fft<N, DIR>(v, q, L);
+    pencil_fft_tw<N, DIR>(v, q, L, tw);
     cpx<T>* dst = out0 + f * a.out_fs;
 #pragma unroll
     for (int s = 0; s < R; ++s) {
@@ -60,13 +62,15 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse_curl(const cpx<T>* __res
   constexpr int R = S::R, Q = S::Q, TC = S::TC;
   __shared__ cpx<T> lds[S::LDS];
   const int c = threadIdx.x % TC, q = threadIdx.x / TC;
-  const int j = blockIdx.x * TC + c;
+  const int j = xcd_tile(blockIdx.x, gridDim.x) * TC + c;
   const bool active = j < g.ky * g.kz;
   const int jj = active ? j : 0;
   const int iy = jj / g.kz, iz = jj - iy * g.kz;
   const T kyv = wavenumber<T>(g.k0y, freq_of(iy, g.ky, g.py));
   const T kzv = wavenumber<T>(g.k0z, iz);
   LdsStrided<N, TC, T> L{lds + c};
+  cpx<T> tw[R];
+  load_twiddles<N>(tw, q);
   int irow[R];
   T kxv[R];
 #pragma unroll
@@ -92,18 +96,18 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse_curl(const cpx<T>* __res
   load(u2, b);
 #pragma unroll
   for (int r = 0; r < R; ++r) w[r] = mul_i(kzv * a[r] - kxv[r] * b[r]);
-  pencil_fft<N, +1>(w, q, L);
+  pencil_fft_tw<N, +1>(w, q, L, tw);
   store(3, w);
-  pencil_fft<N, +1>(b, q, L);
+  pencil_fft_tw<N, +1>(b, q, L, tw);
   store(2, b);
   load(u1, b);
 #pragma unroll
   for (int r = 0; r < R; ++r) w[r] = mul_i(kxv[r] * b[r] - kyv * a[r]);
-  pencil_fft<N, +1>(w, q, L);
+  pencil_fft_tw<N, +1>(w, q, L, tw);
   store(4, w);
-  pencil_fft<N, +1>(b, q, L);
+  pencil_fft_tw<N, +1>(b, q, L, tw);
   store(1, b);
-  pencil_fft<N, +1>(a, q, L);
+  pencil_fft_tw<N, +1>(a, q, L, tw);
   store(0, a);
 }
 
@@ -121,12 +125,14 @@ __global__ __launch_bounds__(kThreads) void k_y_inverse_curl(const cpx<T>* __res
   constexpr int R = S::R, Q = S::Q, TC = S::TC;
   __shared__ cpx<T> lds[S::LDS];
   const int c = threadIdx.x % TC, q = threadIdx.x / TC;
-  const int j = blockIdx.x * TC + c;
+  const int j = xcd_tile(blockIdx.x, gridDim.x) * TC + c;
   const bool active = j < g.nx * g.kz;
   const int jj = active ? j : 0;
   const int x = jj / g.kz, iz = jj - x * g.kz;
   const T kzv = wavenumber<T>(g.k0z, iz);
   LdsStrided<N, TC, T> L{lds + c};
+  cpx<T> tw[R];
+  load_twiddles<N>(tw, q);
   int irow[R];
   T kyv[R];
 #pragma unroll
@@ -152,20 +158,20 @@ __global__ __launch_bounds__(kThreads) void k_y_inverse_curl(const cpx<T>* __res
   load(2, b);
 #pragma unroll
   for (int r = 0; r < R; ++r) w[r] = mul_i(kyv[r] * b[r] - kzv * a[r]);
-  pencil_fft<N, +1>(w, q, L);
+  pencil_fft_tw<N, +1>(w, q, L, tw);
   store(3, w);
-  pencil_fft<N, +1>(b, q, L);
+  pencil_fft_tw<N, +1>(b, q, L, tw);
   store(2, b);
-  pencil_fft<N, +1>(a, q, L);
+  pencil_fft_tw<N, +1>(a, q, L, tw);
   store(1, a);
   load(0, a);
-  pencil_fft<N, +1>(a, q, L);
+  pencil_fft_tw<N, +1>(a, q, L, tw);
   store(0, a);
   load(3, a);
-  pencil_fft<N, +1>(a, q, L);
+  pencil_fft_tw<N, +1>(a, q, L, tw);
   store(4, a);
   load(4, a);
-  pencil_fft<N, +1>(a, q, L);
+  pencil_fft_tw<N, +1>(a, q, L, tw);
   store(5, a);
 }
 
@@ -202,7 +208,7 @@ __device__ __forceinline__ void split_pair(cpx<T> ak, cpx<T> bk, cpx<T>& X, cpx<
 // overwrites fields 0..2 of the same pencils.
 // ---------------------------------------------------------------------------
 template <int N, class T>
-__global__ __launch_bounds__(kThreads) void k_z_fused(cpx<T>* buf, int64_t fs, int64_t npencils,
+__global__ __launch_bounds__(kZThreads) void k_z_fused(cpx<T>* buf, int64_t fs, int64_t npencils,
                                                      int kz, T scale) {
   using B = PencilBlock<N>;
   constexpr int R = B::R, Q = B::Q, PW = B::PW;
@@ -214,6 +220,8 @@ __global__ __launch_bounds__(kThreads) void k_z_fused(cpx<T>* buf, int64_t fs, i
   cpx<T>* lp = lds + p * lds_pencil_elems<N>();
   LdsPencil<N, T> L{lp};
   const int kzl = active ? kz : 0;  // inactive pencils load nothing
+  cpx<T> tw[R];
+  load_twiddles<N>(tw, q);
   cpx<T> z0[R], z1[R], z2[R];
   cpx<T>* zz[3] = {z0, z1, z2};
 #pragma unroll
@@ -221,7 +229,7 @@ __global__ __launch_bounds__(kThreads) void k_z_fused(cpx<T>* buf, int64_t fs, i
 #pragma unroll
     for (int r = 0; r < R; ++r)
       zz[c][r] = herm_pair<N, T>(buf + c * fs + base, buf + (3 + c) * fs + base, q + Q * r, kzl);
-    pencil_fft<N, +1>(zz[c], q, L);
+    pencil_fft_tw<N, +1>(zz[c], q, L, tw);
   }
   // physical space: z_c = u_c + i w_c at position out_index(q, s)
   cpx<T> A[R], Bv[R];
@@ -233,8 +241,8 @@ __global__ __launch_bounds__(kThreads) void k_z_fused(cpx<T>* buf, int64_t fs, i
     A[r] = {(u1 * w2 - u2 * w1) * scale, (u2 * w0 - u0 * w2) * scale};
     Bv[r] = {(u0 * w1 - u1 * w0) * scale, T(0)};
   }
-  pencil_fft<N, -1>(Bv, q, L);
-  pencil_fft<N, -1>(A, q, L);
+  pencil_fft_tw<N, -1>(Bv, q, L, tw);
+  pencil_fft_tw<N, -1>(A, q, L, tw);
   if (active) {
 #pragma unroll
     for (int s = 0; s < R; ++s) {
@@ -258,7 +266,7 @@ __global__ __launch_bounds__(kThreads) void k_z_fused(cpx<T>* buf, int64_t fs, i
 
 // Standalone z c2r of nf fields, two per complex transform.
 template <int N, class T>
-__global__ __launch_bounds__(kThreads) void k_z_c2r(const cpx<T>* __restrict__ in, int64_t in_fs,
+__global__ __launch_bounds__(kZThreads) void k_z_c2r(const cpx<T>* __restrict__ in, int64_t in_fs,
                                                    T* __restrict__ out, int64_t out_fs, int nf,
                                                    int64_t npencils, int kz, T scale) {
   using B = PencilBlock<N>;
@@ -270,6 +278,8 @@ __global__ __launch_bounds__(kThreads) void k_z_c2r(const cpx<T>* __restrict__ i
   const int64_t ib = (active ? pp : 0) * kz, ob = (active ? pp : 0) * N;
   LdsPencil<N, T> L{lds + p * lds_pencil_elems<N>()};
   const int kzl = active ? kz : 0;
+  cpx<T> tw[R];
+  load_twiddles<N>(tw, q);
   for (int f = 0; f < nf; f += 2) {
     const bool two = f + 1 < nf;
     cpx<T> z[R];
@@ -277,7 +287,7 @@ __global__ __launch_bounds__(kThreads) void k_z_c2r(const cpx<T>* __restrict__ i
     for (int r = 0; r < R; ++r)
       z[r] = herm_pair<N, T>(in + f * in_fs + ib, two ? in + (f + 1) * in_fs + ib : nullptr,
                              q + Q * r, kzl);
-    pencil_fft<N, +1>(z, q, L);
+    pencil_fft_tw<N, +1>(z, q, L, tw);
     if (active) {
 #pragma unroll
       for (int s = 0; s < R; ++s) {
@@ -291,7 +301,7 @@ __global__ __launch_bounds__(kThreads) void k_z_c2r(const cpx<T>* __restrict__ i
 
 // Standalone z r2c of nf real fields, two per complex transform.
 template <int N, class T>
-__global__ __launch_bounds__(kThreads) void k_z_r2c(const T* __restrict__ in, int64_t in_fs,
+__global__ __launch_bounds__(kZThreads) void k_z_r2c(const T* __restrict__ in, int64_t in_fs,
                                                    cpx<T>* __restrict__ out, int64_t out_fs,
                                                    int nf, int64_t npencils, int kz) {
   using B = PencilBlock<N>;
@@ -303,6 +313,8 @@ __global__ __launch_bounds__(kThreads) void k_z_r2c(const T* __restrict__ in, in
   const int64_t ib = (active ? pp : 0) * N, ob = (active ? pp : 0) * kz;
   cpx<T>* lp = lds + p * lds_pencil_elems<N>();
   LdsPencil<N, T> L{lp};
+  cpx<T> tw[R];
+  load_twiddles<N>(tw, q);
   for (int f = 0; f < nf; f += 2) {
     const bool two = f + 1 < nf;
     cpx<T> z[R];
@@ -312,7 +324,7 @@ __global__ __launch_bounds__(kThreads) void k_z_r2c(const T* __restrict__ in, in
       z[r] = {active ? in[f * in_fs + ib + n] : T(0),
               (active && two) ? in[(f + 1) * in_fs + ib + n] : T(0)};
     }
-    pencil_fft<N, -1>(z, q, L);
+    pencil_fft_tw<N, -1>(z, q, L, tw);
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < R; ++s) lp[out_index<N>(q, s)] = z[s];
@@ -370,7 +382,7 @@ __global__ __launch_bounds__(kThreads) void k_x_forward_stage(StageArgs<T> a) {
   __shared__ cpx<T> lds[S::LDS];
   const Geo& g = a.g;
   const int c = threadIdx.x % TC, q = threadIdx.x / TC;
-  const int j = blockIdx.x * TC + c;
+  const int j = xcd_tile(blockIdx.x, gridDim.x) * TC + c;
   const bool active = j < g.ky * g.kz;
   const int jj = active ? j : 0;
   const int iy = jj / g.kz, iz = jj - iy * g.kz;
@@ -378,6 +390,8 @@ __global__ __launch_bounds__(kThreads) void k_x_forward_stage(StageArgs<T> a) {
   const T kyv = wavenumber<T>(g.k0y, ny);
   const T kzv = wavenumber<T>(g.k0z, iz);
   LdsStrided<N, TC, T> L{lds + c};
+  cpx<T> tw[R];
+  load_twiddles<N>(tw, q);
   cpx<T> v0[R], v1[R], v2[R];
   cpx<T>* vv[3] = {v0, v1, v2};
 #pragma unroll
@@ -385,7 +399,7 @@ __global__ __launch_bounds__(kThreads) void k_x_forward_stage(StageArgs<T> a) {
     const cpx<T>* src = a.nl + f * a.nl_fs + jj;
 #pragma unroll
     for (int r = 0; r < R; ++r) vv[f][r] = active ? src[int64_t(q + Q * r) * g.jp] : czero<T>();
-    pencil_fft<N, -1>(vv[f], q, L);
+    pencil_fft_tw<N, -1>(vv[f], q, L, tw);
   }
   T coef = a.coef_const;
   if (a.fterm != kTermNone && a.coef_dev) coef = T(*a.coef_dev);
@@ -507,7 +521,7 @@ hipError_t launch_z_fused(int N, cpx<T>* buf, int64_t fs, int64_t npencils, int 
                           hipStream_t s) {
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(npencils, PencilBlock<NN>::PW);
-    hipLaunchKernelGGL((k_z_fused<NN, T>), dim3(nb), dim3(kThreads), 0, s, buf, fs, npencils, kz,
+    hipLaunchKernelGGL((k_z_fused<NN, T>), dim3(nb), dim3(kZThreads), 0, s, buf, fs, npencils, kz,
                        scale);
   });
   return hipGetLastError();
@@ -518,7 +532,7 @@ hipError_t launch_z_c2r(int N, const cpx<T>* in, int64_t in_fs, T* out, int64_t 
                         int64_t npencils, int kz, T scale, hipStream_t s) {
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(npencils, PencilBlock<NN>::PW);
-    hipLaunchKernelGGL((k_z_c2r<NN, T>), dim3(nb), dim3(kThreads), 0, s, in, in_fs, out, out_fs,
+    hipLaunchKernelGGL((k_z_c2r<NN, T>), dim3(nb), dim3(kZThreads), 0, s, in, in_fs, out, out_fs,
                        nf, npencils, kz, scale);
   });
   return hipGetLastError();
@@ -529,7 +543,7 @@ hipError_t launch_z_r2c(int N, const T* in, int64_t in_fs, cpx<T>* out, int64_t 
                         int64_t npencils, int kz, hipStream_t s) {
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(npencils, PencilBlock<NN>::PW);
-    hipLaunchKernelGGL((k_z_r2c<NN, T>), dim3(nb), dim3(kThreads), 0, s, in, in_fs, out, out_fs,
+    hipLaunchKernelGGL((k_z_r2c<NN, T>), dim3(nb), dim3(kZThreads), 0, s, in, in_fs, out, out_fs,
                        nf, npencils, kz);
   });
   return hipGetLastError();

@@ -5,7 +5,8 @@
 
 namespace h3 {
 
-constexpr int kThreads = 256;
+constexpr int kThreads = 256;   // strided passes: 4 waves
+constexpr int kZThreads = 64;   // z passes: one wave, so the pencil exchanges need no cross-wave barrier
 
 template <int N>
 struct StridedShape {
@@ -16,7 +17,7 @@ struct StridedShape {
 template <int N>
 struct PencilBlock {
   static constexpr int R = PencilShape<N>::R, Q = PencilShape<N>::Q;
-  static constexpr int PW = kThreads / Q;  // pencils per workgroup
+  static constexpr int PW = kZThreads / Q;  // pencils per (one-wave) workgroup
   static constexpr int LDS = PW * lds_pencil_elems<N>();
 };
 

@@ -80,6 +80,9 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run 3 device copies of 1 GiB (reference kernel for the "
+                         "FETCH_SIZE / WRITE_SIZE counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU slab "
@@ -89,6 +92,14 @@ def main():
     import torch
     import hit3d as h3
     from hit3d import _lib
+
+    if args.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     N, nu, dt = args.n, 1.0 / 1000, 1e-3
     g = h3.make_grid(N, dtype=torch.float32)
