@@ -115,78 +115,178 @@ __device__ __forceinline__ cpx<T> twiddle(int j) {
 }
 
 // ---------------------------------------------------------------------------
-// Distributed pencil FFT of length N = R*Q over Q threads (one "column").
-// Four-step split inside the workgroup: thread q holds x[q + Q r], r < R;
-// it does the R-point DFT over r in registers, multiplies by w_N^{q k1},
-// exchanges through LDS, then does R/Q Q-point DFTs.  On exit thread t holds
-//     v[m*Q + k2] = X[(t + Q m) + R k2],   m < R/Q, k2 < Q.
-// `Lds` maps (k1, q) of this thread's column to an LDS slot.
-// The function starts with a barrier so the LDS buffer may be reused by
-// consecutive calls; every thread of the block must call it.
+// Distributed pencil FFTs. A pencil of length N is spread over Q threads
+// (its "column"), each holding R = N/Q values in registers; the threads
+// exchange through LDS slots of the column (`slot(i)` maps slot i of this
+// column to LDS). Two shapes, chosen by N (FftFor<N>):
+//
+// Fft2 (N <= 256): N = R*Q, R = 2^ceil(log2(N)/2), Q = N/R.  Thread q enters
+//   with v[r] = x[q + Q r]; R-point DFT over r in registers, twiddle
+//   w_N^{q k1}, one exchange, then R/Q Q-point DFTs.  Exit: thread t holds
+//   v[m*Q + k2] = X[(t + Q m) + R k2].
+// Fft3<N, Q> (Q threads, R = N/Q, needs R*R >= Q): thread q enters with
+//   v[r] = x[q + Q r]; R-point DFT over r, twiddle w_N^{q k1}; the Q-point
+//   DFT over q is split again as q = q1 + R q2 (Q2 = Q/R points over q2,
+//   twiddle w_Q^{q1 j}, then R points over q1), with two exchanges.  Exit:
+//   thread t holds v[s] = X[t + Q s].  Used with Q = 64 (one wave) for
+//   N >= 512, and with Q = 32 for the 256-point z passes.
+// In both, run() starts with a barrier (Sync) so consecutive calls may reuse
+// the slots, and every thread of the block must call it.
 // ---------------------------------------------------------------------------
 
+// Barriers around the LDS exchanges. Multi-wave blocks need the workgroup
+// barrier; a one-wave block only needs the compiler to keep its LDS accesses
+// in program order, since the LDS executes one wave's operations in order.
+struct BlockSync {
+  __device__ static __forceinline__ void sync() { __syncthreads(); }
+};
+struct WaveSync {
+  __device__ static __forceinline__ void sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+};
+
+template <class T, int DIR>
+__device__ __forceinline__ cpx<T> apply_tw(cpx<T> v, cpx<T> w) {  // w has the forward sign
+  if constexpr (DIR > 0) w.y = -w.y;
+  return cmul(v, w);
+}
+
 template <int N>
-struct PencilShape {
+struct Fft2 {
   static constexpr int LOG = ilog2(N);
   static constexpr int R = 1 << ((LOG + 1) / 2);
   static constexpr int Q = N / R;
   static constexpr int M = R / Q;  // 1 or 2
+  static constexpr int kSlots = R * (Q + 1);
   static_assert(R * Q == N && (M == 1 || M == 2), "N must be a power of two");
+
+  __device__ static __forceinline__ int out_index(int t, int s) {
+    return t + Q * (s / Q) + R * (s % Q);
+  }
+  // exit slot s, seen as the entry slot r of a following transform on the
+  // same thread (entry position q + Q r)
+  static constexpr int out_slot_as_input(int s) { return (s / Q) + (R / Q) * (s % Q); }
+
+  template <class T>
+  __device__ static __forceinline__ void load_tw(cpx<T>* tw, int q) {
+#pragma unroll
+    for (int k1 = 0; k1 < R; ++k1) tw[k1] = twiddle<N, -1, T>(q * k1);
+  }
+
+  template <int DIR, class Sync = BlockSync, class T, class Slot>
+  __device__ static __forceinline__ void run(cpx<T>* v, int q, Slot slot, const cpx<T>* tw) {
+    dft<R, DIR>(v);
+#pragma unroll
+    for (int k1 = 1; k1 < R; ++k1) v[k1] = apply_tw<T, DIR>(v[k1], tw[k1]);
+    Sync::sync();
+#pragma unroll
+    for (int k1 = 0; k1 < R; ++k1) *slot(k1 * (Q + 1) + q) = v[k1];
+    Sync::sync();
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      cpx<T> u[Q];
+#pragma unroll
+      for (int qq = 0; qq < Q; ++qq) u[qq] = *slot((q + Q * m) * (Q + 1) + qq);
+      dft<Q, DIR>(u);
+#pragma unroll
+      for (int k2 = 0; k2 < Q; ++k2) v[m * Q + k2] = u[k2];
+    }
+  }
 };
 
-// The inter-step twiddles w_N^{-q k1} (forward sign) of thread q, k1 < R,
-// loaded once when a thread runs several transforms.
-template <int N, class T>
-__device__ __forceinline__ void load_twiddles(cpx<T>* tw, int q) {
-#pragma unroll
-  for (int k1 = 0; k1 < PencilShape<N>::R; ++k1) tw[k1] = twiddle<N, -1, T>(q * k1);
-}
+template <int N, int QT = 64>
+struct Fft3 {
+  static constexpr int Q = QT, R = N / QT;
+  static constexpr int Q2 = QT / R;       // points of the middle DFT (over q2)
+  static constexpr int M1 = R / Q2;       // middle DFTs per thread
+  static constexpr int E1 = QT + 1;       // k1 stride of the first exchange
+  static constexpr int kSlots = (R * E1 > QT * (R + 1)) ? R * E1 : QT * (R + 1);
+  static_assert(R * Q2 == QT && Q2 >= 1 && Q2 <= R && R <= 64, "Fft3 needs R >= Q/R");
 
-template <int N, int DIR, class T, class Lds>
-__device__ __forceinline__ void pencil_fft_tw(cpx<T>* v, int q, Lds lds, const cpx<T>* tw) {
-  constexpr int R = PencilShape<N>::R, Q = PencilShape<N>::Q, M = PencilShape<N>::M;
-  dft<R, DIR>(v);
-#pragma unroll
-  for (int k1 = 1; k1 < R; ++k1) {
-    cpx<T> w = tw[k1];
-    if constexpr (DIR > 0) w.y = -w.y;
-    v[k1] = cmul(v[k1], w);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k1 = 0; k1 < R; ++k1) *lds(k1, q) = v[k1];
-  __syncthreads();
-#pragma unroll
-  for (int m = 0; m < M; ++m) {
-    cpx<T> u[Q];
-#pragma unroll
-    for (int qq = 0; qq < Q; ++qq) u[qq] = *lds(q + Q * m, qq);
-    dft<Q, DIR>(u);
-#pragma unroll
-    for (int k2 = 0; k2 < Q; ++k2) v[m * Q + k2] = u[k2];
-  }
-}
+  __device__ static __forceinline__ int out_index(int t, int s) { return t + QT * s; }
+  static constexpr int out_slot_as_input(int s) { return s; }
 
-template <int N, int DIR, class T, class Lds>
-__device__ __forceinline__ void pencil_fft(cpx<T>* v, int q, Lds lds) {
-  constexpr int R = PencilShape<N>::R, Q = PencilShape<N>::Q, M = PencilShape<N>::M;
-  dft<R, DIR>(v);
+  template <class T>
+  __device__ static __forceinline__ void load_tw(cpx<T>* tw, int q) {
 #pragma unroll
-  for (int k1 = 1; k1 < R; ++k1) v[k1] = cmul(v[k1], twiddle<N, DIR, T>(q * k1));
-  __syncthreads();
-#pragma unroll
-  for (int k1 = 0; k1 < R; ++k1) *lds(k1, q) = v[k1];
-  __syncthreads();
-#pragma unroll
-  for (int m = 0; m < M; ++m) {
-    cpx<T> u[Q];
-#pragma unroll
-    for (int qq = 0; qq < Q; ++qq) u[qq] = *lds(q + Q * m, qq);
-    dft<Q, DIR>(u);
-#pragma unroll
-    for (int k2 = 0; k2 < Q; ++k2) v[m * Q + k2] = u[k2];
+    for (int k1 = 0; k1 < R; ++k1) tw[k1] = twiddle<N, -1, T>(q * k1);
   }
-}
+
+  template <int DIR, class Sync = BlockSync, class T, class Slot>
+  __device__ static __forceinline__ void run(cpx<T>* v, int q, Slot slot, const cpx<T>* tw) {
+    dft<R, DIR>(v);
+#pragma unroll
+    for (int k1 = 1; k1 < R; ++k1) v[k1] = apply_tw<T, DIR>(v[k1], tw[k1]);
+    Sync::sync();
+#pragma unroll
+    for (int k1 = 0; k1 < R; ++k1) *slot(k1 * E1 + q) = v[k1];
+    Sync::sync();
+    // middle DFTs: this thread takes k1 = q % R and q1 = q / R + Q2 m
+    const int k1 = q % R, g = q / R;
+    cpx<T> u[M1][Q2];
+#pragma unroll
+    for (int m = 0; m < M1; ++m)
+#pragma unroll
+      for (int q2 = 0; q2 < Q2; ++q2) u[m][q2] = *slot(k1 * E1 + (g + Q2 * m) + R * q2);
+    Sync::sync();
+#pragma unroll
+    for (int m = 0; m < M1; ++m) {
+      const int q1 = g + Q2 * m;
+      dft<Q2, DIR>(u[m]);
+#pragma unroll
+      for (int j = 0; j < Q2; ++j) {
+        const cpx<T> w = j ? apply_tw<T, DIR>(u[m][j], twiddle<QT, -1, T>(q1 * j)) : u[m][j];
+        *slot((k1 + R * j) * (R + 1) + q1) = w;
+      }
+    }
+    Sync::sync();
+    // last DFT over q1: thread q = k1' + R j' holds X[q + Q s]
+#pragma unroll
+    for (int q1 = 0; q1 < R; ++q1) v[q1] = *slot(q * (R + 1) + q1);
+    dft<R, DIR>(v);
+  }
+};
+
+// strided passes: Fft2 up to 256 points (16 threads per column keeps 16
+// columns = 128-byte rows in a 256-thread block), one-wave Fft3 above
+template <int N, bool THREE = (N >= 512)>
+struct FftSel {
+  using type = Fft2<N>;
+};
+template <int N>
+struct FftSel<N, true> {
+  using type = Fft3<N>;
+};
+template <int N>
+using FftFor = typename FftSel<N>::type;
+
+// z passes (contiguous pencils): fewer values per thread at 256 points
+template <int N>
+struct FftZSel {
+  using type = FftFor<N>;
+};
+template <>
+struct FftZSel<256> {
+  using type = Fft3<256, 32>;
+};
+template <int N>
+using FftZFor = typename FftZSel<N>::type;
+
+// Slot maps. Strided passes interleave TC columns (threads tid = c + TC q);
+// z passes give each pencil its own run of slots (threads tid = q + Q p).
+template <int TC, class T>
+struct SlotStrided {
+  cpx<T>* base;  // already offset by column c
+  __device__ __forceinline__ cpx<T>* operator()(int i) const { return base + i * TC; }
+};
+template <class T>
+struct SlotPencil {
+  cpx<T>* base;  // already offset by pencil p
+  __device__ __forceinline__ cpx<T>* operator()(int i) const { return base + i; }
+};
 
 // Strided passes tile columns across workgroups, and neighbouring tiles share
 // the cache lines at their edges. Workgroups are dealt round-robin to the 8
@@ -197,45 +297,6 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   const int q = nb / X, r = nb % X;
   const int x = b % X, i = b / X;
   return x * q + min(x, r) + i;
-}
-
-// Output slot s = m*Q + k2 of pencil_fft holds frequency/position
-//     (t + Q m) + R k2.
-template <int N>
-__device__ __forceinline__ int out_index(int t, int s) {
-  constexpr int R = PencilShape<N>::R, Q = PencilShape<N>::Q;
-  return t + Q * (s / Q) + R * (s % Q);
-}
-
-// pencil_fft's output slot s, seen as the input slot r of a following
-// pencil_fft on the same thread (input position q + Q r).
-template <int N>
-constexpr int out_slot_as_input(int s) {
-  constexpr int R = PencilShape<N>::R, Q = PencilShape<N>::Q;
-  return (s / Q) + (R / Q) * (s % Q);
-}
-
-// LDS slot maps.  Strided passes: T columns interleaved, threads tid = c + T q.
-template <int N, int T_COLS, class T>
-struct LdsStrided {
-  cpx<T>* base;  // already offset by column c
-  __device__ __forceinline__ cpx<T>* operator()(int k1, int q) const {
-    constexpr int Q = PencilShape<N>::Q;
-    return base + (k1 * (Q + 1) + q) * T_COLS;
-  }
-};
-// Contiguous (z) passes: one padded slab per pencil, threads tid = q + Q p.
-template <int N, class T>
-struct LdsPencil {
-  cpx<T>* base;  // already offset by pencil p
-  __device__ __forceinline__ cpx<T>* operator()(int k1, int q) const {
-    constexpr int Q = PencilShape<N>::Q;
-    return base + k1 * (Q + 1) + q;
-  }
-};
-template <int N>
-constexpr int lds_pencil_elems() {
-  return PencilShape<N>::R * (PencilShape<N>::Q + 1);
 }
 
 // ---------------------------------------------------------------------------

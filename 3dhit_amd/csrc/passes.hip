@@ -16,8 +16,9 @@ namespace h3 {
 // Plain strided pass (any axis, either direction)
 // ---------------------------------------------------------------------------
 template <int N, int DIR, class T>
-__global__ __launch_bounds__(kThreads) void k_strided(StridedArgs<T> a) {
+__global__ __launch_bounds__(StridedShape<N>::THREADS) void k_strided(StridedArgs<T> a) {
   using S = StridedShape<N>;
+  using F = typename S::F;
   constexpr int R = S::R, Q = S::Q, TC = S::TC;
   __shared__ cpx<T> lds[S::LDS];
   const int c = threadIdx.x % TC, q = threadIdx.x / TC;
@@ -27,22 +28,31 @@ __global__ __launch_bounds__(kThreads) void k_strided(StridedArgs<T> a) {
   const int ao = jj / a.B, bo = jj - ao * a.B;
   const cpx<T>* in0 = a.in + ao * a.in_slab + bo;
   cpx<T>* out0 = a.out + ao * a.out_slab + bo;
-  LdsStrided<N, TC, T> L{lds + c};
+  SlotStrided<TC, T> L{lds + c};
   cpx<T> tw[R];
-  load_twiddles<N>(tw, q);
+  F::load_tw(tw, q);
   int irow[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) irow[r] = active ? compact_of_row(q + Q * r, N, a.in_k, a.in_p) : -1;
-  for (int f = 0; f < a.nf; ++f) {
-    cpx<T> v[R];
+  auto load = [&](int f, cpx<T>* v) {
     const cpx<T>* src = in0 + f * a.in_fs;
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = irow[r] >= 0 ? src[int64_t(irow[r]) * a.B] : czero<T>();
-    pencil_fft_tw<N, DIR>(v, q, L, tw);
+  };
+  // field f+1 is loaded before field f's transform, whose barriers would
+  // otherwise hold the loads back
+  cpx<T> nxt[R];
+  load(0, nxt);
+  for (int f = 0; f < a.nf; ++f) {
+    cpx<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = nxt[r];
+    if (f + 1 < a.nf) load(f + 1, nxt);
+    F::template run<DIR>(v, q, L, tw);
     cpx<T>* dst = out0 + f * a.out_fs;
 #pragma unroll
     for (int s = 0; s < R; ++s) {
-      const int o = compact_of_row(out_index<N>(q, s), N, a.out_k, a.out_p);
+      const int o = compact_of_row(F::out_index(q, s), N, a.out_k, a.out_p);
       if (active && o >= 0) dst[int64_t(o) * a.B] = v[s];
     }
   }
@@ -54,11 +64,12 @@ __global__ __launch_bounds__(kThreads) void k_strided(StridedArgs<T> a) {
 // out fields: [u0, u1, u2, w1, w2]
 // ---------------------------------------------------------------------------
 template <int N, class T>
-__global__ __launch_bounds__(kThreads) void k_x_inverse_curl(const cpx<T>* __restrict__ us,
+__global__ __launch_bounds__(StridedShape<N>::THREADS) void k_x_inverse_curl(const cpx<T>* __restrict__ us,
                                                             int64_t us_fs,
                                                             cpx<T>* __restrict__ out,
                                                             int64_t out_fs, Geo g) {
   using S = StridedShape<N>;
+  using F = typename S::F;
   constexpr int R = S::R, Q = S::Q, TC = S::TC;
   __shared__ cpx<T> lds[S::LDS];
   const int c = threadIdx.x % TC, q = threadIdx.x / TC;
@@ -68,9 +79,9 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse_curl(const cpx<T>* __res
   const int iy = jj / g.kz, iz = jj - iy * g.kz;
   const T kyv = wavenumber<T>(g.k0y, freq_of(iy, g.ky, g.py));
   const T kzv = wavenumber<T>(g.k0z, iz);
-  LdsStrided<N, TC, T> L{lds + c};
+  SlotStrided<TC, T> L{lds + c};
   cpx<T> tw[R];
-  load_twiddles<N>(tw, q);
+  F::load_tw(tw, q);
   int irow[R];
   T kxv[R];
 #pragma unroll
@@ -89,25 +100,27 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse_curl(const cpx<T>* __res
     if (!active) return;
     cpx<T>* dst = out + f * out_fs + j;
 #pragma unroll
-    for (int s = 0; s < R; ++s) dst[int64_t(out_index<N>(q, s)) * g.jp] = v[s];
+    for (int s = 0; s < R; ++s) dst[int64_t(F::out_index(q, s)) * g.jp] = v[s];
   };
-  cpx<T> a[R], b[R], w[R];
+  // all three components are loaded up front (the barriers inside run()
+  // would otherwise delay every later load until the previous transform ends)
+  cpx<T> a[R], b[R], c1[R], w[R];
   load(u0, a);
   load(u2, b);
+  load(u1, c1);
 #pragma unroll
   for (int r = 0; r < R; ++r) w[r] = mul_i(kzv * a[r] - kxv[r] * b[r]);
-  pencil_fft_tw<N, +1>(w, q, L, tw);
+  F::template run<+1>(w, q, L, tw);
   store(3, w);
-  pencil_fft_tw<N, +1>(b, q, L, tw);
+  F::template run<+1>(b, q, L, tw);
   store(2, b);
-  load(u1, b);
 #pragma unroll
-  for (int r = 0; r < R; ++r) w[r] = mul_i(kxv[r] * b[r] - kyv * a[r]);
-  pencil_fft_tw<N, +1>(w, q, L, tw);
+  for (int r = 0; r < R; ++r) w[r] = mul_i(kxv[r] * c1[r] - kyv * a[r]);
+  F::template run<+1>(w, q, L, tw);
   store(4, w);
-  pencil_fft_tw<N, +1>(b, q, L, tw);
-  store(1, b);
-  pencil_fft_tw<N, +1>(a, q, L, tw);
+  F::template run<+1>(c1, q, L, tw);
+  store(1, c1);
+  F::template run<+1>(a, q, L, tw);
   store(0, a);
 }
 
@@ -117,11 +130,12 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse_curl(const cpx<T>* __res
 // out [6][Nx][Ny][Kz] fields (u0, u1, u2, w0, w1, w2)
 // ---------------------------------------------------------------------------
 template <int N, class T>
-__global__ __launch_bounds__(kThreads) void k_y_inverse_curl(const cpx<T>* __restrict__ in,
+__global__ __launch_bounds__(StridedShape<N>::THREADS) void k_y_inverse_curl(const cpx<T>* __restrict__ in,
                                                             int64_t in_fs,
                                                             cpx<T>* __restrict__ out,
                                                             int64_t out_fs, Geo g) {
   using S = StridedShape<N>;
+  using F = typename S::F;
   constexpr int R = S::R, Q = S::Q, TC = S::TC;
   __shared__ cpx<T> lds[S::LDS];
   const int c = threadIdx.x % TC, q = threadIdx.x / TC;
@@ -130,9 +144,9 @@ __global__ __launch_bounds__(kThreads) void k_y_inverse_curl(const cpx<T>* __res
   const int jj = active ? j : 0;
   const int x = jj / g.kz, iz = jj - x * g.kz;
   const T kzv = wavenumber<T>(g.k0z, iz);
-  LdsStrided<N, TC, T> L{lds + c};
+  SlotStrided<TC, T> L{lds + c};
   cpx<T> tw[R];
-  load_twiddles<N>(tw, q);
+  F::load_tw(tw, q);
   int irow[R];
   T kyv[R];
 #pragma unroll
@@ -151,27 +165,29 @@ __global__ __launch_bounds__(kThreads) void k_y_inverse_curl(const cpx<T>* __res
     if (!active) return;
     cpx<T>* p = dst0 + f * out_fs;
 #pragma unroll
-    for (int s = 0; s < R; ++s) p[out_index<N>(q, s) * g.kz] = v[s];
+    for (int s = 0; s < R; ++s) p[F::out_index(q, s) * g.kz] = v[s];
   };
-  cpx<T> a[R], b[R], w[R];
+  // each field's loads are issued one transform ahead of its use (pf holds
+  // the prefetched field), so the barriers inside run() hide their latency
+  cpx<T> a[R], b[R], pf[R], w[R];
   load(1, a);
   load(2, b);
+  load(0, pf);
 #pragma unroll
   for (int r = 0; r < R; ++r) w[r] = mul_i(kyv[r] * b[r] - kzv * a[r]);
-  pencil_fft_tw<N, +1>(w, q, L, tw);
+  F::template run<+1>(w, q, L, tw);
   store(3, w);
-  pencil_fft_tw<N, +1>(b, q, L, tw);
+  F::template run<+1>(b, q, L, tw);
   store(2, b);
-  pencil_fft_tw<N, +1>(a, q, L, tw);
+  load(3, b);
+  F::template run<+1>(a, q, L, tw);
   store(1, a);
-  load(0, a);
-  pencil_fft_tw<N, +1>(a, q, L, tw);
-  store(0, a);
-  load(3, a);
-  pencil_fft_tw<N, +1>(a, q, L, tw);
-  store(4, a);
   load(4, a);
-  pencil_fft_tw<N, +1>(a, q, L, tw);
+  F::template run<+1>(pf, q, L, tw);
+  store(0, pf);
+  F::template run<+1>(b, q, L, tw);
+  store(4, b);
+  F::template run<+1>(a, q, L, tw);
   store(5, a);
 }
 
@@ -208,20 +224,22 @@ __device__ __forceinline__ void split_pair(cpx<T> ak, cpx<T> bk, cpx<T>& X, cpx<
 // overwrites fields 0..2 of the same pencils.
 // ---------------------------------------------------------------------------
 template <int N, class T>
-__global__ __launch_bounds__(kZThreads) void k_z_fused(cpx<T>* buf, int64_t fs, int64_t npencils,
+__global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_fused(cpx<T>* buf, int64_t fs, int64_t npencils,
                                                      int kz, T scale) {
   using B = PencilBlock<N>;
+  using F = typename B::F;
+  static_assert(B::THREADS == 64, "WaveSync exchanges need one-wave blocks");
   constexpr int R = B::R, Q = B::Q, PW = B::PW;
   __shared__ cpx<T> lds[B::LDS];
   const int q = threadIdx.x % Q, p = threadIdx.x / Q;
   const int64_t pp = int64_t(blockIdx.x) * PW + p;
   const bool active = pp < npencils;
   const int64_t base = (active ? pp : 0) * kz;
-  cpx<T>* lp = lds + p * lds_pencil_elems<N>();
-  LdsPencil<N, T> L{lp};
+  cpx<T>* lp = lds + p * F::kSlots;
+  SlotPencil<T> L{lp};
   const int kzl = active ? kz : 0;  // inactive pencils load nothing
   cpx<T> tw[R];
-  load_twiddles<N>(tw, q);
+  F::load_tw(tw, q);
   cpx<T> z0[R], z1[R], z2[R];
   cpx<T>* zz[3] = {z0, z1, z2};
 #pragma unroll
@@ -229,7 +247,7 @@ __global__ __launch_bounds__(kZThreads) void k_z_fused(cpx<T>* buf, int64_t fs, 
 #pragma unroll
     for (int r = 0; r < R; ++r)
       zz[c][r] = herm_pair<N, T>(buf + c * fs + base, buf + (3 + c) * fs + base, q + Q * r, kzl);
-    pencil_fft_tw<N, +1>(zz[c], q, L, tw);
+    F::template run<+1, WaveSync>(zz[c], q, L, tw);
   }
   // physical space: z_c = u_c + i w_c at position out_index(q, s)
   cpx<T> A[R], Bv[R];
@@ -237,23 +255,23 @@ __global__ __launch_bounds__(kZThreads) void k_z_fused(cpx<T>* buf, int64_t fs, 
   for (int s = 0; s < R; ++s) {
     const T u0 = z0[s].x, u1 = z1[s].x, u2 = z2[s].x;
     const T w0 = z0[s].y, w1 = z1[s].y, w2 = z2[s].y;
-    const int r = out_slot_as_input<N>(s);
+    const int r = F::out_slot_as_input(s);
     A[r] = {(u1 * w2 - u2 * w1) * scale, (u2 * w0 - u0 * w2) * scale};
     Bv[r] = {(u0 * w1 - u1 * w0) * scale, T(0)};
   }
-  pencil_fft_tw<N, -1>(Bv, q, L, tw);
-  pencil_fft_tw<N, -1>(A, q, L, tw);
+  F::template run<-1, WaveSync>(Bv, q, L, tw);
+  F::template run<-1, WaveSync>(A, q, L, tw);
   if (active) {
 #pragma unroll
     for (int s = 0; s < R; ++s) {
-      const int k = out_index<N>(q, s);
+      const int k = F::out_index(q, s);
       if (k < kz) buf[2 * fs + base + k] = Bv[s];
     }
   }
-  __syncthreads();
+  WaveSync::sync();
 #pragma unroll
-  for (int s = 0; s < R; ++s) lp[out_index<N>(q, s)] = A[s];
-  __syncthreads();
+  for (int s = 0; s < R; ++s) lp[F::out_index(q, s)] = A[s];
+  WaveSync::sync();
   if (active) {
     for (int k = q; k < kz; k += Q) {
       cpx<T> X, Y;
@@ -266,20 +284,22 @@ __global__ __launch_bounds__(kZThreads) void k_z_fused(cpx<T>* buf, int64_t fs, 
 
 // Standalone z c2r of nf fields, two per complex transform.
 template <int N, class T>
-__global__ __launch_bounds__(kZThreads) void k_z_c2r(const cpx<T>* __restrict__ in, int64_t in_fs,
+__global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_c2r(const cpx<T>* __restrict__ in, int64_t in_fs,
                                                    T* __restrict__ out, int64_t out_fs, int nf,
                                                    int64_t npencils, int kz, T scale) {
   using B = PencilBlock<N>;
+  using F = typename B::F;
+  static_assert(B::THREADS == 64, "WaveSync exchanges need one-wave blocks");
   constexpr int R = B::R, Q = B::Q, PW = B::PW;
   __shared__ cpx<T> lds[B::LDS];
   const int q = threadIdx.x % Q, p = threadIdx.x / Q;
   const int64_t pp = int64_t(blockIdx.x) * PW + p;
   const bool active = pp < npencils;
   const int64_t ib = (active ? pp : 0) * kz, ob = (active ? pp : 0) * N;
-  LdsPencil<N, T> L{lds + p * lds_pencil_elems<N>()};
+  SlotPencil<T> L{lds + p * F::kSlots};
   const int kzl = active ? kz : 0;
   cpx<T> tw[R];
-  load_twiddles<N>(tw, q);
+  F::load_tw(tw, q);
   for (int f = 0; f < nf; f += 2) {
     const bool two = f + 1 < nf;
     cpx<T> z[R];
@@ -287,11 +307,11 @@ __global__ __launch_bounds__(kZThreads) void k_z_c2r(const cpx<T>* __restrict__ 
     for (int r = 0; r < R; ++r)
       z[r] = herm_pair<N, T>(in + f * in_fs + ib, two ? in + (f + 1) * in_fs + ib : nullptr,
                              q + Q * r, kzl);
-    pencil_fft_tw<N, +1>(z, q, L, tw);
+    F::template run<+1, WaveSync>(z, q, L, tw);
     if (active) {
 #pragma unroll
       for (int s = 0; s < R; ++s) {
-        const int n = out_index<N>(q, s);
+        const int n = F::out_index(q, s);
         out[f * out_fs + ob + n] = z[s].x * scale;
         if (two) out[(f + 1) * out_fs + ob + n] = z[s].y * scale;
       }
@@ -301,20 +321,22 @@ __global__ __launch_bounds__(kZThreads) void k_z_c2r(const cpx<T>* __restrict__ 
 
 // Standalone z r2c of nf real fields, two per complex transform.
 template <int N, class T>
-__global__ __launch_bounds__(kZThreads) void k_z_r2c(const T* __restrict__ in, int64_t in_fs,
+__global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_r2c(const T* __restrict__ in, int64_t in_fs,
                                                    cpx<T>* __restrict__ out, int64_t out_fs,
                                                    int nf, int64_t npencils, int kz) {
   using B = PencilBlock<N>;
+  using F = typename B::F;
+  static_assert(B::THREADS == 64, "WaveSync exchanges need one-wave blocks");
   constexpr int R = B::R, Q = B::Q, PW = B::PW;
   __shared__ cpx<T> lds[B::LDS];
   const int q = threadIdx.x % Q, p = threadIdx.x / Q;
   const int64_t pp = int64_t(blockIdx.x) * PW + p;
   const bool active = pp < npencils;
   const int64_t ib = (active ? pp : 0) * N, ob = (active ? pp : 0) * kz;
-  cpx<T>* lp = lds + p * lds_pencil_elems<N>();
-  LdsPencil<N, T> L{lp};
+  cpx<T>* lp = lds + p * F::kSlots;
+  SlotPencil<T> L{lp};
   cpx<T> tw[R];
-  load_twiddles<N>(tw, q);
+  F::load_tw(tw, q);
   for (int f = 0; f < nf; f += 2) {
     const bool two = f + 1 < nf;
     cpx<T> z[R];
@@ -324,11 +346,11 @@ __global__ __launch_bounds__(kZThreads) void k_z_r2c(const T* __restrict__ in, i
       z[r] = {active ? in[f * in_fs + ib + n] : T(0),
               (active && two) ? in[(f + 1) * in_fs + ib + n] : T(0)};
     }
-    pencil_fft_tw<N, -1>(z, q, L, tw);
-    __syncthreads();
+    F::template run<-1, WaveSync>(z, q, L, tw);
+    WaveSync::sync();
 #pragma unroll
-    for (int s = 0; s < R; ++s) lp[out_index<N>(q, s)] = z[s];
-    __syncthreads();
+    for (int s = 0; s < R; ++s) lp[F::out_index(q, s)] = z[s];
+    WaveSync::sync();
     if (active) {
       for (int k = q; k < kz; k += Q) {
         cpx<T> X, Y;
@@ -341,7 +363,7 @@ __global__ __launch_bounds__(kZThreads) void k_z_r2c(const T* __restrict__ in, i
 }
 
 // ---------------------------------------------------------------------------
-// Deterministic block reduction of three doubles (blockDim == 256):
+// Deterministic block reduction of three doubles (blockDim a multiple of 64, <= 1024):
 // xor-butterfly inside each wave, then the four wave sums in fixed order.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void block_sum3(double a, double b, double c, double* dst) {
@@ -351,8 +373,8 @@ __device__ __forceinline__ void block_sum3(double a, double b, double c, double*
     b += __shfl_xor(b, off);
     c += __shfl_xor(c, off);
   }
-  __shared__ double sh[3][kThreads / 64];
-  const int w = threadIdx.x / 64;
+  __shared__ double sh[3][16];
+  const int w = threadIdx.x / 64, nw = blockDim.x / 64;
   if ((threadIdx.x & 63) == 0) {
     sh[0][w] = a;
     sh[1][w] = b;
@@ -360,11 +382,9 @@ __device__ __forceinline__ void block_sum3(double a, double b, double c, double*
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-#pragma unroll
     for (int i = 0; i < 3; ++i) {
       double t = sh[i][0];
-#pragma unroll
-      for (int k = 1; k < kThreads / 64; ++k) t += sh[i][k];
+      for (int k = 1; k < nw; ++k) t += sh[i][k];
       dst[i] = t;
     }
   }
@@ -376,8 +396,9 @@ __device__ __forceinline__ void block_sum3(double a, double b, double c, double*
 // and the RK4 stage update (integrators.py:34-38).
 // ---------------------------------------------------------------------------
 template <int N, int MODE, class T>
-__global__ __launch_bounds__(kThreads) void k_x_forward_stage(StageArgs<T> a) {
+__global__ __launch_bounds__(StridedShape<N>::THREADS) void k_x_forward_stage(StageArgs<T> a) {
   using S = StridedShape<N>;
+  using F = typename S::F;
   constexpr int R = S::R, Q = S::Q, TC = S::TC;
   __shared__ cpx<T> lds[S::LDS];
   const Geo& g = a.g;
@@ -389,18 +410,20 @@ __global__ __launch_bounds__(kThreads) void k_x_forward_stage(StageArgs<T> a) {
   const int ny = freq_of(iy, g.ky, g.py);
   const T kyv = wavenumber<T>(g.k0y, ny);
   const T kzv = wavenumber<T>(g.k0z, iz);
-  LdsStrided<N, TC, T> L{lds + c};
+  SlotStrided<TC, T> L{lds + c};
   cpx<T> tw[R];
-  load_twiddles<N>(tw, q);
+  F::load_tw(tw, q);
   cpx<T> v0[R], v1[R], v2[R];
   cpx<T>* vv[3] = {v0, v1, v2};
+  // all three fields are loaded before the first transform (see k_z_fused)
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
     const cpx<T>* src = a.nl + f * a.nl_fs + jj;
 #pragma unroll
     for (int r = 0; r < R; ++r) vv[f][r] = active ? src[int64_t(q + Q * r) * g.jp] : czero<T>();
-    pencil_fft_tw<N, -1>(vv[f], q, L, tw);
   }
+#pragma unroll
+  for (int f = 0; f < 3; ++f) F::template run<-1>(vv[f], q, L, tw);
   T coef = a.coef_const;
   if (a.fterm != kTermNone && a.coef_dev) coef = T(*a.coef_dev);
   const int wgt = hermitian_weight(iz, g.nz);
@@ -408,7 +431,7 @@ __global__ __launch_bounds__(kThreads) void k_x_forward_stage(StageArgs<T> a) {
   const T hdt = T(0.5) * a.dt, sdt = a.dt / T(6), tdt = a.dt / T(3);
 #pragma unroll
   for (int s = 0; s < R; ++s) {
-    const int ix = compact_of_row(out_index<N>(q, s), N, g.kx, g.px);
+    const int ix = compact_of_row(F::out_index(q, s), N, g.kx, g.px);
     if (!active || ix < 0) continue;
     const int nx = freq_of(ix, g.kx, g.px);
     const T kxv = wavenumber<T>(g.k0x, nx);
@@ -488,8 +511,8 @@ hipError_t launch_strided(int N, int dir, const StridedArgs<T>& a, hipStream_t s
   if (a.J <= 0 || a.nf <= 0) return hipSuccess;
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(a.J, StridedShape<NN>::TC);
-    if (dir < 0) hipLaunchKernelGGL((k_strided<NN, -1, T>), dim3(nb), dim3(kThreads), 0, s, a);
-    else hipLaunchKernelGGL((k_strided<NN, +1, T>), dim3(nb), dim3(kThreads), 0, s, a);
+    if (dir < 0) hipLaunchKernelGGL((k_strided<NN, -1, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a);
+    else hipLaunchKernelGGL((k_strided<NN, +1, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a);
   });
   return hipGetLastError();
 }
@@ -499,7 +522,7 @@ hipError_t launch_x_inverse_curl(int N, const cpx<T>* us, int64_t us_fs, cpx<T>*
                                  int64_t out_fs, const Geo& g, hipStream_t s) {
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(int64_t(g.ky) * g.kz, StridedShape<NN>::TC);
-    hipLaunchKernelGGL((k_x_inverse_curl<NN, T>), dim3(nb), dim3(kThreads), 0, s, us, us_fs, out,
+    hipLaunchKernelGGL((k_x_inverse_curl<NN, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, us, us_fs, out,
                        out_fs, g);
   });
   return hipGetLastError();
@@ -510,7 +533,7 @@ hipError_t launch_y_inverse_curl(int N, const cpx<T>* in, int64_t in_fs, cpx<T>*
                                  int64_t out_fs, const Geo& g, hipStream_t s) {
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(int64_t(g.nx) * g.kz, StridedShape<NN>::TC);
-    hipLaunchKernelGGL((k_y_inverse_curl<NN, T>), dim3(nb), dim3(kThreads), 0, s, in, in_fs, out,
+    hipLaunchKernelGGL((k_y_inverse_curl<NN, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, in, in_fs, out,
                        out_fs, g);
   });
   return hipGetLastError();
@@ -521,7 +544,7 @@ hipError_t launch_z_fused(int N, cpx<T>* buf, int64_t fs, int64_t npencils, int 
                           hipStream_t s) {
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(npencils, PencilBlock<NN>::PW);
-    hipLaunchKernelGGL((k_z_fused<NN, T>), dim3(nb), dim3(kZThreads), 0, s, buf, fs, npencils, kz,
+    hipLaunchKernelGGL((k_z_fused<NN, T>), dim3(nb), dim3(PencilBlock<NN>::THREADS), 0, s, buf, fs, npencils, kz,
                        scale);
   });
   return hipGetLastError();
@@ -532,7 +555,7 @@ hipError_t launch_z_c2r(int N, const cpx<T>* in, int64_t in_fs, T* out, int64_t 
                         int64_t npencils, int kz, T scale, hipStream_t s) {
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(npencils, PencilBlock<NN>::PW);
-    hipLaunchKernelGGL((k_z_c2r<NN, T>), dim3(nb), dim3(kZThreads), 0, s, in, in_fs, out, out_fs,
+    hipLaunchKernelGGL((k_z_c2r<NN, T>), dim3(nb), dim3(PencilBlock<NN>::THREADS), 0, s, in, in_fs, out, out_fs,
                        nf, npencils, kz, scale);
   });
   return hipGetLastError();
@@ -543,7 +566,7 @@ hipError_t launch_z_r2c(int N, const T* in, int64_t in_fs, cpx<T>* out, int64_t 
                         int64_t npencils, int kz, hipStream_t s) {
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(npencils, PencilBlock<NN>::PW);
-    hipLaunchKernelGGL((k_z_r2c<NN, T>), dim3(nb), dim3(kZThreads), 0, s, in, in_fs, out, out_fs,
+    hipLaunchKernelGGL((k_z_r2c<NN, T>), dim3(nb), dim3(PencilBlock<NN>::THREADS), 0, s, in, in_fs, out, out_fs,
                        nf, npencils, kz);
   });
   return hipGetLastError();
@@ -557,11 +580,11 @@ hipError_t launch_x_forward_stage(int N, int mode, const StageArgs<T>& a, hipStr
     const unsigned nb = ceil_div(J, StridedShape<NN>::TC);
     if (nblocks_out) *nblocks_out = int(nb);
     switch (mode) {
-      case kStageRhs: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageRhs, T>), dim3(nb), dim3(kThreads), 0, s, a); break;
-      case kStage1: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage1, T>), dim3(nb), dim3(kThreads), 0, s, a); break;
-      case kStage2: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage2, T>), dim3(nb), dim3(kThreads), 0, s, a); break;
-      case kStage3: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage3, T>), dim3(nb), dim3(kThreads), 0, s, a); break;
-      case kStage4: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage4, T>), dim3(nb), dim3(kThreads), 0, s, a); break;
+      case kStageRhs: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageRhs, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a); break;
+      case kStage1: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage1, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a); break;
+      case kStage2: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage2, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a); break;
+      case kStage3: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage3, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a); break;
+      case kStage4: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage4, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a); break;
       default: return hipErrorInvalidValue;
     }
   });

@@ -10,17 +10,20 @@ constexpr int kZThreads = 64;   // z passes: one wave, so the pencil exchanges n
 
 template <int N>
 struct StridedShape {
-  static constexpr int R = PencilShape<N>::R, Q = PencilShape<N>::Q;
-  static constexpr int TC = kThreads / Q;  // columns per workgroup
-  static constexpr int LDS = TC * R * (Q + 1);
+  using F = FftFor<N>;
+  static constexpr int R = F::R, Q = F::Q;
+  static constexpr int TC = (kThreads / Q) > 8 ? kThreads / Q : 8;  // columns per workgroup
+  static constexpr int THREADS = TC * Q;
+  static constexpr int LDS = TC * F::kSlots;
 };
 template <int N>
 struct PencilBlock {
-  static constexpr int R = PencilShape<N>::R, Q = PencilShape<N>::Q;
-  static constexpr int PW = kZThreads / Q;  // pencils per (one-wave) workgroup
-  static constexpr int LDS = PW * lds_pencil_elems<N>();
+  using F = FftZFor<N>;
+  static constexpr int R = F::R, Q = F::Q;
+  static constexpr int PW = (kZThreads / Q) > 1 ? kZThreads / Q : 1;  // pencils per workgroup
+  static constexpr int THREADS = PW * Q;
+  static constexpr int LDS = PW * F::kSlots;
 };
-
 
 inline int ceil_div_i(int64_t a, int64_t b) { return int((a + b - 1) / b); }
 

@@ -51,6 +51,28 @@ def pass_bytes(N, elem=8):
     return {k: v * elem for k, v in per.items()}, step * elem
 
 
+def hbm_copy_gbs(torch, reps=4):
+    """Device-to-device copy of 2 GiB with the 256 MB Infinity Cache flushed
+    (a 1 GiB write) before each copy: the bandwidth this box's HBM delivers to
+    a plain streaming kernel, reported beside the 8 TB/s datasheet peak."""
+    n = 1 << 29
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    flush = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    vals = []
+    for i in range(reps + 1):
+        flush.fill_(float(i))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        torch.cuda.synchronize()
+        if i:
+            vals.append(2 * 4 * n / (e0.elapsed_time(e1) * 1e-3) / 1e9)
+    del src, dst, flush
+    return round(float(np.median(vals)), 1)
+
+
 def cpu_baseline(u0, N, nu, dt, budget_s=15.0):
     """Oracle (float32 NumPy/SciPy, all host cores) on the same initial field:
     one warm-up step, then steps until `budget_s` (at least two)."""
@@ -162,6 +184,8 @@ def main():
     torch.cuda.synchronize()
     spec_ms = e0.elapsed_time(e1) / 10
 
+    copy_gbs = hbm_copy_gbs(torch)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(u0.cpu().numpy(), N, nu, dt)
@@ -189,6 +213,7 @@ def main():
         "hbm_frac_step": round(step_bytes * sps / (HBM_PEAK_GBS * 1e9), 4),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "hbm_copy_gbs": copy_gbs,
         "passes": passes,
         "spectrum_ms": round(spec_ms, 4),
         "energy": {"start": E0, "end": E1},
