@@ -299,3 +299,9 @@ def test_bad_inputs_raise():
     with pytest.raises(ValueError):
         h3.rollout(rhs, torch.zeros((3, 16, 16, 9), dtype=torch.complex64, device="cuda"),
                    h3.NSParams(nu=0.1), 1e-3, -1)
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-13), (np.float32, 1e-6)])
+def test_taylor_green_matches_oracle(dtype, tol):
+    og, g = _grids((16, 32, 16), dtype)
+    assert rel(h3.taylor_green(g), O.taylor_green(og)) < tol

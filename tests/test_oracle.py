@@ -331,3 +331,17 @@ def test_philox_uniform_mapping():
         assert u64[i] == (int(raw[i]) >> 11) / 2.0**53 - 0.5
         assert u32[i] == np.float32((int(raw[i]) >> 40) / 2.0**24 - 0.5)
     assert u64.min() >= -0.5 and u64.max() < 0.5
+
+
+def test_taylor_green_closed_form():
+    """fields.py:71-87: the field is band-limited (|n_i| = 1), so its physical
+    values come back exactly and it is divergence-free."""
+    g = O.make_grid(16)
+    u = O.to_physical(O.taylor_green(g), g)
+    x = (np.arange(16) * 2 * np.pi / 16).reshape(-1, 1, 1)
+    y, z = x.reshape(1, -1, 1), x.reshape(1, 1, -1)
+    assert np.allclose(u[0], np.sin(x) * np.cos(y) * np.cos(z), atol=1e-14)
+    assert np.allclose(u[1], -np.cos(x) * np.sin(y) * np.cos(z), atol=1e-14)
+    assert np.allclose(u[2], 0, atol=1e-14)
+    uh = O.taylor_green(g)
+    assert np.max(np.abs(g.kx * uh[0] + g.ky * uh[1] + g.kz * uh[2])) < 1e-10
