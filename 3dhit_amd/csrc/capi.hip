@@ -507,6 +507,155 @@ int to_spectral_impl(const hit3d_grid& gd, const T* in, cpx<T>* out, int nf, cpx
   return HIT3D_OK;
 }
 
+// ---- adjoint (VJP) of the RHS and of one RK4 step --------------------------
+
+struct VjpBuffers {
+  void* vp = nullptr;    // [3] P M v
+  void* S = nullptr;     // [9] transform inputs, later [6] transformed cross products
+  void* tw = nullptr;    // [9] transform workspace
+  void* ph = nullptr;    // [9] real physical u, w, g
+  void* cr = nullptr;    // [6] real w x g, g x u
+  double* partials = nullptr;
+  double* scalars = nullptr;
+  double* dot = nullptr;
+};
+
+VjpBuffers carve_vjp(const hit3d_grid& gd, void* work, size_t* total) {
+  Carve c;
+  c.base = static_cast<char*>(work);
+  const size_t cb = cbytes(gd);
+  const size_t fs = size_t(gd.nx) * gd.ny * (gd.nz / 2 + 1);
+  const size_t pr = size_t(gd.nx) * gd.ny * gd.nz * size_t(gd.precision);
+  VjpBuffers b;
+  b.vp = c.take(3 * fs * cb);
+  b.S = c.take(9 * fs * cb);
+  b.tw = c.take(9 * fs * cb);
+  b.ph = c.take(9 * pr);
+  b.cr = c.take(6 * pr);
+  b.partials = static_cast<double*>(c.take(size_t(3) * h3::kReduceBlocks * sizeof(double)));
+  b.scalars = static_cast<double*>(c.take(16 * sizeof(double)));
+  b.dot = static_cast<double*>(c.take(16 * sizeof(double)));
+  *total = c.off;
+  return b;
+}
+
+template <class T>
+int rhs_vjp_impl(const hit3d_grid& gd, const cpx<T>* u, const cpx<T>* v, cpx<T>* out, double nu,
+                 const hit3d_forcing* f, void* work, hipStream_t s) {
+  size_t total;
+  VjpBuffers b = carve_vjp(gd, work, &total);
+  const Geo g = make_geo(gd, false);
+  const int64_t fs = int64_t(g.nx) * g.ny * g.nzh;
+  const int64_t np = int64_t(g.nx) * g.ny * g.nz;
+  const double ntot = double(gd.nx) * double(gd.ny) * double(gd.nz);
+  cpx<T>* vp = static_cast<cpx<T>*>(b.vp);
+  cpx<T>* S = static_cast<cpx<T>*>(b.S);
+  cpx<T>* tw = static_cast<cpx<T>*>(b.tw);
+  T* ph = static_cast<T*>(b.ph);
+  T* cr = static_cast<T*>(b.cr);
+  H3_HIP(h3::launch_dealias_project<T>(v, vp, fs, 3, g, 1, 1, s));
+  H3_HIP(h3::launch_adj_prep<T>(u, vp, S, fs, g, ntot, s));
+  int rc = to_physical_impl<T>(gd, S, ph, 9, tw, s);
+  if (rc) return rc;
+  H3_HIP(h3::launch_adj_cross<T>(ph, np, cr, np, np, s));
+  rc = to_spectral_impl<T>(gd, cr, S, 6, tw, s);
+  if (rc) return rc;
+  T kmin2 = T(1), kmax2 = T(0);
+  h3::AdjCombine c{};
+  c.fkind = f ? f->kind : HIT3D_FORCING_NONE;
+  c.nu = nu;
+  c.inv_ntot = 1.0 / ntot;
+  c.inv_ntot2 = ntot2_inv(gd);
+  c.tiny = gd.precision == 4 ? double(FLT_EPSILON) : DBL_EPSILON;
+  c.scalars = b.scalars;
+  c.dot = b.dot + 1;
+  if (f && f->kind == HIT3D_FORCING_LINEAR) c.a = f->a;
+  if (f && (f->kind == HIT3D_FORCING_BAND || f->kind == HIT3D_FORCING_LUNDGREN)) {
+    const bool band = f->kind == HIT3D_FORCING_BAND;
+    if (band) {
+      kmin2 = T(f->kmin * f->kmin);
+      kmax2 = T(f->kmax * f->kmax);
+    }
+    c.fnu = f->nu;
+    c.tau = f->tau;
+    rc = reduce_state<T>(gd, g, u, f, b.partials, b.scalars, s);  // coef, S_E, S_Om, S_band
+    if (rc) return rc;
+    H3_HIP(h3::launch_dot_partials<T>(u, vp, fs, g, kmin2, kmax2, band ? 1 : 0, b.partials, s));
+    FinalizeArgs fa{};
+    fa.kind = HIT3D_FORCING_NONE;
+    fa.inv_ntot2 = 1.0;
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, b.partials, h3::kReduceBlocks, fa,
+                       b.dot);  // dot[1] = the raw sum
+    H3_HIP(hipGetLastError());
+  }
+  H3_HIP(h3::launch_adj_combine<T>(S, v, vp, u, out, fs, g, kmin2, kmax2, c, s));
+  return HIT3D_OK;
+}
+
+size_t state_bytes(const hit3d_grid& gd) {
+  return size_t(3) * gd.nx * gd.ny * (gd.nz / 2 + 1) * cbytes(gd);
+}
+
+struct StepVjpBuffers {
+  void *ub, *k, *y2, *y3, *y4, *kb, *yb, *rhs_ws, *vjp_ws;
+  size_t rhs_bytes, vjp_bytes;
+};
+
+StepVjpBuffers carve_step_vjp(const hit3d_grid& gd, void* work, size_t* total) {
+  Carve c;
+  c.base = static_cast<char*>(work);
+  StepVjpBuffers b{};
+  const size_t st = state_bytes(gd);
+  b.ub = c.take(st);
+  b.k = c.take(st);
+  b.y2 = c.take(st);
+  b.y3 = c.take(st);
+  b.y4 = c.take(st);
+  b.kb = c.take(st);
+  b.yb = c.take(st);
+  carve_step(gd, make_geo(gd, false), false, false, nullptr, &b.rhs_bytes);
+  carve_vjp(gd, nullptr, &b.vjp_bytes);
+  b.rhs_ws = c.take(b.rhs_bytes);
+  b.vjp_ws = c.take(b.vjp_bytes);
+  *total = c.off;
+  return b;
+}
+
+template <class T>
+int rk4_step_vjp_impl(const hit3d_grid& gd, const cpx<T>* u, const cpx<T>* ub_in, cpx<T>* out,
+                      double nu, double dt, const hit3d_forcing* f, void* work, hipStream_t s) {
+  size_t total;
+  StepVjpBuffers b = carve_step_vjp(gd, work, &total);
+  const int64_t n = int64_t(3) * gd.nx * gd.ny * (gd.nz / 2 + 1);
+  auto P = [](void* p) { return static_cast<cpx<T>*>(p); };
+  cpx<T>*ub = P(b.ub), *k = P(b.k), *y2 = P(b.y2), *y3 = P(b.y3), *y4 = P(b.y4), *kb = P(b.kb),
+  *yb = P(b.yb);
+  const T hdt = T(0.5 * dt), fdt = T(dt), sdt = T(dt / 6.0), tdt = T(dt / 3.0);
+  H3_HIP(hipMemcpyAsync(ub, ub_in, n * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s));
+  // stage inputs (integrators.py:34-37), dense transforms
+  int rc = rhs_impl<T>(gd, u, k, nu, f, false, b.rhs_ws, s);
+  if (rc) return rc;
+  H3_HIP(h3::launch_axpby<T>(y2, T(1), u, hdt, k, n, s));
+  if ((rc = rhs_impl<T>(gd, y2, k, nu, f, false, b.rhs_ws, s))) return rc;
+  H3_HIP(h3::launch_axpby<T>(y3, T(1), u, hdt, k, n, s));
+  if ((rc = rhs_impl<T>(gd, y3, k, nu, f, false, b.rhs_ws, s))) return rc;
+  H3_HIP(h3::launch_axpby<T>(y4, T(1), u, fdt, k, n, s));
+  // reverse sweep: kbar_i from the combination weights, then stage adjoints 4..1
+  H3_HIP(h3::launch_axpby<T>(kb, sdt, ub, T(0), ub, n, s));
+  if ((rc = rhs_vjp_impl<T>(gd, y4, kb, yb, nu, f, b.vjp_ws, s))) return rc;
+  H3_HIP(h3::launch_axpby<T>(out, T(1), ub, T(1), yb, n, s));
+  H3_HIP(h3::launch_axpby<T>(kb, tdt, ub, fdt, yb, n, s));
+  if ((rc = rhs_vjp_impl<T>(gd, y3, kb, yb, nu, f, b.vjp_ws, s))) return rc;
+  H3_HIP(h3::launch_axpby<T>(out, T(1), out, T(1), yb, n, s));
+  H3_HIP(h3::launch_axpby<T>(kb, tdt, ub, hdt, yb, n, s));
+  if ((rc = rhs_vjp_impl<T>(gd, y2, kb, yb, nu, f, b.vjp_ws, s))) return rc;
+  H3_HIP(h3::launch_axpby<T>(out, T(1), out, T(1), yb, n, s));
+  H3_HIP(h3::launch_axpby<T>(kb, sdt, ub, hdt, yb, n, s));
+  if ((rc = rhs_vjp_impl<T>(gd, u, kb, yb, nu, f, b.vjp_ws, s))) return rc;
+  H3_HIP(h3::launch_axpby<T>(out, T(1), out, T(1), yb, n, s));
+  return HIT3D_OK;
+}
+
 int check_work(void* work, size_t have, size_t need) {
   if (need > have)
     return fail(HIT3D_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) +
@@ -567,6 +716,12 @@ int hit3d_workspace_size(const hit3d_grid* grid, int32_t op, int64_t ncomp, int3
       return HIT3D_OK;
     case HIT3D_OP_REDUCE:
       *bytes = reduce_bytes(*grid, nullptr, nullptr, nullptr);
+      return HIT3D_OK;
+    case HIT3D_OP_RHS_VJP:
+      carve_vjp(*grid, nullptr, bytes);
+      return HIT3D_OK;
+    case HIT3D_OP_RK4_VJP:
+      carve_step_vjp(*grid, nullptr, bytes);
       return HIT3D_OK;
     default:
       return fail(HIT3D_ERR_INVALID, "unknown op " + std::to_string(op));
@@ -858,6 +1013,54 @@ int hit3d_profile_rk4(const hit3d_grid* grid, void* u_hat, double nu, double dt,
     counts[tm.id[i]] += 1;
   }
   return HIT3D_OK;
+}
+
+int hit3d_rhs_vjp(const hit3d_grid* grid, const void* u_hat, const void* v, void* ubar, double nu,
+                  const hit3d_forcing* forcing, double t, void* work, size_t work_bytes,
+                  void* stream) {
+  (void)t;
+  H3_TRY(check_grid(grid));
+  H3_TRY(check_forcing(forcing));
+  H3_TRY(check_ptr(u_hat, "u_hat"));
+  H3_TRY(check_ptr(v, "v"));
+  H3_TRY(check_ptr(ubar, "ubar"));
+  if (ubar == u_hat || ubar == v) return fail(HIT3D_ERR_INVALID, "ubar must not alias u_hat or v");
+  size_t need;
+  carve_vjp(*grid, nullptr, &need);
+  H3_TRY(check_work(work, work_bytes, need));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (grid->precision == 4)
+    return rhs_vjp_impl<float>(*grid, static_cast<const cpx<float>*>(u_hat),
+                               static_cast<const cpx<float>*>(v), static_cast<cpx<float>*>(ubar),
+                               nu, forcing, work, s);
+  return rhs_vjp_impl<double>(*grid, static_cast<const cpx<double>*>(u_hat),
+                              static_cast<const cpx<double>*>(v), static_cast<cpx<double>*>(ubar),
+                              nu, forcing, work, s);
+}
+
+int hit3d_rk4_step_vjp(const hit3d_grid* grid, const void* u_hat, const void* ubar_next,
+                       void* ubar, double nu, double dt, double t, const hit3d_forcing* forcing,
+                       void* work, size_t work_bytes, void* stream) {
+  (void)t;
+  H3_TRY(check_grid(grid));
+  H3_TRY(check_forcing(forcing));
+  H3_TRY(check_ptr(u_hat, "u_hat"));
+  H3_TRY(check_ptr(ubar_next, "ubar_next"));
+  H3_TRY(check_ptr(ubar, "ubar"));
+  if (ubar == u_hat) return fail(HIT3D_ERR_INVALID, "ubar must not alias u_hat");
+  if (!std::isfinite(dt) || !std::isfinite(nu))
+    return fail(HIT3D_ERR_INVALID, "dt and nu must be finite");
+  size_t need;
+  carve_step_vjp(*grid, nullptr, &need);
+  H3_TRY(check_work(work, work_bytes, need));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (grid->precision == 4)
+    return rk4_step_vjp_impl<float>(*grid, static_cast<const cpx<float>*>(u_hat),
+                                    static_cast<const cpx<float>*>(ubar_next),
+                                    static_cast<cpx<float>*>(ubar), nu, dt, forcing, work, s);
+  return rk4_step_vjp_impl<double>(*grid, static_cast<const cpx<double>*>(u_hat),
+                                   static_cast<const cpx<double>*>(ubar_next),
+                                   static_cast<cpx<double>*>(ubar), nu, dt, forcing, work, s);
 }
 
 }  // extern "C"

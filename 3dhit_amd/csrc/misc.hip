@@ -312,6 +312,127 @@ __global__ __launch_bounds__(kBlock) void k_tables(Geo g, T* k2o, T* ik2o, uint8
   if (shello) shello[idx] = shell_of<T>(k2, g.k0x);
 }
 
+// ---- adjoint of the RHS: elementwise pieces ----------------------------------
+// The VJP of rhs at u for a cotangent v (vp = P M v) is
+//   ubar = (w/Ntot) F[w_p x g] + K (w/Ntot) F[g x u_p] - nu k2 v + forcing part,
+// with g = Ntot irfftn(vp / w) the adjoint of rfftn, K = i k x (self-adjoint)
+// and w the Hermitian weight of the mode's kz plane.
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_adj_prep(const cpx<T>* __restrict__ u,
+                                                     const cpx<T>* __restrict__ vp,
+                                                     cpx<T>* __restrict__ S, int64_t fs, Geo g,
+                                                     double ntot) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= n) return;
+  const ModeIdx m = mode_of(idx, g);
+  const T kx = wavenumber<T>(g.k0x, m.nx), ky = wavenumber<T>(g.k0y, m.ny),
+          kz = wavenumber<T>(g.k0z, m.nz);
+  const cpx<T> u0 = u[idx], u1 = u[fs + idx], u2 = u[2 * fs + idx];
+  S[idx] = u0;
+  S[fs + idx] = u1;
+  S[2 * fs + idx] = u2;
+  S[3 * fs + idx] = mul_i(ky * u2 - kz * u1);
+  S[4 * fs + idx] = mul_i(kz * u0 - kx * u2);
+  S[5 * fs + idx] = mul_i(kx * u1 - ky * u0);
+  const T sc = T(ntot / hermitian_weight(m.nz, g.nz));
+  for (int c = 0; c < 3; ++c) S[(6 + c) * fs + idx] = sc * vp[c * fs + idx];
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_adj_cross(const T* __restrict__ ph, int64_t pfs,
+                                                      T* __restrict__ out, int64_t ofs,
+                                                      int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const T u0 = ph[i], u1 = ph[pfs + i], u2 = ph[2 * pfs + i];
+  const T w0 = ph[3 * pfs + i], w1 = ph[4 * pfs + i], w2 = ph[5 * pfs + i];
+  const T g0 = ph[6 * pfs + i], g1 = ph[7 * pfs + i], g2 = ph[8 * pfs + i];
+  out[i] = w1 * g2 - w2 * g1;
+  out[ofs + i] = w2 * g0 - w0 * g2;
+  out[2 * ofs + i] = w0 * g1 - w1 * g0;
+  out[3 * ofs + i] = g1 * u2 - g2 * u1;
+  out[4 * ofs + i] = g2 * u0 - g0 * u2;
+  out[5 * ofs + i] = g0 * u1 - g1 * u0;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_dot_partials(const cpx<T>* __restrict__ x,
+                                                         const cpx<T>* __restrict__ y, int64_t fs,
+                                                         Geo g, T kmin2, T kmax2, int use_band,
+                                                         double* __restrict__ partials) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  double s = 0.0;
+  for (int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x; idx < n;
+       idx += int64_t(gridDim.x) * kBlock) {
+    if (use_band) {
+      const ModeIdx m = mode_of(idx, g);
+      const T k2 = k2_of(wavenumber<T>(g.k0x, m.nx), wavenumber<T>(g.k0y, m.ny),
+                         wavenumber<T>(g.k0z, m.nz));
+      if (!(kmin2 <= k2 && k2 <= kmax2)) continue;
+    }
+    for (int c = 0; c < 3; ++c) {
+      const cpx<T> a = x[c * fs + idx], b = y[c * fs + idx];
+      s += double(a.x) * b.x + double(a.y) * b.y;
+    }
+  }
+  block_sum3m(s, 0.0, 0.0, partials + 3 * blockIdx.x);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_adj_combine(const cpx<T>* __restrict__ AB,
+                                                        const cpx<T>* __restrict__ v,
+                                                        const cpx<T>* __restrict__ vp,
+                                                        const cpx<T>* __restrict__ u,
+                                                        cpx<T>* __restrict__ out, int64_t fs,
+                                                        Geo g, T kmin2, T kmax2, AdjCombine c) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= n) return;
+  const ModeIdx m = mode_of(idx, g);
+  const T kx = wavenumber<T>(g.k0x, m.nx), ky = wavenumber<T>(g.k0y, m.ny),
+          kz = wavenumber<T>(g.k0z, m.nz);
+  const T k2 = k2_of(kx, ky, kz);
+  const int w = hermitian_weight(m.nz, g.nz);
+  const T sc = T(w * c.inv_ntot);
+  const cpx<T> b0 = AB[3 * fs + idx], b1 = AB[4 * fs + idx], b2 = AB[5 * fs + idx];
+  cpx<T> r[3] = {AB[idx] + mul_i(ky * b2 - kz * b1), AB[fs + idx] + mul_i(kz * b0 - kx * b2),
+                 AB[2 * fs + idx] + mul_i(kx * b1 - ky * b0)};
+  const T nk2 = T(c.nu) * k2;
+  // forcing part: coef * [band] * vp + dot * grad(coef) (grad along u)
+  T fc = T(0), gu = T(0);
+  if (c.fkind == 1) {  // LINEAR
+    fc = T(c.a);
+  } else if (c.fkind == 2) {  // BAND: coef = eps / S_B
+    const double sB = c.scalars[3], coef = c.scalars[0];
+    const bool in_band = kmin2 <= k2 && k2 <= kmax2;
+    if (sB > c.tiny && in_band) {
+      fc = T(coef);
+      gu = T(-(coef / sB) * 2.0 * w * c.inv_ntot2 * *c.dot);
+    }
+  } else if (c.fkind == 3) {  // LUNDGREN: coef = (nu S_Om - (S_E/2 - E*)/tau) / S_E
+    const double sE = c.scalars[1], coef = c.scalars[0];
+    if (sE > c.tiny) {
+      double dA_dE = -coef / sE;
+      if (c.tau > 0.0 && isfinite(c.tau)) dA_dE -= 0.5 / c.tau / sE;
+      fc = T(coef);
+      gu = T((*c.dot) * 2.0 * w * c.inv_ntot2 * (c.fnu / sE * double(k2) + dA_dE));
+    }
+  }
+  for (int i = 0; i < 3; ++i) {
+    const int64_t o = i * fs + idx;
+    out[o] = sc * r[i] - nk2 * v[o] + fc * vp[o] + gu * u[o];
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_axpby(cpx<T>* out, T a, const cpx<T>* x, T b,
+                                                  const cpx<T>* y, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) out[i] = a * x[i] + b * y[i];
+}
+
 // ---- launchers -------------------------------------------------------------
 
 template <class T>
@@ -402,6 +523,63 @@ hipError_t launch_tables(const Geo& g, T* k2, T* inv_k2, uint8_t* mask, int32_t*
                      shell);
   return hipGetLastError();
 }
+
+template <class T>
+hipError_t launch_adj_prep(const cpx<T>* u, const cpx<T>* vp, cpx<T>* S, int64_t fs, const Geo& g,
+                           double ntot, hipStream_t s) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  hipLaunchKernelGGL((k_adj_prep<T>), dim3(nblocks_for(n)), dim3(kBlock), 0, s, u, vp, S, fs, g,
+                     ntot);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_adj_cross(const T* ph, int64_t pfs, T* out, int64_t ofs, int64_t n,
+                            hipStream_t s) {
+  hipLaunchKernelGGL((k_adj_cross<T>), dim3(nblocks_for(n)), dim3(kBlock), 0, s, ph, pfs, out, ofs,
+                     n);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_dot_partials(const cpx<T>* x, const cpx<T>* y, int64_t fs, const Geo& g,
+                               T kmin2, T kmax2, int use_band, double* partials, hipStream_t s) {
+  hipLaunchKernelGGL((k_dot_partials<T>), dim3(kReduceBlocks), dim3(kBlock), 0, s, x, y, fs, g,
+                     kmin2, kmax2, use_band, partials);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_adj_combine(const cpx<T>* AB, const cpx<T>* v, const cpx<T>* vp,
+                              const cpx<T>* u, cpx<T>* out, int64_t fs, const Geo& g, T kmin2,
+                              T kmax2, const AdjCombine& c, hipStream_t s) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  hipLaunchKernelGGL((k_adj_combine<T>), dim3(nblocks_for(n)), dim3(kBlock), 0, s, AB, v, vp, u,
+                     out, fs, g, kmin2, kmax2, c);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_axpby(cpx<T>* out, T a, const cpx<T>* x, T b, const cpx<T>* y, int64_t n,
+                        hipStream_t s) {
+  hipLaunchKernelGGL((k_axpby<T>), dim3(nblocks_for(n)), dim3(kBlock), 0, s, out, a, x, b, y, n);
+  return hipGetLastError();
+}
+
+#define H3_INSTANTIATE_ADJ(T)                                                                      \
+  template hipError_t launch_adj_prep<T>(const cpx<T>*, const cpx<T>*, cpx<T>*, int64_t,          \
+                                         const Geo&, double, hipStream_t);                         \
+  template hipError_t launch_adj_cross<T>(const T*, int64_t, T*, int64_t, int64_t, hipStream_t);  \
+  template hipError_t launch_dot_partials<T>(const cpx<T>*, const cpx<T>*, int64_t, const Geo&, T, \
+                                             T, int, double*, hipStream_t);                        \
+  template hipError_t launch_adj_combine<T>(const cpx<T>*, const cpx<T>*, const cpx<T>*,          \
+                                            const cpx<T>*, cpx<T>*, int64_t, const Geo&, T, T,     \
+                                            const AdjCombine&, hipStream_t);                       \
+  template hipError_t launch_axpby<T>(cpx<T>*, T, const cpx<T>*, T, const cpx<T>*, int64_t,       \
+                                      hipStream_t);
+
+H3_INSTANTIATE_ADJ(float)
+H3_INSTANTIATE_ADJ(double)
 
 #define H3_INSTANTIATE_MISC(T)                                                                     \
   template hipError_t launch_gather<T>(const cpx<T>*, int64_t, cpx<T>*, int64_t, int, const Geo&, \

@@ -7,6 +7,7 @@ complex128 for float64 grids. There is no CPU path: importing works without
 a GPU, but every operator needs the built libhit3d_hip.so and a device.
 """
 
+from .adjoint import differentiable_rollout, make_ns_rhs_vjp, rk4_step_vjp, rollout_vjp
 from .diagnostics import (
     component_spectra, dissipation, energy, energy_budget, energy_budget_cb,
     energy_spectrum, enstrophy,
@@ -33,4 +34,5 @@ __all__ = [
     "rk4_step", "rollout", "evolve", "State", "Callback", "Diagnostic",
     "energy", "enstrophy", "dissipation", "energy_spectrum", "component_spectra",
     "energy_budget", "energy_budget_cb",
+    "make_ns_rhs_vjp", "rk4_step_vjp", "rollout_vjp", "differentiable_rollout",
 ]

@@ -12,7 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libhit3d_hip.so")
 
 HIT3D_OK, HIT3D_ERR_INVALID, HIT3D_ERR_HIP, HIT3D_ERR_WORKSPACE = 0, 1, 2, 3
-OP_TO_PHYSICAL, OP_TO_SPECTRAL, OP_RHS, OP_RK4, OP_SPECTRUM, OP_REDUCE = range(6)
+OP_TO_PHYSICAL, OP_TO_SPECTRAL, OP_RHS, OP_RK4, OP_SPECTRUM, OP_REDUCE, OP_RHS_VJP, OP_RK4_VJP = range(8)
 FORCING_NONE, FORCING_LINEAR, FORCING_BAND, FORCING_LUNDGREN = range(4)
 
 
@@ -58,6 +58,8 @@ SIGNATURES = {
     "hit3d_uniform_noise": [_G, ctypes.c_uint64, _P, _P],
     "hit3d_shell_scale": [_G, _P, _I64, _P, _I64, _P],
     "hit3d_profile_rk4": [_G, _P, _D, _D, _I64, _F, _I32, _P, _P, _P, _SZ, _P],
+    "hit3d_rhs_vjp": [_G, _P, _P, _P, _D, _F, _D, _P, _SZ, _P],
+    "hit3d_rk4_step_vjp": [_G, _P, _P, _P, _D, _D, _D, _F, _P, _SZ, _P],
 }
 
 PASS_NAMES = ("x_inverse_curl", "y_inverse_curl", "z_c2r_cross_r2c", "y_forward",

@@ -64,6 +64,8 @@ typedef struct hit3d_forcing {
 #define HIT3D_OP_RK4 3
 #define HIT3D_OP_SPECTRUM 4
 #define HIT3D_OP_REDUCE 5
+#define HIT3D_OP_RHS_VJP 6
+#define HIT3D_OP_RK4_VJP 7
 
 /* Message of the last failed call on this host thread (always NUL-terminated);
    returns the full message length. */
@@ -148,6 +150,24 @@ int hit3d_uniform_noise(const hit3d_grid* grid, uint64_t seed, void* u, void* st
 /* u_hat[c][mode] *= factors[shell(mode)] for shell < nbins (fields.py:58-68). */
 int hit3d_shell_scale(const hit3d_grid* grid, void* u_hat, int64_t ncomp, const double* factors,
                       int64_t nbins, void* stream);
+
+/* Adjoint of the RHS: ubar = (d rhs/d u_hat at u_hat)^* v, the derivative
+   jax.vjp takes of `rhs` (navierstokes.py:62-97; tests/test_grad.py:48-56),
+   written out by hand. Gradient convention: for a real J, dJ = Re sum over
+   all entries of conj(ubar) * du_hat (jax's cotangent is conj(ubar)).
+   Includes the derivative of the state-dependent forcing coefficients
+   (BAND, LUNDGREN). Dense transforms; ubar must not alias u_hat or v. */
+int hit3d_rhs_vjp(const hit3d_grid* grid, const void* u_hat, const void* v, void* ubar, double nu,
+                  const hit3d_forcing* forcing, double t, void* work, size_t work_bytes,
+                  void* stream);
+
+/* Adjoint of one classic RK4 step (integrators.py:32-38): from the step's
+   start state u_hat and the cotangent ubar_next of its result, writes the
+   cotangent ubar of u_hat. Recomputes the stage inputs, then runs the four
+   stage adjoints in reverse. ubar may alias ubar_next. */
+int hit3d_rk4_step_vjp(const hit3d_grid* grid, const void* u_hat, const void* ubar_next,
+                       void* ubar, double nu, double dt, double t, const hit3d_forcing* forcing,
+                       void* work, size_t work_bytes, void* stream);
 
 /* Measurement form of hit3d_rk4_steps (no reference counterpart): runs the
    same nsteps with a HIP event after every launch on `stream`, waits for the

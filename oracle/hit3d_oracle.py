@@ -362,6 +362,124 @@ def rollout(rhs, u_hat, params, dt, nsteps, t0=0.0, observe=None):
 
 
 # ---------------------------------------------------------------------------
+# Adjoint (vector-Jacobian products) of rhs, rk4_step and rollout. The
+# reference differentiates `rollout` with jax.vjp (tests/test_grad.py:29-74);
+# this is the same derivative written out by hand. Convention: for a real
+# J(u_hat), the gradient g satisfies dJ = Re sum(conj(g) * du_hat) over every
+# array entry (torch's convention; jax.vjp's cotangent is conj(g)). Pinned by
+# central finite differences (tests/test_oracle.py).
+# ---------------------------------------------------------------------------
+
+
+def _curl(x, grid):
+    """K x = i k x x, the map navierstokes.py:73-77 applies to u_hat. K is
+    self-adjoint under <a, b> = Re sum(conj(a) b)."""
+    kx, ky, kz = grid.kx, grid.ky, grid.kz
+    return 1j * np.stack((ky * x[2] - kz * x[1], kz * x[0] - kx * x[2], kx * x[1] - ky * x[0]))
+
+
+def _cross(a, b):
+    return np.stack((a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]))
+
+
+def _inner(a, b):
+    return float(np.sum(np.real(np.conj(a) * b)))
+
+
+def _forcing_vjp(forcing, u_hat, vp, grid):
+    """Cotangent of u_hat through the forcing term, given vp = P M v."""
+    if isinstance(forcing, NoForcing):
+        return 0.0
+    if isinstance(forcing, LinearForcing):
+        return forcing.A * vp
+    w, nt2 = grid.weights, _ntot2(grid)
+    tiny = np.finfo(grid.dtype).eps
+    a2 = np.sum(np.abs(u_hat) ** 2, axis=0)
+    if isinstance(forcing, BandForcing):
+        band = forcing._mask(grid)
+        sB = np.sum(w * band * a2) / nt2
+        if not sB > tiny:
+            return 0.0
+        c = forcing.eps / sB
+        grad_c = -(c / sB) * 2 * w * band * u_hat / nt2
+        return c * band * vp + _inner(band * u_hat, vp) * grad_c
+    if isinstance(forcing, LundgrenForcing):
+        sE = np.sum(w * a2) / nt2
+        sO = np.sum(w * grid.k2 * a2) / nt2
+        if not sE > tiny:
+            return 0.0
+        A = forcing.coefficient(u_hat, grid)
+        dA_dE = -A / sE
+        if forcing.tau is not None:
+            dA_dE -= 0.5 / forcing.tau / sE
+        grad_A = (forcing.nu / sE) * 2 * w * grid.k2 * u_hat / nt2 + dA_dE * 2 * w * u_hat / nt2
+        return A * vp + _inner(vp, u_hat) * grad_A
+    raise TypeError(f"no adjoint for {forcing!r}")
+
+
+def make_ns_rhs_vjp(grid):
+    """VJP of rhs (navierstokes.py:69-95) at the state u_hat: v -> (d rhs)^* v.
+    With g = N^3 irfftn(P M v / w) (the adjoint of rfftn), the nonlinear part
+    is (w/N^3) rfftn(w_p x g) + K (w/N^3) rfftn(g x u_p)."""
+    nt = grid.Nx * grid.Ny * grid.Nz
+
+    def vjp(u_hat, v, params, t):
+        w = grid.weights
+        vp = project(dealias(v, grid), grid)
+        g = nt * to_physical(vp / w, grid)
+        u = to_physical(u_hat, grid)
+        om = to_physical(_curl(u_hat, grid), grid)
+        a_hat = (w / nt) * to_spectral(_cross(om, g), grid)
+        b_hat = (w / nt) * to_spectral(_cross(g, u), grid)
+        out = a_hat + _curl(b_hat, grid) - params.nu * grid.k2 * v
+        return out + _forcing_vjp(params.forcing, u_hat, vp, grid)
+
+    return vjp
+
+
+def rk4_step_vjp(rhs, rhs_vjp, u_hat, ubar, params, dt, t):
+    """VJP of rk4_step (integrators.py:32-38): recomputes the stage inputs,
+    then runs the four stage adjoints backwards."""
+    k1 = rhs(u_hat, params, t)
+    y2 = u_hat + 0.5 * dt * k1
+    k2 = rhs(y2, params, t + 0.5 * dt)
+    y3 = u_hat + 0.5 * dt * k2
+    k3 = rhs(y3, params, t + 0.5 * dt)
+    y4 = u_hat + dt * k3
+    out = ubar.copy()
+    kb1 = kb4 = (dt / 6.0) * ubar
+    kb2 = kb3 = (dt / 3.0) * ubar
+    yb = rhs_vjp(y4, kb4, params, t + dt)
+    out, kb3 = out + yb, kb3 + dt * yb
+    yb = rhs_vjp(y3, kb3, params, t + 0.5 * dt)
+    out, kb2 = out + yb, kb2 + 0.5 * dt * yb
+    yb = rhs_vjp(y2, kb2, params, t + 0.5 * dt)
+    out, kb1 = out + yb, kb1 + 0.5 * dt * yb
+    return out + rhs_vjp(u_hat, kb1, params, t)
+
+
+def rollout_vjp(rhs, rhs_vjp, u_hat, ubar, params, dt, nsteps, t0=0.0, checkpoint=20):
+    """VJP of rollout (integrators.py:41-53) with checkpointed recomputation:
+    the forward keeps every `checkpoint`-th state; the backward recomputes
+    each segment's states from its checkpoint and runs the step adjoints in
+    reverse."""
+    ckpt = {0: u_hat}
+    u = u_hat
+    for n in range(nsteps):
+        u = rk4_step(rhs, u, params, dt, t0 + n * dt)
+        if (n + 1) % checkpoint == 0 and n + 1 < nsteps:
+            ckpt[n + 1] = u
+    for start in sorted(ckpt, reverse=True):
+        stop = min(start + checkpoint, nsteps)
+        states = [ckpt[start]]
+        for n in range(start, stop - 1):
+            states.append(rk4_step(rhs, states[-1], params, dt, t0 + n * dt))
+        for n in range(stop - 1, start - 1, -1):
+            ubar = rk4_step_vjp(rhs, rhs_vjp, states[n - start], ubar, params, dt, t0 + n * dt)
+    return ubar
+
+
+# ---------------------------------------------------------------------------
 # Diagnostics — diagnostics.py:45-114, 162-172
 # ---------------------------------------------------------------------------
 

@@ -1,0 +1,141 @@
+"""The hand-written adjoint on the GPU: parity with the oracle's adjoint
+(itself pinned by central differences in test_oracle.py), the autograd
+wrapper, and a central-difference check of the adjoint configuration's
+objective on the GPU."""
+
+import numpy as np
+import pytest
+import torch
+
+import hit3d_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+h3 = pytest.importorskip("hit3d")
+
+FORCINGS = [
+    (O.NoForcing(), h3.NoForcing()),
+    (O.LinearForcing(A=0.15), h3.LinearForcing(A=0.15)),
+    (O.BandForcing(eps=0.3, kmin=1.0, kmax=3.0), h3.BandForcing(eps=0.3, kmin=1.0, kmax=3.0)),
+    (O.LundgrenForcing(nu=0.02), h3.LundgrenForcing(nu=0.02)),
+    (O.LundgrenForcing(nu=0.02, energy=0.6, tau=0.3), h3.LundgrenForcing(nu=0.02, energy=0.6, tau=0.3)),
+]
+
+
+def rel(a, b):
+    a = a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(np.asarray(b).ravel()))
+
+
+def dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def _rand_c(seed, shape):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+@pytest.mark.parametrize("fi", range(len(FORCINGS)))
+@pytest.mark.parametrize("shape", [(16, 16, 16), (32, 16, 8)])
+def test_rhs_vjp_matches_oracle(fi, shape):
+    og, g = O.make_grid(*shape), h3.make_grid(*shape, dtype=torch.float64)
+    u = O.random_field(1, og, O.config_spectrum)
+    u = u + 1e-3 * np.abs(u).max() * _rand_c(2, u.shape)        # dense (non-dealiased) input
+    v = _rand_c(3, u.shape)
+    of, gf = FORCINGS[fi]
+    ref = O.make_ns_rhs_vjp(og)(u, v, O.NSParams(nu=0.02, forcing=of), 0.0)
+    got = h3.make_ns_rhs_vjp(g)(dev(u), dev(v), h3.NSParams(nu=0.02, forcing=gf), 0.0)
+    assert rel(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("fi", [0, 2, 3])
+def test_rk4_step_and_rollout_vjp_match_oracle(fi):
+    og, g = O.make_grid(16), h3.make_grid(16, dtype=torch.float64)
+    u = O.random_field(4, og, O.config_spectrum)
+    ub = _rand_c(5, u.shape)
+    of, gf = FORCINGS[fi]
+    op, gp = O.NSParams(nu=0.02, forcing=of), h3.NSParams(nu=0.02, forcing=gf)
+    orhs, grhs = O.make_ns_rhs(og), h3.make_ns_rhs(g)
+    ref = O.rk4_step_vjp(orhs, O.make_ns_rhs_vjp(og), u, ub, op, 2e-3, 0.0)
+    assert rel(h3.rk4_step_vjp(grhs, dev(u), dev(ub), gp, 2e-3, 0.0), ref) < 1e-12
+    ref = O.rollout_vjp(orhs, O.make_ns_rhs_vjp(og), u, ub, op, 2e-3, 7, checkpoint=3)
+    for ck in (1, 3, 20):
+        assert rel(h3.rollout_vjp(grhs, dev(u), dev(ub), gp, 2e-3, 7, checkpoint=ck), ref) < 1e-11
+
+
+def _shell_energy_weights(g, lo, hi):
+    """w / (2 Ntot^2) on modes whose shell s has lo <= s < hi (the energy in
+    those shells is sum(W |u|^2))."""
+    shell = g.shell.to(torch.int64)
+    band = ((shell >= lo) & (shell < hi)).to(g.dtype)
+    return band * g.weights / (2.0 * float(g.Nx * g.Ny * g.Nz) ** 2)
+
+
+def test_autograd_through_rollout_matches_oracle():
+    og, g = O.make_grid(16), h3.make_grid(16, dtype=torch.float64)
+    u = O.random_field(6, og, O.config_spectrum)
+    params = h3.NSParams(nu=0.02, forcing=h3.BandForcing(eps=0.1, kmin=1.0, kmax=2.0))
+    W = _shell_energy_weights(g, 3, 6)
+    u0 = dev(u).requires_grad_(True)
+    uN = h3.differentiable_rollout(h3.make_ns_rhs(g), u0, params, 2e-3, 6, checkpoint=4)
+    J = torch.sum(W * (uN.real ** 2 + uN.imag ** 2))
+    J.backward()
+    # dJ/du_N = 2 W u_N (torch convention); the oracle maps it back through the steps
+    op = O.NSParams(nu=0.02, forcing=O.BandForcing(eps=0.1, kmin=1.0, kmax=2.0))
+    uNo = O.rollout(O.make_ns_rhs(og), u, op, 2e-3, 6)
+    ref = O.rollout_vjp(O.make_ns_rhs(og), O.make_ns_rhs_vjp(og), u, 2 * W.cpu().numpy() * uNo, op,
+                        2e-3, 6, checkpoint=4)
+    assert rel(u0.grad, ref) < 1e-11
+
+
+def _adjoint_config_grad(dtype, N, steps, u0=None):
+    """The adjoint configuration: J = energy in shells 8 <= |k| < 16 after
+    `steps` RK4 steps (dt = 2e-3, nu = 1/600, band forcing eps = 0.1 on
+    1 <= |k| <= 2), gradient by the checkpointed adjoint (every 20 steps)."""
+    g = h3.make_grid(N, dtype=dtype)
+    params = h3.NSParams(nu=1 / 600, forcing=h3.BandForcing(eps=0.1, kmin=1.0, kmax=2.0))
+    rhs = h3.make_ns_rhs(g)
+    if u0 is None:
+        u0 = h3.random_field(0, g, h3.config_spectrum)
+    W = _shell_energy_weights(g, 8, 16)
+    x = u0.clone().requires_grad_(True)
+    uN = h3.differentiable_rollout(rhs, x, params, 2e-3, steps, checkpoint=20)
+    torch.sum(W * (uN.real ** 2 + uN.imag ** 2)).backward()
+    return g, params, rhs, u0, W, x.grad.detach()
+
+
+def test_adjoint_config_float32_gradient_matches_float64():
+    """128^3, 200 steps: the float32 adjoint gradient against the float64 one
+    from the same initial field (relative L2)."""
+    g32, _, _, u32, _, g_32 = _adjoint_config_grad(torch.float32, 128, 200)
+    _, _, _, _, _, g_64 = _adjoint_config_grad(torch.float64, 128, 200, u32.to(torch.complex128))
+    err = float(torch.linalg.vector_norm(g_32.to(torch.complex128) - g_64) / torch.linalg.vector_norm(g_64))
+    print(f"adjoint config: float32 vs float64 gradient rel L2 = {err:.2e}")
+    assert err < 1e-3
+
+
+@pytest.mark.parametrize("dtype,N,steps,tol", [(torch.float64, 32, 20, 1e-6), (torch.float64, 128, 200, 1e-5)])
+def test_adjoint_objective_central_differences(dtype, N, steps, tol):
+    """The adjoint configuration's gradient against central differences of J
+    along 4 seeded random solenoidal directions (eps = 1e-3 of the field's
+    scale); 128^3 x 200 steps is the configuration itself."""
+    g, params, rhs, u0, W, grad = _adjoint_config_grad(dtype, N, steps)
+
+    def J(u):
+        uN = h3.rollout(rhs, u, params, 2e-3, steps)
+        return float(torch.sum(W * (uN.real.double() ** 2 + uN.imag.double() ** 2)))
+
+    scale = float(torch.sqrt(torch.mean(u0.real ** 2 + u0.imag ** 2)))
+    gen = torch.Generator(device="cpu").manual_seed(1)
+    errs = []
+    for _ in range(4):
+        d = torch.complex(torch.randn(u0.shape, generator=gen, dtype=torch.float64),
+                          torch.randn(u0.shape, generator=gen, dtype=torch.float64))
+        d = h3.project(h3.dealias(d.to(u0.dtype).cuda(), g), g)   # keep the perturbed field solenoidal
+        eps = 1e-3 * scale / float(torch.sqrt(torch.mean(d.real ** 2 + d.imag ** 2)))
+        fd = (J(u0 + eps * d) - J(u0 - eps * d)) / (2 * eps)
+        an = float(torch.sum(grad.real.double() * d.real.double() + grad.imag.double() * d.imag.double()))
+        errs.append(abs(an - fd) / abs(fd))
+    print(f"adjoint vs central differences ({dtype}, {N}^3, {steps} steps): rel err {max(errs):.2e}")
+    assert max(errs) < tol

@@ -333,6 +333,51 @@ def test_philox_uniform_mapping():
     assert u64.min() >= -0.5 and u64.max() < 0.5
 
 
+ALL_FORCINGS = [O.NoForcing(), O.LinearForcing(A=0.15), O.BandForcing(eps=0.3, kmin=1.0, kmax=3.0),
+                O.LundgrenForcing(nu=0.02), O.LundgrenForcing(nu=0.02, energy=0.6, tau=0.3)]
+
+
+def _rand_c(seed, shape):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+@pytest.mark.parametrize("fi", range(len(ALL_FORCINGS)))
+@pytest.mark.parametrize("dealiased", [True, False])
+def test_rhs_vjp_matches_central_differences(fi, dealiased):
+    """<vjp(u, v), d> equals the central difference of <v, rhs(u)> along d
+    (dJ = Re sum conj(g) du) for every forcing, dense and dealiased inputs."""
+    g = O.make_grid(16)
+    u = O.random_field(4, g, O.config_spectrum)
+    if not dealiased:
+        u = u + 1e-2 * np.abs(u).max() * _rand_c(5, u.shape)
+    v, d = _rand_c(6, u.shape), _rand_c(7, u.shape)
+    params = O.NSParams(nu=0.02, forcing=ALL_FORCINGS[fi])
+    rhs, vjp = O.make_ns_rhs(g), O.make_ns_rhs_vjp(g)
+    eps = 1e-6 * np.abs(u).max() / np.abs(d).max()
+    fd = (O._inner(v, rhs(u + eps * d, params, 0.0)) - O._inner(v, rhs(u - eps * d, params, 0.0))) / (2 * eps)
+    an = O._inner(vjp(u, v, params, 0.0), d)
+    assert abs(an - fd) < 1e-7 * abs(fd)
+
+
+def test_rk4_step_and_rollout_vjp_match_central_differences():
+    g = O.make_grid(16)
+    u = O.random_field(8, g, O.config_spectrum)
+    params = O.NSParams(nu=0.02, forcing=O.LundgrenForcing(nu=0.02))
+    rhs, vjp = O.make_ns_rhs(g), O.make_ns_rhs_vjp(g)
+    ub, d = _rand_c(9, u.shape), _rand_c(10, u.shape)
+    eps = 1e-6 * np.abs(u).max() / np.abs(d).max()
+    J = lambda x: O._inner(ub, O.rk4_step(rhs, x, params, 2e-3, 0.0))  # noqa: E731
+    fd = (J(u + eps * d) - J(u - eps * d)) / (2 * eps)
+    an = O._inner(O.rk4_step_vjp(rhs, vjp, u, ub, params, 2e-3, 0.0), d)
+    assert abs(an - fd) < 1e-7 * abs(fd)
+    Jn = lambda x: O._inner(ub, O.rollout(rhs, x, params, 2e-3, 7))  # noqa: E731
+    fd = (Jn(u + eps * d) - Jn(u - eps * d)) / (2 * eps)
+    for ck in (1, 3, 20):
+        an = O._inner(O.rollout_vjp(rhs, vjp, u, ub, params, 2e-3, 7, checkpoint=ck), d)
+        assert abs(an - fd) < 1e-7 * abs(fd)
+
+
 def test_taylor_green_closed_form():
     """fields.py:71-87: the field is band-limited (|n_i| = 1), so its physical
     values come back exactly and it is divergence-free."""
