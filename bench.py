@@ -73,6 +73,42 @@ def hbm_copy_gbs(torch, reps=4):
     return round(float(np.median(vals)), 1)
 
 
+def adjoint_timing(torch, h3, N=128, steps=200, checkpoint=20):
+    """Gradient of the energy in shells 8 <= |k| < 16 after `steps` RK4 steps
+    (float32, nu = 1/600, dt = 2e-3, band forcing eps = 0.1 on 1 <= |k| <= 2)
+    with respect to the initial field, by the checkpointed hand-written
+    adjoint: wall time of forward + backward."""
+    g = h3.make_grid(N, dtype=torch.float32)
+    params = h3.NSParams(nu=1 / 600, forcing=h3.BandForcing(eps=0.1, kmin=1.0, kmax=2.0))
+    rhs = h3.make_ns_rhs(g)
+    u0 = h3.random_field(0, g, h3.config_spectrum)
+    shell = g.shell.to(torch.int64)
+    W = ((shell >= 8) & (shell < 16)).to(torch.float32) * g.weights / (2.0 * float(N**3) ** 2)
+
+    def run():
+        x = u0.clone().requires_grad_(True)
+        uN = h3.differentiable_rollout(rhs, x, params, 2e-3, steps, checkpoint=checkpoint)
+        J = torch.sum(W * (uN.real ** 2 + uN.imag ** 2))
+        J.backward()
+        return float(J.detach()), x.grad
+
+    run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J, grad = run()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    fw0 = time.perf_counter()
+    h3.rollout(rhs, u0, params, 2e-3, steps)
+    torch.cuda.synchronize()
+    fw = time.perf_counter() - fw0
+    return {"config": f"{N}^3 float32, {steps} RK4 steps, checkpoint every {checkpoint}, "
+                      "J = energy in shells 8..15, band forcing", "J": J,
+            "grad_norm": float(torch.linalg.vector_norm(grad)),
+            "forward_s": round(fw, 4), "forward_backward_s": round(el, 4),
+            "backward_over_forward": round((el - fw) / fw, 2)}
+
+
 def cpu_baseline(u0, N, nu, dt, budget_s=15.0):
     """Oracle (float32 NumPy/SciPy, all host cores) on the same initial field:
     one warm-up step, then steps until `budget_s` (at least two)."""
@@ -101,6 +137,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the 128^3 adjoint (gradient) timing")
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run 3 device copies of 1 GiB (reference kernel for the "
@@ -185,6 +223,7 @@ def main():
     spec_ms = e0.elapsed_time(e1) / 10
 
     copy_gbs = hbm_copy_gbs(torch)
+    adj = None if args.no_adjoint else adjoint_timing(torch, h3)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -214,6 +253,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "hbm_copy_gbs": copy_gbs,
+        "adjoint": adj,
         "passes": passes,
         "spectrum_ms": round(spec_ms, 4),
         "energy": {"start": E0, "end": E1},
