@@ -112,14 +112,16 @@ def test_adjoint_config_float32_gradient_matches_float64():
     _, _, _, _, _, g_64 = _adjoint_config_grad(torch.float64, 128, 200, u32.to(torch.complex128))
     err = float(torch.linalg.vector_norm(g_32.to(torch.complex128) - g_64) / torch.linalg.vector_norm(g_64))
     print(f"adjoint config: float32 vs float64 gradient rel L2 = {err:.2e}")
-    assert err < 1e-3
+    assert err < 1e-4
 
 
 @pytest.mark.parametrize("dtype,N,steps,tol", [(torch.float64, 32, 20, 1e-6), (torch.float64, 128, 200, 1e-5)])
 def test_adjoint_objective_central_differences(dtype, N, steps, tol):
     """The adjoint configuration's gradient against central differences of J
-    along 4 seeded random solenoidal directions (eps = 1e-3 of the field's
-    scale); 128^3 x 200 steps is the configuration itself."""
+    along 16 seeded random solenoidal directions (eps = 1e-3 of the field's
+    scale); 128^3 x 200 steps is the configuration itself. The pairing
+    an = Re sum(conj(grad) * d) is torch's convention; jax's cotangent for
+    the same J is conj(grad)."""
     g, params, rhs, u0, W, grad = _adjoint_config_grad(dtype, N, steps)
 
     def J(u):
@@ -129,7 +131,7 @@ def test_adjoint_objective_central_differences(dtype, N, steps, tol):
     scale = float(torch.sqrt(torch.mean(u0.real ** 2 + u0.imag ** 2)))
     gen = torch.Generator(device="cpu").manual_seed(1)
     errs = []
-    for _ in range(4):
+    for _ in range(16):
         d = torch.complex(torch.randn(u0.shape, generator=gen, dtype=torch.float64),
                           torch.randn(u0.shape, generator=gen, dtype=torch.float64))
         d = h3.project(h3.dealias(d.to(u0.dtype).cuda(), g), g)   # keep the perturbed field solenoidal
