@@ -299,6 +299,17 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   return x * q + min(x, r) + i;
 }
 
+// The same runs, each walked from its end: a pass that follows one mapped by
+// xcd_tile over the same index order starts on the tiles its producer wrote
+// last, which are still in the 256 MB memory-side cache.
+__device__ __forceinline__ int xcd_tile_rev(int b, int nb) {
+  constexpr int X = 8;
+  const int q = nb / X, r = nb % X;
+  const int x = b % X, i = b / X;
+  const int n = q + (x < r ? 1 : 0);
+  return x * q + min(x, r) + (n - 1 - i);
+}
+
 // ---------------------------------------------------------------------------
 // Grid geometry and the reference's wavenumber / mask / shell rules
 // (grid.py:89-132, diagnostics.py:76-82), evaluated from indices.

@@ -169,14 +169,18 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) void k_y_inverse_curl(con
   };
   // each field's loads are issued one transform ahead of its use (pf holds
   // the prefetched field), so the barriers inside run() hide their latency
+  // At N >= 512 the three-level transform needs more registers, so field 0
+  // is fetched after w's transform (three arrays live instead of four).
+  constexpr bool kEarly = N < 512;
   cpx<T> a[R], b[R], pf[R], w[R];
   load(1, a);
   load(2, b);
-  load(0, pf);
+  if constexpr (kEarly) load(0, pf);
 #pragma unroll
   for (int r = 0; r < R; ++r) w[r] = mul_i(kyv[r] * b[r] - kzv * a[r]);
   F::template run<+1>(w, q, L, tw);
   store(3, w);
+  if constexpr (!kEarly) load(0, pf);
   F::template run<+1>(b, q, L, tw);
   store(2, b);
   load(3, b);
@@ -195,19 +199,23 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) void k_y_inverse_curl(con
 // z-axis helpers: Hermitian extension of two half spectra U, W into the
 // spectrum of u + i w (c2r semantics: only Re of the kz = 0 and Nyquist bins).
 // ---------------------------------------------------------------------------
+// Branch-free: every lane loads (from index 0 when its mode is not stored,
+// kz == 0 included) and the value is zeroed by a select, so a thread's loads
+// for all its elements issue back to back and share one wait. A branch around
+// each load pair would serialise them (one memory round trip per element).
 template <int N, class T>
 __device__ __forceinline__ cpx<T> herm_pair(const cpx<T>* U, const cpx<T>* W, int k, int kz) {
   const bool mirror = 2 * k > N;
   const int src = mirror ? N - k : k;
-  if (src >= kz) return czero<T>();
-  cpx<T> u = U[src];
-  cpx<T> w = W ? W[src] : czero<T>();
-  if (src == 0 || 2 * src == N) {
-    u.y = T(0);
-    w.y = T(0);
-  }
+  const bool stored = src < kz;
+  const int at = stored ? src : 0;
+  cpx<T> u = U[at];
+  cpx<T> w = W ? W[at] : czero<T>();
+  const bool real_mode = src == 0 || 2 * src == N;
+  const T uy = (stored && !real_mode) ? u.y : T(0), wy = (stored && !real_mode) ? w.y : T(0);
+  const T ux = stored ? u.x : T(0), wx = stored ? w.x : T(0);
   // u + i w, or conj(u) + i conj(w) on the mirrored half
-  return mirror ? cpx<T>{u.x + w.y, w.x - u.y} : cpx<T>{u.x - w.y, u.y + w.x};
+  return mirror ? cpx<T>{ux + wy, wx - uy} : cpx<T>{ux - wy, uy + wx};
 }
 
 // Separate A = FFT(a + i b) into FFT(a)[k], FFT(b)[k] using A[N-k].
@@ -232,7 +240,7 @@ __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_fused(cpx<T>* buf
   constexpr int R = B::R, Q = B::Q, PW = B::PW;
   __shared__ cpx<T> lds[B::LDS];
   const int q = threadIdx.x % Q, p = threadIdx.x / Q;
-  const int64_t pp = int64_t(blockIdx.x) * PW + p;
+  const int64_t pp = int64_t(xcd_tile_rev(blockIdx.x, gridDim.x)) * PW + p;
   const bool active = pp < npencils;
   const int64_t base = (active ? pp : 0) * kz;
   cpx<T>* lp = lds + p * F::kSlots;
