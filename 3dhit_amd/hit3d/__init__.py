@@ -8,6 +8,7 @@ a GPU, but every operator needs the built libhit3d_hip.so and a device.
 """
 
 from .adjoint import differentiable_rollout, make_ns_rhs_vjp, rk4_step_vjp, rollout_vjp
+from .callbacks import FieldWriter, Hdf5Sink, MemorySink, save_series
 from .diagnostics import (
     component_spectra, dissipation, energy, energy_budget, energy_budget_cb,
     energy_spectrum, enstrophy,
@@ -22,7 +23,9 @@ from .grid import (
     to_spectral,
 )
 from .integrators import Callback, Diagnostic, State, evolve, rk4_step, rollout
+from .labels import label, run_label
 from .navierstokes import NSParams, is_dealiased, linear_operator, make_ns_rhs
+from .report import load_run, read_series
 
 __all__ = [
     "Grid", "make_grid", "spectral_shape", "ddx", "ddy", "ddz", "laplacian",
@@ -32,6 +35,8 @@ __all__ = [
     "NoForcing", "LinearForcing", "BandForcing", "LundgrenForcing",
     "NSParams", "make_ns_rhs", "linear_operator", "is_dealiased",
     "rk4_step", "rollout", "evolve", "State", "Callback", "Diagnostic",
+    "FieldWriter", "save_series", "Hdf5Sink", "MemorySink", "load_run", "read_series",
+    "label", "run_label",
     "energy", "enstrophy", "dissipation", "energy_spectrum", "component_spectra",
     "energy_budget", "energy_budget_cb",
     "make_ns_rhs_vjp", "rk4_step_vjp", "rollout_vjp", "differentiable_rollout",

@@ -10,12 +10,10 @@ host only enqueues. `evolve` is the host-side observation driver over
 from __future__ import annotations
 
 import ctypes
-import math
 import time
 
-import torch
-
 from . import _lib
+from .callbacks import Callback, Diagnostic, State
 from .grid import check_spectral, stream
 from .navierstokes import is_dealiased
 
@@ -50,68 +48,8 @@ def rollout(rhs, u_hat, params, dt, nsteps, t0=0.0):
 
 
 # ---------------------------------------------------------------------------
-# Observation driver (integrators.py:56-109, callbacks.py:33-105)
+# Observation driver (integrators.py:56-123)
 # ---------------------------------------------------------------------------
-
-
-class State(tuple):
-    """callbacks.py:33-40: read-only (u_hat, t, step, params, grid)."""
-
-    __slots__ = ()
-    _fields = ("u_hat", "t", "step", "params", "grid")
-
-    def __new__(cls, u_hat, t, step, params, grid):
-        return tuple.__new__(cls, (u_hat, t, step, params, grid))
-
-    u_hat = property(lambda s: s[0])
-    t = property(lambda s: s[1])
-    step = property(lambda s: s[2])
-    params = property(lambda s: s[3])
-    grid = property(lambda s: s[4])
-
-
-class Callback:
-    """callbacks.py:43-66: calls f(state) every `every` steps, or (when
-    `every_time` is set) at step 0 and whenever t crosses a multiple of
-    `every_time`."""
-
-    def __init__(self, f, *, every=1, every_time=None):
-        self.f, self.every, self.every_time = f, every, every_time
-
-    def is_due(self, step, t, dt):
-        if self.every_time is None:
-            return step % self.every == 0
-        return step == 0 or math.floor(t / self.every_time) > math.floor((t - dt) / self.every_time)
-
-    def fire(self, state):
-        self.f(state)
-
-    def finalize(self):
-        pass
-
-
-class Diagnostic:
-    """callbacks.py:69-105: records float(t) and f(state) every `every`
-    steps (values pulled to the host). HDF5 persistence (path/name) is not
-    part of this package."""
-
-    def __init__(self, f, *, every=1, path=None, name=None):
-        if path is not None and name is None:
-            raise ValueError("Diagnostic(..., path=...) also needs `name`")
-        if path is not None:
-            raise NotImplementedError("HDF5 series persistence is not implemented")
-        self.f, self.every, self.path, self.name = f, every, path, name
-        self.times, self.values = [], []
-
-    def is_due(self, step, t, dt):
-        return step % self.every == 0
-
-    def fire(self, state):
-        self.times.append(float(state.t))
-        self.values.append(self.f(state))
-
-    def finalize(self):
-        pass
 
 
 def evolve(rhs, u_hat, params, grid, dt, nsteps, *, t0=0.0, callbacks=(), progress=False):
@@ -143,9 +81,24 @@ def evolve(rhs, u_hat, params, grid, dt, nsteps, *, t0=0.0, callbacks=(), progre
         u_hat = rollout(rhs, u_hat, params, dt, n - prev, t0 + prev * dt)
         fire(n, u_hat)
         if print_every and (n % print_every == 0 or n == nsteps):
-            el = time.time() - wall0
-            print(f"step {n}/{nsteps}  t = {t0 + n * dt:<9.4g}  elapsed {el:.1f}s", flush=True)
+            _print_progress(n, nsteps, t0 + n * dt, wall0)
         prev = n
     for cb in callbacks:
         cb.finalize()
     return u_hat
+
+
+def _print_progress(n, nsteps, t, wall0):
+    """Progress line of integrators.py:112-120: step, percentage, simulation
+    time, wall time and the linear ETA."""
+    elapsed = time.time() - wall0
+    eta = elapsed / n * (nsteps - n)
+    width = len(str(nsteps))
+    print(f"step {n:{width}d}/{nsteps} ({round(100 * n / nsteps):3d}%)  "
+          f"t = {t:<9.4g}  elapsed {_fmt_secs(elapsed)}  eta {_fmt_secs(eta)}", flush=True)
+
+
+def _fmt_secs(s):
+    if s < 60:
+        return f"{s:.1f}s"
+    return f"{int(s // 60)}m{round(s % 60):02d}s"

@@ -1,0 +1,119 @@
+"""Host-side observation layer without a GPU: the HDF5 key schema, run
+labels and reprs, callback / snapshot scheduling and the progress line
+(reference tests/test_report_schema_labels.py:28-117 and
+tests/test_callbacks.py:64-106, whose expected values are used here)."""
+
+import math
+import os
+
+import numpy as np
+import pytest
+import torch
+
+hit3d = pytest.importorskip("hit3d")
+from hit3d import (BandForcing, Callback, Diagnostic, FieldWriter, LinearForcing, MemorySink,
+                   NoForcing, NSParams, label, make_grid, run_label)
+from hit3d.integrators import _fmt_secs, _print_progress
+from hit3d.schema import grid_key, is_step_key, series_field, series_key, step_key, step_num
+
+
+def test_schema():
+    assert step_key(50) == "step_00000050"
+    assert step_num("step_00000050") == 50
+    assert is_step_key("step_00000050")
+    assert not is_step_key("grid")
+    assert grid_key("Nx") == "grid/Nx"
+    assert series_key("energy_budget") == "series/energy_budget"
+    assert series_field("energy_budget", "E") == "series/energy_budget/E"
+
+
+def test_labels_and_reprs():
+    g = make_grid(16, dtype=torch.float64)
+    assert label(g) == "N16"
+    assert label(make_grid(16, 8, 4)) == "N16x8x4"
+    p = NSParams(nu=1e-3)
+    assert label(p) == "NavierStokes_nu0.001_NoForcing"
+    f = BandForcing(eps=0.1, kmin=1.0, kmax=2.5)
+    assert label(f) == "BandForcing_eps0.1_k1-2.5"
+    assert label(NSParams(nu=1e-3, forcing=f)) == "NavierStokes_nu0.001_BandForcing_eps0.1_k1-2.5"
+    lf = LinearForcing(A=0.1333)
+    assert label(lf) == "LinearForcing_A0.1333"
+    assert repr(lf) == "LinearForcing(A=0.1333)"
+    assert run_label(g, p) == "N16_NavierStokes_nu0.001_NoForcing_RK4"
+    assert label(NSParams(nu=np.float32(1e-3))) == "NavierStokes_nu0.001_NoForcing"
+    assert repr(g) == "Grid[float64](16x16x16, L=(6.28319, 6.28319, 6.28319))"
+    assert repr(NoForcing()) == "NoForcing()"
+    assert repr(BandForcing(eps=0.1, kmin=1.0, kmax=2.5)) == "BandForcing(eps=0.1, k=[1, 2.5])"
+    rp = repr(NSParams(nu=1e-3, forcing=f))
+    assert rp == "NavierStokes(nu=0.001, forcing=BandForcing(eps=0.1, k=[1, 2.5]))"
+
+
+def _due_steps(cb, dt, nsteps, t0=0.0):
+    return [n for n in range(nsteps + 1) if cb.is_due(n, t0 + n * dt, dt)]
+
+
+def test_fieldwriter_time_schedule():
+    # no snapshot before warmup_time, then every every_time regardless of dt:
+    # t = 1.0, 1.5, 2.0, 2.5, 3.0 -> steps 10, 15, 20, 25, 30
+    w = FieldWriter(sink=MemorySink(), every_time=0.5, warmup_time=1.0)
+    assert _due_steps(w, 0.1, 30) == [10, 15, 20, 25, 30]
+    # float32 accumulated times must not count a crossing twice
+    w = FieldWriter(sink=MemorySink(), every_time=0.1)
+    dt = np.float32(0.01)
+    t, due = np.float32(0.0), []
+    for n in range(101):
+        if w.is_due(n, float(t), float(dt)):
+            due.append(n)
+        t = np.float32(t + dt)
+    assert due == list(range(0, 101, 10))
+
+
+def test_step_schedules():
+    assert _due_steps(FieldWriter(sink=MemorySink(), every=50), 1e-2, 100) == [0, 50, 100]
+    assert _due_steps(Diagnostic(lambda s: 0.0, every=10), 1e-2, 100) == list(range(0, 101, 10))
+    # time-based Callback: step 0 plus each crossing of a multiple of 0.25
+    due = _due_steps(Callback(lambda s: None, every_time=0.25), 1e-2, 100)
+    assert due[0] == 0 and 4 <= len(due) <= 6
+    assert all(math.floor(n * 1e-2 / 0.25) > math.floor((n - 1) * 1e-2 / 0.25) for n in due[1:])
+
+
+def test_fieldwriter_overwrite_semantics(tmp_path):
+    path = str(tmp_path / "sub" / "snap.h5")
+    os.makedirs(os.path.dirname(path))
+    open(path, "w").close()
+    FieldWriter(path, sink=MemorySink(), overwrite=False)
+    assert os.path.exists(path)
+    FieldWriter(path, sink=MemorySink())          # default overwrite=True deletes
+    assert not os.path.exists(path)
+    FieldWriter(str(tmp_path / "new" / "dir" / "s.h5"), sink=MemorySink())
+    assert os.path.isdir(tmp_path / "new" / "dir")   # parent directories are created
+
+
+def test_diagnostic_path_without_name_errors():
+    with pytest.raises(ValueError):
+        Diagnostic(lambda s: 0.0, path="somewhere.h5")
+
+
+def test_hdf5_layer_fails_loudly_without_h5py(tmp_path):
+    try:
+        import h5py  # noqa: F401
+        pytest.skip("h5py is installed here")
+    except ImportError:
+        pass
+    with pytest.raises(ImportError, match="h5py"):
+        FieldWriter(str(tmp_path / "a.h5"))
+    with pytest.raises(ImportError, match="h5py"):
+        Diagnostic(lambda s: 0.0, path=str(tmp_path / "a.h5"), name="E")
+    with pytest.raises(ImportError, match="h5py"):
+        hit3d.load_run(str(tmp_path / "a.h5"))
+
+
+def test_progress_line(capsys):
+    import time
+    _print_progress(5, 10, 0.005, time.time() - 2.0)
+    _print_progress(10, 10, 0.01, time.time() - 4.0)
+    out = capsys.readouterr().out
+    assert out.count("\n") == 2
+    assert "step 10/10 (100%)" in out and "step  5/10 ( 50%)" in out
+    assert "t = " in out and "eta " in out
+    assert _fmt_secs(5.04) == "5.0s" and _fmt_secs(125.0) == "2m05s"
