@@ -28,6 +28,7 @@ UNITS = [
     ("passes_f32.o", "passes.hip", ["-DH3_REAL=float"]),
     ("passes_f64.o", "passes.hip", ["-DH3_REAL=double"]),
     ("misc.o", "misc.hip", []),
+    ("closure.o", "closure.hip", []),
     ("capi.o", "capi.hip", []),
 ]
 

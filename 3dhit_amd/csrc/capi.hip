@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/hit3d.h"
+#include "closure.hpp"
 #include "misc.hpp"
 #include "passes.hpp"
 
@@ -1064,3 +1065,205 @@ int hit3d_rk4_step_vjp(const hit3d_grid* grid, const void* u_hat, const void* ub
 }
 
 }  // extern "C"
+
+// ---- learned subgrid closure (no reference counterpart) --------------------
+
+namespace {
+
+constexpr int kConvChunk = 1024;  // voxels per backward-weight partial
+
+int check_conv(const hit3d_conv* c) {
+  if (!c) return fail(HIT3D_ERR_INVALID, "conv is NULL");
+  if (c->d < 1 || c->h < 1 || c->w < 16 || c->w % 16 != 0)
+    return fail(HIT3D_ERR_INVALID, "conv grid needs w a multiple of 16 and d, h >= 1");
+  const int64_t V = int64_t(c->d) * c->h * c->w;
+  if (V % kConvChunk != 0) return fail(HIT3D_ERR_INVALID, "conv voxel count must be a multiple of 1024");
+  if ((c->cin_pad != 16 && c->cin_pad != 32) || (c->cout_pad != 16 && c->cout_pad != 32))
+    return fail(HIT3D_ERR_INVALID, "stored channel counts must be 16 or 32");
+  if (c->cin < 1 || c->cin > c->cin_pad || c->cout < 1 || c->cout > c->cout_pad)
+    return fail(HIT3D_ERR_INVALID, "channel counts must be in [1, stored channels]");
+  return HIT3D_OK;
+}
+
+h3::ConvGeom conv_geom(const hit3d_conv* c) { return h3::ConvGeom{c->d, c->h, c->w}; }
+
+size_t conv_work_bytes(const hit3d_conv* c) {
+  const int64_t V = int64_t(c->d) * c->h * c->w, nchunk = V / kConvChunk;
+  size_t b = round_up(V * c->cout_pad * 2, 256);                               // dP (bf16)
+  b += round_up(nchunk * 27 * c->cout_pad * c->cin_pad * int64_t(sizeof(float)), 256);  // partials
+  b += round_up(nchunk * 4 * c->cout_pad * int64_t(sizeof(float)), 256);              // bias partials
+  return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hit3d_conv_sizes(const hit3d_conv* conv, int64_t* kp, int64_t* kt, size_t* work_bytes) {
+  H3_TRY(check_conv(conv));
+  if (kp) *kp = h3::conv_kp(conv->cin_pad);
+  if (kt) *kt = h3::conv_kt(conv->cout_pad);
+  if (work_bytes) *work_bytes = conv_work_bytes(conv);
+  return HIT3D_OK;
+}
+
+int hit3d_conv_pack(const hit3d_conv* conv, const float* w, void* wp, void* wt, void* stream) {
+  H3_TRY(check_conv(conv));
+  H3_TRY(check_ptr(w, "w"));
+  H3_TRY(check_ptr(wp, "wp"));
+  H3_TRY(check_ptr(wt, "wt"));
+  H3_HIP(h3::launch_pack_weights(w, conv->cout, conv->cin, conv->cout_pad, conv->cin_pad,
+                                 h3::conv_kp(conv->cin_pad), h3::conv_kt(conv->cout_pad),
+                                 static_cast<__bf16*>(wp), static_cast<__bf16*>(wt),
+                                 static_cast<hipStream_t>(stream)));
+  return HIT3D_OK;
+}
+
+int hit3d_conv_forward(const hit3d_conv* conv, const void* x, const void* wp, const float* bias,
+                       int32_t gelu, float* pre, void* act, float* out, void* stream) {
+  H3_TRY(check_conv(conv));
+  H3_TRY(check_ptr(x, "x"));
+  H3_TRY(check_ptr(wp, "wp"));
+  H3_TRY(check_ptr(bias, "bias"));
+  if (gelu) {
+    H3_TRY(check_ptr(pre, "pre"));
+    H3_TRY(check_ptr(act, "act"));
+  } else {
+    H3_TRY(check_ptr(out, "out"));
+  }
+  H3_HIP(h3::launch_conv_fwd(conv->cin_pad, conv->cout_pad, static_cast<const __bf16*>(x),
+                             static_cast<const __bf16*>(wp), h3::conv_kp(conv->cin_pad), bias,
+                             conv->cout, conv_geom(conv), gelu ? 0 : 1, pre,
+                             static_cast<__bf16*>(act), out, static_cast<hipStream_t>(stream)));
+  return HIT3D_OK;
+}
+
+int hit3d_conv_backward(const hit3d_conv* conv, const void* x, const void* wt, const float* pre,
+                        const float* grad_out, int32_t gelu, float* grad_x, float* grad_w,
+                        float* grad_b, void* work, size_t work_bytes, void* stream) {
+  H3_TRY(check_conv(conv));
+  H3_TRY(check_ptr(x, "x"));
+  H3_TRY(check_ptr(wt, "wt"));
+  H3_TRY(check_ptr(grad_out, "grad_out"));
+  H3_TRY(check_ptr(grad_w, "grad_w"));
+  H3_TRY(check_ptr(grad_b, "grad_b"));
+  if (gelu) H3_TRY(check_ptr(pre, "pre"));
+  H3_TRY(check_work(work, work_bytes, conv_work_bytes(conv)));
+  const int64_t V = int64_t(conv->d) * conv->h * conv->w, nchunk = V / kConvChunk;
+  char* base = static_cast<char*>(work);
+  __bf16* dP = reinterpret_cast<__bf16*>(base);
+  float* part = reinterpret_cast<float*>(base + round_up(V * conv->cout_pad * 2, 256));
+  float* pb = part + round_up(nchunk * 27 * conv->cout_pad * conv->cin_pad * 4, 256) / 4;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const h3::ConvGeom g = conv_geom(conv);
+  if (gelu)
+    H3_HIP(h3::launch_gelu_bwd(grad_out, pre, V * conv->cout_pad, conv->cout_pad, conv->cout, dP, s));
+  else
+    H3_HIP(h3::launch_to_bf16_padded(grad_out, V, conv->cout, conv->cout_pad, dP, s));
+  if (grad_x)
+    H3_HIP(h3::launch_conv_bwd_data(conv->cout_pad, conv->cin_pad, dP, static_cast<const __bf16*>(wt),
+                                    h3::conv_kt(conv->cout_pad), g, grad_x, s));
+  H3_HIP(h3::launch_conv_bwd_weight(conv->cout_pad, conv->cin_pad, dP, static_cast<const __bf16*>(x), g,
+                                    kConvChunk, part, pb, s));
+  H3_HIP(h3::launch_conv_reduce(part, pb, int(nchunk), conv->cout_pad, conv->cin_pad, conv->cout,
+                                conv->cin, grad_w, grad_b, s));
+  return HIT3D_OK;
+}
+
+int hit3d_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float beta1,
+               float beta2, float eps, int64_t t, void* stream) {
+  H3_TRY(check_ptr(p, "p"));
+  H3_TRY(check_ptr(g, "g"));
+  H3_TRY(check_ptr(m, "m"));
+  H3_TRY(check_ptr(v, "v"));
+  if (n < 1 || t < 1) return fail(HIT3D_ERR_INVALID, "adam needs n >= 1 and step t >= 1");
+  const float c1 = float(1.0 - std::pow(double(beta1), double(t)));
+  const float c2 = float(1.0 - std::pow(double(beta2), double(t)));
+  H3_HIP(h3::launch_adam(p, g, m, v, n, lr, beta1, beta2, eps, c1, c2, static_cast<hipStream_t>(stream)));
+  return HIT3D_OK;
+}
+
+int hit3d_channels_pack(const float* soa, int32_t c, int64_t nvox, const float* scale, int32_t cpad,
+                        void* aos, void* stream) {
+  H3_TRY(check_ptr(soa, "soa"));
+  H3_TRY(check_ptr(scale, "scale"));
+  H3_TRY(check_ptr(aos, "aos"));
+  if (c < 1 || c > cpad || nvox < 1) return fail(HIT3D_ERR_INVALID, "need 1 <= c <= cpad, nvox >= 1");
+  H3_HIP(h3::launch_soa_aos(soa, c, nvox, scale, cpad, 0, static_cast<__bf16*>(aos), nullptr, nullptr,
+                            static_cast<hipStream_t>(stream)));
+  return HIT3D_OK;
+}
+
+int hit3d_channels_unpack(const float* aos, int32_t c, int32_t ld, int64_t nvox, const float* scale,
+                          float* soa, void* stream) {
+  H3_TRY(check_ptr(aos, "aos"));
+  H3_TRY(check_ptr(scale, "scale"));
+  H3_TRY(check_ptr(soa, "soa"));
+  if (c < 1 || c > ld || nvox < 1) return fail(HIT3D_ERR_INVALID, "need 1 <= c <= ld, nvox >= 1");
+  H3_HIP(h3::launch_soa_aos(nullptr, c, nvox, scale, ld, 1, nullptr, aos, soa,
+                            static_cast<hipStream_t>(stream)));
+  return HIT3D_OK;
+}
+
+int hit3d_spectral_truncate(const hit3d_grid* fine, const void* u_fine, const hit3d_grid* coarse,
+                            void* u_coarse, int64_t ncomp, double kcut, void* stream) {
+  H3_TRY(check_grid(fine));
+  H3_TRY(check_grid(coarse));
+  H3_TRY(check_ptr(u_fine, "u_fine"));
+  H3_TRY(check_ptr(u_coarse, "u_coarse"));
+  if (fine->precision != coarse->precision) return fail(HIT3D_ERR_INVALID, "precisions differ");
+  if (coarse->nx > fine->nx || coarse->ny > fine->ny || coarse->nz > fine->nz)
+    return fail(HIT3D_ERR_INVALID, "the coarse grid must not exceed the fine grid");
+  if (fine->lx != coarse->lx || fine->ly != coarse->ly || fine->lz != coarse->lz)
+    return fail(HIT3D_ERR_INVALID, "fine and coarse boxes differ");
+  if (fine->lx != fine->ly || fine->lx != fine->lz)
+    return fail(HIT3D_ERR_INVALID, "the spherical filter needs a cubic box");
+  if (ncomp < 1 || ncomp > 64 || !(kcut >= 0)) return fail(HIT3D_ERR_INVALID, "bad ncomp or kcut");
+  // integer frequencies n kept when |n| k0 <= kcut
+  const double k0 = 2.0 * M_PI / fine->lx;
+  const double kc2 = (kcut / k0) * (kcut / k0);
+  const double scale = double(coarse->nx) * coarse->ny * coarse->nz / (double(fine->nx) * fine->ny * fine->nz);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (fine->precision == 4)
+    H3_HIP(h3::launch_truncate<float>(static_cast<const cpx<float>*>(u_fine), int(fine->nx), int(fine->ny),
+                                      int(fine->nz), static_cast<cpx<float>*>(u_coarse), int(coarse->nx),
+                                      int(coarse->ny), int(coarse->nz), int(ncomp), kc2, float(scale), s));
+  else
+    H3_HIP(h3::launch_truncate<double>(static_cast<const cpx<double>*>(u_fine), int(fine->nx), int(fine->ny),
+                                       int(fine->nz), static_cast<cpx<double>*>(u_coarse), int(coarse->nx),
+                                       int(coarse->ny), int(coarse->nz), int(ncomp), kc2, scale, s));
+  return HIT3D_OK;
+}
+
+int hit3d_velocity_products(const hit3d_grid* grid, const void* u, void* uu, void* stream) {
+  H3_TRY(check_grid(grid));
+  H3_TRY(check_ptr(u, "u"));
+  H3_TRY(check_ptr(uu, "uu"));
+  const int64_t V = grid->nx * grid->ny * grid->nz;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (grid->precision == 4)
+    H3_HIP(h3::launch_products<float>(static_cast<const float*>(u), V, static_cast<float*>(uu), s));
+  else
+    H3_HIP(h3::launch_products<double>(static_cast<const double*>(u), V, static_cast<double*>(uu), s));
+  return HIT3D_OK;
+}
+
+int hit3d_stress_divergence(const hit3d_grid* grid, const void* in, void* out, int32_t adjoint,
+                            void* stream) {
+  H3_TRY(check_grid(grid));
+  H3_TRY(check_ptr(in, "in"));
+  H3_TRY(check_ptr(out, "out"));
+  if (in == out) return fail(HIT3D_ERR_INVALID, "in and out must not alias");
+  const Geo g = make_geo(*grid, false);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (grid->precision == 4)
+    H3_HIP(h3::launch_stress_div<float>(static_cast<const cpx<float>*>(in), static_cast<cpx<float>*>(out), g,
+                                        adjoint ? 1 : 0, s));
+  else
+    H3_HIP(h3::launch_stress_div<double>(static_cast<const cpx<double>*>(in), static_cast<cpx<double>*>(out),
+                                         g, adjoint ? 1 : 0, s));
+  return HIT3D_OK;
+}
+
+}  // extern "C"
+

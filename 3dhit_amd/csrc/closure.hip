@@ -1,0 +1,567 @@
+// Learned subgrid closure for the coarse LES (gfx950): periodic 3x3x3
+// convolutions as implicit GEMMs on BF16 MFMA with FP32 accumulation, their
+// backward passes, the optimiser step, and the spectral/physical kernels of
+// the filtered-DNS pipeline (spectral truncation, velocity products, the
+// divergence of a stress tensor). The reference has no counterpart: the
+// closure is an addition to hit3d's solver.
+//
+// Layouts. A field of V = D*H*W voxels (v = (a*H + b)*W + c, c fastest: the
+// physical layout real[nx][ny][nz] of the solver) with C channels is stored
+// channels-last, X[v][C]. A layer's weights w[co][o][ci] (o = 9*(dz+1) +
+// 3*(dy+1) + (dx+1) over the 27 neighbour offsets) are packed for the MFMA A
+// operand as Wp[COUT][KP] with k = o*CIN + ci (zero beyond 27*CIN), and
+// transposed for the backward-data GEMM as WT[CIN][KT] with k = o*COUT + co.
+//
+// v_mfma_f32_16x16x32_bf16 lane layout (lane l, element e): A[i][k] at
+// l = i + 16*(k/8), e = k%8; B[k][j] at l = j + 16*(k/8), e = k%8;
+// D[i][j] at l = j + 16*(i/4), e = i%4.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "closure.hpp"
+
+namespace h3 {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// neighbour of voxel (a, b, c) at offset o
+__device__ __forceinline__ int neighbour(int a, int b, int c, int o, const ConvGeom& g) {
+  const int da = o / 9 - 1, db = (o / 3) % 3 - 1, dc = o % 3 - 1;
+  return (wrap(a + da, g.D) * g.H + wrap(b + db, g.H)) * g.W + wrap(c + dc, g.W);
+}
+
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
+  const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
+  return cdf + x * pdf;
+}
+
+// ---------------------------------------------------------------------------
+// Forward: out[v][co] = bias[co] + sum_{o,ci} w[co][o][ci] X[nbr(v,o)][ci].
+// One wave computes COUT output channels for 64 consecutive voxels (4 column
+// blocks of 16, same row since W % 16 == 0).
+// CIN: stored input channels (16: two offsets per K-step, 32: one).
+// mode 0: pre (fp32 [V][COUT]) and act = GELU(pre) (bf16 [V][COUT]);
+// mode 1: out (fp32 [V][cout_real]) only.
+// ---------------------------------------------------------------------------
+template <int CIN, int COUT>
+__global__ __launch_bounds__(64) void k_conv_fwd(const __bf16* __restrict__ X,
+                                                 const __bf16* __restrict__ Wp, int KP,
+                                                 const float* __restrict__ bias, int cout_real,
+                                                 ConvGeom g, int mode, float* __restrict__ pre,
+                                                 __bf16* __restrict__ act, float* __restrict__ out) {
+  constexpr int RB = COUT / 16;
+  static_assert(CIN == 16 || CIN == 32, "CIN must be 16 or 32");
+  const int l = threadIdx.x, li = l % 16, lg = l / 16;
+  const int v0 = blockIdx.x * 64;
+  int va[4], vb[4], vc[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const int v = v0 + 16 * b + li;
+    vc[b] = v % g.W;
+    vb[b] = (v / g.W) % g.H;
+    va[b] = v / (g.W * g.H);
+  }
+  f32x4 acc[RB][4];
+#pragma unroll
+  for (int r = 0; r < RB; ++r)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[r][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int ksteps = KP / 32;
+  for (int s = 0; s < ksteps; ++s) {
+    const int k = 32 * s + 8 * lg;          // this lane's first k
+    const int o = k / CIN, ci = k % CIN;    // offset and first channel
+    bf16x8 A[RB], B[4];
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+      A[r] = *reinterpret_cast<const bf16x8*>(Wp + int64_t(16 * r + li) * KP + k);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      if (o < 27) {
+        const int n = neighbour(va[b], vb[b], vc[b], o, g);
+        B[b] = *reinterpret_cast<const bf16x8*>(X + int64_t(n) * CIN + ci);
+      } else {
+        B[b] = bf16x8{};  // K padding (the packed weights are zero there too)
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[r][b] = mfma(A[r], B[b], acc[r][b]);
+  }
+#pragma unroll
+  for (int r = 0; r < RB; ++r)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int v = v0 + 16 * b + li;
+      const int co0 = 16 * r + 4 * lg;
+      float y[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) y[e] = acc[r][b][e] + (co0 + e < cout_real ? bias[co0 + e] : 0.f);
+      if (mode == 0) {
+        *reinterpret_cast<float4*>(pre + int64_t(v) * COUT + co0) = make_float4(y[0], y[1], y[2], y[3]);
+        __bf16 h[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) h[e] = (__bf16)(co0 + e < cout_real ? gelu(y[e]) : 0.f);
+        *reinterpret_cast<uint2*>(act + int64_t(v) * COUT + co0) = *reinterpret_cast<uint2*>(h);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (co0 + e < cout_real) out[int64_t(v) * cout_real + co0 + e] = y[e];
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Backward data: dX[u][ci] = sum_{o,co} w[co][o][ci] dP[u - o][co]
+// (D[ci][u] = WT[ci][k] * dP[u - o(k)][co(k)], k = o*COUT + co).
+// dP bf16 [V][COUT]; dX fp32 [V][CIN].
+// ---------------------------------------------------------------------------
+template <int COUT, int CIN>
+__global__ __launch_bounds__(64) void k_conv_bwd_data(const __bf16* __restrict__ dP,
+                                                      const __bf16* __restrict__ WT, int KT,
+                                                      ConvGeom g, float* __restrict__ dX) {
+  constexpr int RB = CIN / 16;
+  static_assert(COUT == 16 || COUT == 32, "COUT must be 16 or 32");
+  const int l = threadIdx.x, li = l % 16, lg = l / 16;
+  const int u0 = blockIdx.x * 64;
+  int ua[4], ub[4], uc[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const int u = u0 + 16 * b + li;
+    uc[b] = u % g.W;
+    ub[b] = (u / g.W) % g.H;
+    ua[b] = u / (g.W * g.H);
+  }
+  f32x4 acc[RB][4];
+#pragma unroll
+  for (int r = 0; r < RB; ++r)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[r][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int ksteps = KT / 32;
+  for (int s = 0; s < ksteps; ++s) {
+    const int k = 32 * s + 8 * lg;
+    const int o = k / COUT, co = k % COUT;
+    bf16x8 A[RB], B[4];
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+      A[r] = *reinterpret_cast<const bf16x8*>(WT + int64_t(16 * r + li) * KT + k);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      if (o < 27) {
+        const int n = neighbour(ua[b], ub[b], uc[b], 26 - o, g);  // u - o: the mirrored offset
+        B[b] = *reinterpret_cast<const bf16x8*>(dP + int64_t(n) * COUT + co);
+      } else {
+        B[b] = bf16x8{};
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[r][b] = mfma(A[r], B[b], acc[r][b]);
+  }
+#pragma unroll
+  for (int r = 0; r < RB; ++r)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int u = u0 + 16 * b + li;
+      const int ci0 = 16 * r + 4 * lg;
+      *reinterpret_cast<float4*>(dX + int64_t(u) * CIN + ci0) =
+          make_float4(acc[r][b][0], acc[r][b][1], acc[r][b][2], acc[r][b][3]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Backward weights, per voxel chunk and group of 9 offsets:
+//   part[chunk][o][co][ci] = sum_{u in chunk} dP[u][co] X[u + o][ci]
+// (D[co][ci] = dP^T[co][u] * Xo[u][ci], K = the chunk's voxels), and for the
+// first group the bias partials pb[chunk][co] = sum_u dP[u][co].
+// ---------------------------------------------------------------------------
+template <int COUT, int CIN>
+__global__ __launch_bounds__(64) void k_conv_bwd_weight(const __bf16* __restrict__ dP,
+                                                        const __bf16* __restrict__ X, ConvGeom g,
+                                                        int chunk, float* __restrict__ part,
+                                                        float* __restrict__ pb) {
+  constexpr int RB = COUT / 16, CB = CIN / 16, OG = 9;
+  const int l = threadIdx.x, li = l % 16, lg = l / 16;
+  const int c = blockIdx.x, og = blockIdx.y;
+  f32x4 acc[OG][RB][CB];
+#pragma unroll
+  for (int o = 0; o < OG; ++o)
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+      for (int q = 0; q < CB; ++q) acc[o][r][q] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float bsum[RB];
+#pragma unroll
+  for (int r = 0; r < RB; ++r) bsum[r] = 0.f;
+  const int ubeg = c * chunk;
+  for (int u0 = ubeg; u0 < ubeg + chunk; u0 += 32) {
+    // A operand: dP[u][co] for u = u0 + 8*lg + e, co = 16 r + li
+    bf16x8 A[RB];
+    int ua[8], ub[8], uc[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int u = u0 + 8 * lg + e;
+      uc[e] = u % g.W;
+      ub[e] = (u / g.W) % g.H;
+      ua[e] = u / (g.W * g.H);
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const __bf16 x = dP[int64_t(u) * COUT + 16 * r + li];
+        A[r][e] = x;
+        bsum[r] += float(x);
+      }
+    }
+#pragma unroll
+    for (int o = 0; o < OG; ++o) {
+      bf16x8 B[CB];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int n = neighbour(ua[e], ub[e], uc[e], OG * og + o, g);
+#pragma unroll
+        for (int q = 0; q < CB; ++q) B[q][e] = X[int64_t(n) * CIN + 16 * q + li];
+      }
+#pragma unroll
+      for (int r = 0; r < RB; ++r)
+#pragma unroll
+        for (int q = 0; q < CB; ++q) acc[o][r][q] = mfma(A[r], B[q], acc[o][r][q]);
+    }
+  }
+  // D[co][ci]: lane holds ci = 16 q + li, co = 16 r + 4 lg + e
+#pragma unroll
+  for (int o = 0; o < OG; ++o)
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+      for (int q = 0; q < CB; ++q)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int co = 16 * r + 4 * lg + e, ci = 16 * q + li;
+          part[((int64_t(c) * 27 + OG * og + o) * COUT + co) * CIN + ci] = acc[o][r][q][e];
+        }
+  if (og == 0) {
+    // lanes li, li+16, li+32, li+48 hold disjoint voxel sums of channel 16 r + li
+#pragma unroll
+    for (int r = 0; r < RB; ++r) pb[(int64_t(c) * 4 + lg) * COUT + 16 * r + li] = bsum[r];
+  }
+}
+
+// dW[co][o][ci] (real sizes) and db[co] from the chunk partials, summed in
+// chunk order (deterministic).
+__global__ void k_conv_reduce(const float* __restrict__ part, const float* __restrict__ pb,
+                              int nchunk, int COUT, int CIN, int cout_real, int cin_real,
+                              float* __restrict__ dW, float* __restrict__ db) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nw = cout_real * 27 * cin_real;
+  if (i < nw) {
+    const int co = i / (27 * cin_real), o = (i / cin_real) % 27, ci = i % cin_real;
+    float s = 0.f;
+    for (int c = 0; c < nchunk; ++c) s += part[((int64_t(c) * 27 + o) * COUT + co) * CIN + ci];
+    dW[i] = s;
+  } else if (i < nw + cout_real) {
+    const int co = i - nw;
+    float s = 0.f;
+    for (int c = 0; c < nchunk * 4; ++c) s += pb[int64_t(c) * COUT + co];
+    db[co] = s;
+  }
+}
+
+// dP = dA * GELU'(pre) in bf16 (channels beyond cout_real are zero).
+__global__ void k_gelu_bwd(const float* __restrict__ dA, const float* __restrict__ pre,
+                           int64_t n, int C, int creal, __bf16* __restrict__ dP) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int ch = int(i % C);
+    dP[i] = (__bf16)(ch < creal ? dA[i] * gelu_grad(pre[i]) : 0.f);
+  }
+}
+
+// fp32 [V][C] -> bf16 [V][C] (the last layer's output gradient).
+__global__ void k_to_bf16_padded(const float* __restrict__ src, int64_t nvox, int creal, int C,
+                                 __bf16* __restrict__ dst) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nvox * C;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t v = i / C;
+    const int ch = int(i % C);
+    dst[i] = (__bf16)(ch < creal ? src[v * creal + ch] : 0.f);
+  }
+}
+
+// Packs w[co][o][ci] (fp32, real sizes) into Wp[COUT][KP] and WT[CIN][KT].
+__global__ void k_pack_weights(const float* __restrict__ w, int cout_real, int cin_real, int COUT,
+                               int CIN, int KP, int KT, __bf16* __restrict__ Wp,
+                               __bf16* __restrict__ WT) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int np = COUT * KP, nt = CIN * KT;
+  if (i < np) {
+    const int co = i / KP, k = i % KP, o = k / CIN, ci = k % CIN;
+    const float x = (co < cout_real && o < 27 && ci < cin_real)
+                        ? w[(co * 27 + o) * cin_real + ci] : 0.f;
+    Wp[i] = (__bf16)x;
+  } else if (i < np + nt) {
+    const int j = i - np, ci = j / KT, k = j % KT, o = k / COUT, co = k % COUT;
+    const float x = (co < cout_real && o < 27 && ci < cin_real)
+                        ? w[(co * 27 + o) * cin_real + ci] : 0.f;
+    WT[j] = (__bf16)x;
+  }
+}
+
+// Adam (Kingma & Ba 2015) with bias correction; c1 = 1 - beta1^t, c2 = 1 - beta2^t.
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ gr, float* __restrict__ m,
+                       float* __restrict__ v, int64_t n, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const float gi = gr[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  }
+}
+
+// SoA fp32 [C][V] (scaled by s[c]) <-> channels-last [V][CP]: dir 0 writes
+// bf16 channels-last (zero channels >= C), dir 1 reads fp32 channels-last
+// (ld = its channel stride) into SoA.
+__global__ void k_soa_aos(const float* __restrict__ soa, int C, int64_t V, const float* __restrict__ s,
+                          int CP, int dir, __bf16* __restrict__ aos_bf, const float* __restrict__ aos_f,
+                          float* __restrict__ soa_out) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < V * CP;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t v = i / CP;
+    const int ch = int(i % CP);
+    if (dir == 0) {
+      aos_bf[i] = (__bf16)(ch < C ? soa[ch * V + v] * s[ch] : 0.f);
+    } else if (ch < C) {
+      soa_out[ch * V + v] = aos_f[i] * s[ch];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Spectral pieces of the filtered-DNS pipeline and the closure force.
+// ---------------------------------------------------------------------------
+
+// coarse[c][ix][iy][iz] = scale * fine[c][...] for integer frequencies with
+// n2 = nx^2 + ny^2 + nz^2 <= kc2 (sharp spherical filter), 0 elsewhere.
+template <class T>
+__global__ void k_truncate(const cpx<T>* __restrict__ fine, int fx, int fy, int fz,
+                           cpx<T>* __restrict__ coarse, int cx, int cy, int cz, int ncomp,
+                           double kc2, T scale) {
+  const int czh = cz / 2 + 1, fzh = fz / 2 + 1;
+  const int64_t n = int64_t(cx) * cy * czh;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n * ncomp;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int comp = int(i / n);
+    const int64_t r = i % n;
+    const int iz = int(r % czh), iy = int((r / czh) % cy), ix = int(r / (int64_t(czh) * cy));
+    const int nx = ix < cx / 2 ? ix : ix - cx, ny = iy < cy / 2 ? iy : iy - cy, nz = iz;
+    const bool inside = 2 * abs(nx) < cx && 2 * abs(ny) < cy && 2 * nz < cz &&
+                        double(nx) * nx + double(ny) * ny + double(nz) * nz <= kc2;
+    cpx<T> val{T(0), T(0)};
+    if (inside) {
+      const int jx = nx >= 0 ? nx : nx + fx, jy = ny >= 0 ? ny : ny + fy;
+      const cpx<T> f = fine[int64_t(comp) * fx * fy * fzh + (int64_t(jx) * fy + jy) * fzh + nz];
+      val = {f.x * scale, f.y * scale};
+    }
+    coarse[i] = val;
+  }
+}
+
+// uu[6][V] = (u0u0, u0u1, u0u2, u1u1, u1u2, u2u2) from u[3][V].
+template <class T>
+__global__ void k_products(const T* __restrict__ u, int64_t V, T* __restrict__ uu) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < V;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const T a = u[i], b = u[V + i], c = u[2 * V + i];
+    uu[i] = a * a;
+    uu[V + i] = a * b;
+    uu[2 * V + i] = a * c;
+    uu[3 * V + i] = b * b;
+    uu[4 * V + i] = b * c;
+    uu[5 * V + i] = c * c;
+  }
+}
+
+// Stress index of (i, j) in the (00, 01, 02, 11, 12, 22) order.
+__device__ __forceinline__ int sidx(int i, int j) {
+  const int a = i < j ? i : j, b = i < j ? j : i;
+  return a == 0 ? b : (a == 1 ? 2 + b : 5);
+}
+
+// dir 0: F_i = -sum_j i k_j tau_ij (6 -> 3 fields);
+// dir 1 (adjoint in the convention dL = Re sum conj(g) dx):
+//   gtau_ij = i k_j gF_i + i k_i gF_j (i != j), i k_i gF_i (i == j).
+template <class T>
+__global__ void k_stress_div(const cpx<T>* __restrict__ in, cpx<T>* __restrict__ out, Geo g, int dir) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  for (int64_t m = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; m < n;
+       m += int64_t(gridDim.x) * blockDim.x) {
+    const int iz = int(m % g.nzh), iy = int((m / g.nzh) % g.ny), ix = int(m / (int64_t(g.nzh) * g.ny));
+    // integer frequencies as numpy.fft.fftfreq (index N/2 -> -N/2), the grid's convention
+    const T k[3] = {wavenumber<T>(g.k0x, ix < g.nx / 2 ? ix : ix - g.nx),
+                    wavenumber<T>(g.k0y, iy < g.ny / 2 ? iy : iy - g.ny),
+                    wavenumber<T>(g.k0z, iz)};
+    if (dir == 0) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        cpx<T> s{T(0), T(0)};
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const cpx<T> t = in[sidx(i, j) * n + m];
+          s = {s.x + k[j] * t.x, s.y + k[j] * t.y};
+        }
+        out[i * n + m] = {s.y, -s.x};  // -i * s
+      }
+    } else {
+      cpx<T> gf[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) gf[i] = in[i * n + m];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = i; j < 3; ++j) {
+          cpx<T> s{k[j] * gf[i].x, k[j] * gf[i].y};
+          if (j != i) s = {s.x + k[i] * gf[j].x, s.y + k[i] * gf[j].y};
+          out[sidx(i, j) * n + m] = {-s.y, s.x};  // i * s
+        }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+
+static inline unsigned grid_for(int64_t n, int threads = 256) {
+  int64_t b = (n + threads - 1) / threads;
+  return unsigned(b < 65536 ? (b > 0 ? b : 1) : 65536);
+}
+
+hipError_t launch_conv_fwd(int cin, int cout, const __bf16* X, const __bf16* Wp, int KP,
+                           const float* bias, int cout_real, const ConvGeom& g, int mode, float* pre,
+                           __bf16* act, float* out, hipStream_t s) {
+  const unsigned nb = unsigned(int64_t(g.D) * g.H * g.W / 64);
+#define H3_CONV_FWD(CI, CO) \
+  hipLaunchKernelGGL((k_conv_fwd<CI, CO>), dim3(nb), dim3(64), 0, s, X, Wp, KP, bias, cout_real, g, mode, pre, act, out)
+  if (cin == 16 && cout == 32) H3_CONV_FWD(16, 32);
+  else if (cin == 32 && cout == 32) H3_CONV_FWD(32, 32);
+  else if (cin == 32 && cout == 16) H3_CONV_FWD(32, 16);
+  else if (cin == 16 && cout == 16) H3_CONV_FWD(16, 16);
+  else return hipErrorInvalidValue;
+#undef H3_CONV_FWD
+  return hipGetLastError();
+}
+
+hipError_t launch_conv_bwd_data(int cout, int cin, const __bf16* dP, const __bf16* WT, int KT,
+                                const ConvGeom& g, float* dX, hipStream_t s) {
+  const unsigned nb = unsigned(int64_t(g.D) * g.H * g.W / 64);
+#define H3_CONV_BD(CO, CI) hipLaunchKernelGGL((k_conv_bwd_data<CO, CI>), dim3(nb), dim3(64), 0, s, dP, WT, KT, g, dX)
+  if (cout == 32 && cin == 16) H3_CONV_BD(32, 16);
+  else if (cout == 32 && cin == 32) H3_CONV_BD(32, 32);
+  else if (cout == 16 && cin == 32) H3_CONV_BD(16, 32);
+  else if (cout == 16 && cin == 16) H3_CONV_BD(16, 16);
+  else return hipErrorInvalidValue;
+#undef H3_CONV_BD
+  return hipGetLastError();
+}
+
+hipError_t launch_conv_bwd_weight(int cout, int cin, const __bf16* dP, const __bf16* X,
+                                  const ConvGeom& g, int chunk, float* part, float* pb,
+                                  hipStream_t s) {
+  const int64_t V = int64_t(g.D) * g.H * g.W;
+  const dim3 grid(unsigned(V / chunk), 3);
+#define H3_CONV_BW(CO, CI) hipLaunchKernelGGL((k_conv_bwd_weight<CO, CI>), grid, dim3(64), 0, s, dP, X, g, chunk, part, pb)
+  if (cout == 32 && cin == 16) H3_CONV_BW(32, 16);
+  else if (cout == 32 && cin == 32) H3_CONV_BW(32, 32);
+  else if (cout == 16 && cin == 32) H3_CONV_BW(16, 32);
+  else if (cout == 16 && cin == 16) H3_CONV_BW(16, 16);
+  else return hipErrorInvalidValue;
+#undef H3_CONV_BW
+  return hipGetLastError();
+}
+
+hipError_t launch_conv_reduce(const float* part, const float* pb, int nchunk, int cout, int cin,
+                              int cout_real, int cin_real, float* dW, float* db, hipStream_t s) {
+  const int n = cout_real * 27 * cin_real + cout_real;
+  hipLaunchKernelGGL(k_conv_reduce, dim3((n + 255) / 256), dim3(256), 0, s, part, pb, nchunk, cout, cin,
+                     cout_real, cin_real, dW, db);
+  return hipGetLastError();
+}
+
+hipError_t launch_gelu_bwd(const float* dA, const float* pre, int64_t n, int C, int creal, __bf16* dP,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_gelu_bwd, dim3(grid_for(n)), dim3(256), 0, s, dA, pre, n, C, creal, dP);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_bf16_padded(const float* src, int64_t nvox, int creal, int C, __bf16* dst,
+                                 hipStream_t s) {
+  hipLaunchKernelGGL(k_to_bf16_padded, dim3(grid_for(nvox * C)), dim3(256), 0, s, src, nvox, creal, C, dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_weights(const float* w, int cout_real, int cin_real, int cout, int cin, int KP,
+                               int KT, __bf16* Wp, __bf16* WT, hipStream_t s) {
+  const int n = cout * KP + cin * KT;
+  hipLaunchKernelGGL(k_pack_weights, dim3((n + 255) / 256), dim3(256), 0, s, w, cout_real, cin_real, cout,
+                     cin, KP, KT, Wp, WT);
+  return hipGetLastError();
+}
+
+hipError_t launch_adam(float* p, const float* gr, float* m, float* v, int64_t n, float lr, float b1,
+                       float b2, float eps, float c1, float c2, hipStream_t s) {
+  hipLaunchKernelGGL(k_adam, dim3(grid_for(n)), dim3(256), 0, s, p, gr, m, v, n, lr, b1, b2, eps, c1, c2);
+  return hipGetLastError();
+}
+
+hipError_t launch_soa_aos(const float* soa, int C, int64_t V, const float* scale, int CP, int dir,
+                          __bf16* aos_bf, const float* aos_f, float* soa_out, hipStream_t s) {
+  hipLaunchKernelGGL(k_soa_aos, dim3(grid_for(V * CP)), dim3(256), 0, s, soa, C, V, scale, CP, dir, aos_bf,
+                     aos_f, soa_out);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_truncate(const cpx<T>* fine, int fx, int fy, int fz, cpx<T>* coarse, int cx, int cy,
+                           int cz, int ncomp, double kc2, T scale, hipStream_t s) {
+  const int64_t n = int64_t(cx) * cy * (cz / 2 + 1) * ncomp;
+  hipLaunchKernelGGL((k_truncate<T>), dim3(grid_for(n)), dim3(256), 0, s, fine, fx, fy, fz, coarse, cx, cy,
+                     cz, ncomp, kc2, scale);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_products(const T* u, int64_t V, T* uu, hipStream_t s) {
+  hipLaunchKernelGGL((k_products<T>), dim3(grid_for(V)), dim3(256), 0, s, u, V, uu);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_stress_div(const cpx<T>* in, cpx<T>* out, const Geo& g, int dir, hipStream_t s) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  hipLaunchKernelGGL((k_stress_div<T>), dim3(grid_for(n)), dim3(256), 0, s, in, out, g, dir);
+  return hipGetLastError();
+}
+
+#define H3_CLOSURE_INST(T)                                                                          \
+  template hipError_t launch_truncate<T>(const cpx<T>*, int, int, int, cpx<T>*, int, int, int, int,  \
+                                         double, T, hipStream_t);                                    \
+  template hipError_t launch_products<T>(const T*, int64_t, T*, hipStream_t);                        \
+  template hipError_t launch_stress_div<T>(const cpx<T>*, cpx<T>*, const Geo&, int, hipStream_t);
+H3_CLOSURE_INST(float)
+H3_CLOSURE_INST(double)
+
+}  // namespace h3

@@ -33,8 +33,17 @@ class ForcingDesc(ctypes.Structure):
     ]
 
 
+class ConvDesc(ctypes.Structure):
+    _fields_ = [
+        ("d", ctypes.c_int32), ("h", ctypes.c_int32), ("w", ctypes.c_int32),
+        ("cin", ctypes.c_int32), ("cout", ctypes.c_int32),
+        ("cin_pad", ctypes.c_int32), ("cout_pad", ctypes.c_int32),
+    ]
+
+
 _P = ctypes.c_void_p
 _G = ctypes.POINTER(GridDesc)
+_C = ctypes.POINTER(ConvDesc)
 _F = ctypes.POINTER(ForcingDesc)
 _SZ = ctypes.c_size_t
 _I32, _I64, _D = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
@@ -60,6 +69,17 @@ SIGNATURES = {
     "hit3d_profile_rk4": [_G, _P, _D, _D, _I64, _F, _I32, _P, _P, _P, _SZ, _P],
     "hit3d_rhs_vjp": [_G, _P, _P, _P, _D, _F, _D, _P, _SZ, _P],
     "hit3d_rk4_step_vjp": [_G, _P, _P, _P, _D, _D, _D, _F, _P, _SZ, _P],
+    "hit3d_conv_sizes": [_C, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_SZ)],
+    "hit3d_conv_pack": [_C, _P, _P, _P, _P],
+    "hit3d_conv_forward": [_C, _P, _P, _P, _I32, _P, _P, _P, _P],
+    "hit3d_conv_backward": [_C, _P, _P, _P, _P, _I32, _P, _P, _P, _P, _SZ, _P],
+    "hit3d_adam": [_P, _P, _P, _P, _I64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                   ctypes.c_float, _I64, _P],
+    "hit3d_channels_pack": [_P, _I32, _I64, _P, _I32, _P, _P],
+    "hit3d_channels_unpack": [_P, _I32, _I32, _I64, _P, _P, _P],
+    "hit3d_spectral_truncate": [_G, _P, _G, _P, _I64, _D, _P],
+    "hit3d_velocity_products": [_G, _P, _P, _P],
+    "hit3d_stress_divergence": [_G, _P, _P, _I32, _P],
 }
 
 PASS_NAMES = ("x_inverse_curl", "y_inverse_curl", "z_c2r_cross_r2c", "y_forward",

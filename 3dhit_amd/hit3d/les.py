@@ -1,0 +1,238 @@
+"""Coarse LES with the learned closure, and the filtered-DNS pipeline that
+trains it (no reference counterpart: an addition to hit3d's solver).
+
+Filtered DNS. A DNS state u on the fine grid is sharp-filtered to |k| <=
+kcut and truncated onto the LES grid (`filter_state`); the exact subgrid
+stress of that filter is tau_ij = G(u_i u_j) - G(ubar_i ubar_j)
+(`subgrid_stress`, evaluated spectrally on both grids: the fine-grid
+products are transformed and filtered, the coarse products are free of
+aliasing below the cut because 2 kcut < N_les - kcut). The closure input is
+the filtered velocity-gradient tensor d_j ubar_i in physical space
+(`velocity_gradient`, channel 3 j + i).
+
+LES. d ubar/dt = rhs(ubar) + P M [-i k_j tau_hat_ij(net(grad ubar))], with
+rhs the solver's Navier-Stokes RHS (forcing included) and the closure term
+projected and dealiased like the nonlinear term. Every piece is a
+differentiable torch operation backed by the library: the RHS's backward
+is the hand-written adjoint (hit3d_rhs_vjp), the linear spectral operators
+use their exact adjoints, and the net's backward runs the BF16-MFMA
+backward kernels, so a-posteriori training differentiates whole LES
+rollouts.
+
+Gradient convention of the spectral tensors: torch's, dL = Re sum conj(g) dx.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from .closure import ClosureNet, apply_net
+from .diagnostics import energy_spectrum
+from .grid import check_spectral, make_grid, stream, to_physical, to_spectral
+from .navierstokes import NSParams
+
+__all__ = ["filter_state", "subgrid_stress", "velocity_gradient", "les_rhs", "les_step",
+           "les_rollout", "closure_force", "ClosureNet"]
+
+
+# ---------------------------------------------------------------------------
+# Filtered-DNS pipeline
+# ---------------------------------------------------------------------------
+
+def filter_state(u_fine, fine, coarse, kcut, ncomp=3):
+    """Sharp spherical filter |k| <= kcut of a fine-grid spectral field,
+    expressed on the coarse grid (scaled to the coarse grid's rfftn)."""
+    out = torch.empty((ncomp, coarse.Nx, coarse.Ny, coarse.Nz // 2 + 1),
+                      dtype=coarse.complex_dtype, device=coarse.device)
+    _lib.call("hit3d_spectral_truncate", fine.desc, u_fine.contiguous().data_ptr(), coarse.desc,
+              out.data_ptr(), ncomp, float(kcut), stream())
+    return out
+
+
+def _products(u_phys, grid):
+    uu = torch.empty((6, grid.Nx, grid.Ny, grid.Nz), dtype=grid.dtype, device=grid.device)
+    _lib.call("hit3d_velocity_products", grid.desc, u_phys.data_ptr(), uu.data_ptr(), stream())
+    return uu
+
+
+def subgrid_stress(u_fine, fine, coarse, kcut):
+    """(ubar, tau_hat): the filtered state on the coarse grid and the
+    spectral subgrid stress tau_ij = G(u_i u_j) - G(ubar_i ubar_j) (6
+    components, order 00, 01, 02, 11, 12, 22) on the coarse grid."""
+    ubar = filter_state(u_fine, fine, coarse, kcut)
+    uu_f = to_spectral(_products(to_physical(u_fine, fine), fine), fine)
+    g_uu = filter_state(uu_f, fine, coarse, kcut, ncomp=6)
+    del uu_f
+    vv = to_spectral(_products(to_physical(ubar, coarse), coarse), coarse)
+    g_vv = filter_state(vv, coarse, coarse, kcut, ncomp=6)
+    return ubar, g_uu - g_vv
+
+
+def velocity_gradient(ubar, grid):
+    """Physical d_j ubar_i, channel 3 j + i, fp32/fp64 [9][Nx][Ny][Nz]."""
+    G = _Grad9.apply(ubar, grid)
+    return _ToPhysical.apply(G, grid)
+
+
+# ---------------------------------------------------------------------------
+# Differentiable building blocks (library kernels, hand-written adjoints)
+# ---------------------------------------------------------------------------
+
+def _deriv(x, grid, op):
+    out = torch.empty_like(x)
+    _lib.call("hit3d_spectral_derivative", grid.desc, x.data_ptr(), out.data_ptr(), x.shape[0], op,
+              stream())
+    return out
+
+
+class _Grad9(torch.autograd.Function):
+    """u[3] -> G[3 j + i] = i k_j u_i; adjoint: sum_j conj(i k_j) g = -i k_j g."""
+
+    @staticmethod
+    def forward(ctx, u, grid):
+        ctx.grid = grid
+        u = u.contiguous()
+        return torch.cat([_deriv(u, grid, j) for j in range(3)], 0)
+
+    @staticmethod
+    def backward(ctx, g):
+        g = g.contiguous()
+        out = sum(_deriv(g[3 * j:3 * j + 3].contiguous(), ctx.grid, j) for j in range(3))
+        return -out, None
+
+
+class _ToPhysical(torch.autograd.Function):
+    """irfftn with 1/N; adjoint (w / N) rfftn(g)."""
+
+    @staticmethod
+    def forward(ctx, x, grid):
+        ctx.grid = grid
+        return to_physical(x.contiguous(), grid)
+
+    @staticmethod
+    def backward(ctx, g):
+        grid = ctx.grid
+        n = float(grid.Nx * grid.Ny * grid.Nz)
+        return to_spectral(g.contiguous(), grid) * (grid.weights / n), None
+
+
+class _ToSpectral(torch.autograd.Function):
+    """rfftn (unnormalised); adjoint N irfftn(g / w)."""
+
+    @staticmethod
+    def forward(ctx, x, grid):
+        ctx.grid = grid
+        return to_spectral(x.contiguous(), grid)
+
+    @staticmethod
+    def backward(ctx, g):
+        grid = ctx.grid
+        n = float(grid.Nx * grid.Ny * grid.Nz)
+        return to_physical((g / grid.weights).contiguous(), grid) * n, None
+
+
+class _StressDiv(torch.autograd.Function):
+    """tau_hat[6] -> F_i = -sum_j i k_j tau_ij; adjoint by the library kernel."""
+
+    @staticmethod
+    def forward(ctx, t, grid):
+        ctx.grid = grid
+        out = torch.empty((3,) + tuple(t.shape[1:]), dtype=t.dtype, device=t.device)
+        _lib.call("hit3d_stress_divergence", grid.desc, t.contiguous().data_ptr(), out.data_ptr(), 0,
+                  stream())
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        out = torch.empty((6,) + tuple(g.shape[1:]), dtype=g.dtype, device=g.device)
+        _lib.call("hit3d_stress_divergence", ctx.grid.desc, g.contiguous().data_ptr(), out.data_ptr(),
+                  1, stream())
+        return out, None
+
+
+def _dealias_project_raw(x, grid):
+    out = torch.empty_like(x)
+    _lib.call("hit3d_dealias_project", grid.desc, x.contiguous().data_ptr(), out.data_ptr(), 3, 1, 1,
+              stream())
+    return out
+
+
+class _DealiasProject(torch.autograd.Function):
+    """P M per mode: real and symmetric, hence its own adjoint."""
+
+    @staticmethod
+    def forward(ctx, x, grid):
+        ctx.grid = grid
+        return _dealias_project_raw(x, grid)
+
+    @staticmethod
+    def backward(ctx, g):
+        return _dealias_project_raw(g, ctx.grid), None
+
+
+class _NSRhs(torch.autograd.Function):
+    """The solver's RHS (hit3d_rhs) with the hand-written adjoint
+    (hit3d_rhs_vjp) as its backward."""
+
+    @staticmethod
+    def forward(ctx, u, rhs, params, t):
+        ctx.save_for_backward(u)
+        ctx.rhs, ctx.params, ctx.t = rhs, params, t
+        return rhs(u.detach(), params, t, dealiased=True)   # LES states stay inside the 2/3 box
+
+    @staticmethod
+    def backward(ctx, g):
+        (u,) = ctx.saved_tensors
+        from .adjoint import make_ns_rhs_vjp
+        return make_ns_rhs_vjp(ctx.rhs.grid)(u, g.contiguous(), ctx.params, ctx.t), None, None, None
+
+
+# ---------------------------------------------------------------------------
+# LES right-hand side and time stepping
+# ---------------------------------------------------------------------------
+
+def closure_force(u, grid, net):
+    """P M [-i k_j tau_hat_ij], tau = net(grad u) (differentiable)."""
+    g = velocity_gradient(u, grid)
+    V = grid.Nx * grid.Ny * grid.Nz
+    tau = apply_net(net, g.reshape(9, V)).reshape(6, grid.Nx, grid.Ny, grid.Nz)
+    return _DealiasProject.apply(_StressDiv.apply(_ToSpectral.apply(tau, grid), grid), grid)
+
+
+def les_rhs(u, rhs, params, net, t=0.0):
+    du = _NSRhs.apply(u, rhs, params, t)
+    if net is None:
+        return du
+    return du + closure_force(u, rhs.grid, net)
+
+
+def les_step(u, rhs, params, net, dt, t=0.0):
+    """Classic RK4 (integrators.py:32-38 structure) on the LES RHS."""
+    k1 = les_rhs(u, rhs, params, net, t)
+    k2 = les_rhs(u + (0.5 * dt) * k1, rhs, params, net, t + 0.5 * dt)
+    k3 = les_rhs(u + (0.5 * dt) * k2, rhs, params, net, t + 0.5 * dt)
+    k4 = les_rhs(u + dt * k3, rhs, params, net, t + dt)
+    return u + (dt / 6.0) * (k1 + 2.0 * k2 + 2.0 * k3 + k4)
+
+
+def check_les_state(u, grid):
+    """LES states must lie inside the 2/3 box (the RHS runs pruned)."""
+    from .navierstokes import is_dealiased
+    if not is_dealiased(check_spectral(u, grid, 3), grid):
+        raise ValueError("LES state has content outside the 2/3 box; filter it first")
+
+
+def les_rollout(u, rhs, params, net, dt, nsteps, t0=0.0, observe=None, every=1):
+    """nsteps LES steps without gradients; observe(n, u) every `every` steps."""
+    check_les_state(u, rhs.grid)
+    with torch.no_grad():
+        for n in range(int(nsteps)):
+            u = les_step(u, rhs, params, net, dt, t0 + n * dt)
+            if observe is not None and (n + 1) % every == 0:
+                observe(n + 1, u)
+    return u

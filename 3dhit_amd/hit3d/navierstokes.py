@@ -55,10 +55,14 @@ class NSRhs:
     def __init__(self, grid):
         self.grid = grid
 
-    def __call__(self, u_hat, params, t):
+    def __call__(self, u_hat, params, t, dealiased=None):
+        """`dealiased` (bool) skips the per-call check (one host sync) when
+        the caller knows whether u_hat is inside the 2/3 box."""
         g = self.grid
         u = check_spectral(u_hat, g, 3)
-        pruned = 1 if is_dealiased(u, g) else 0
+        if dealiased is None:
+            dealiased = is_dealiased(u, g)
+        pruned = 1 if dealiased else 0
         du = torch.empty_like(u)
         ws = g.workspace(_lib.OP_RHS, 3, pruned)
         f = params.forcing.desc()

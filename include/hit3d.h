@@ -179,6 +179,70 @@ int hit3d_profile_rk4(const hit3d_grid* grid, void* u_hat, double nu, double dt,
                       const hit3d_forcing* forcing, int32_t dealiased, double* ms, int64_t* counts,
                       void* work, size_t work_bytes, void* stream);
 
+/* ---- Learned subgrid closure of the coarse LES ------------------------------
+ * No reference counterpart: the closure, its training kernels and the
+ * filtered-DNS pipeline are additions to hit3d's solver.
+ *
+ * A layer is a periodic 3x3x3 convolution over a d x h x w voxel grid
+ * (v = (a*h + b)*w + c, c fastest, i.e. the physical layout real[nx][ny][nz]
+ * with (d, h, w) = (nx, ny, nz)), computed as an implicit GEMM on BF16 MFMA
+ * with FP32 accumulation. Activations are channels-last, x[v][cin_pad]
+ * (bf16, channels >= cin zero); weights w[cout][27][cin] (fp32, offset
+ * o = 9*(da+1) + 3*(db+1) + (dc+1)) are packed to bf16 by hit3d_conv_pack;
+ * out[v][co] = bias[co] + sum_{o,ci} w[co][o][ci] x[v + offset(o)][ci]. */
+typedef struct hit3d_conv {
+  int32_t d, h, w;           /* voxels per axis: w % 16 == 0, d*h*w % 1024 == 0 */
+  int32_t cin, cout;         /* channels (1 <= cin <= cin_pad, 1 <= cout <= cout_pad) */
+  int32_t cin_pad, cout_pad; /* stored channels, 16 or 32 */
+} hit3d_conv;
+
+/* Packed weight widths kp (Wp[cout_pad][kp]) and kt (WT[cin_pad][kt]), and the
+   workspace of hit3d_conv_backward. */
+int hit3d_conv_sizes(const hit3d_conv* conv, int64_t* kp, int64_t* kt, size_t* work_bytes);
+
+/* w fp32 [cout][27][cin] -> wp bf16 [cout_pad][kp], wt bf16 [cin_pad][kt]. */
+int hit3d_conv_pack(const hit3d_conv* conv, const float* w, void* wp, void* wt, void* stream);
+
+/* gelu = 1: pre fp32 [V][cout_pad] = conv(x) + bias, act bf16 [V][cout_pad] =
+   GELU(pre) (exact erf form; zero beyond cout). gelu = 0: out fp32 [V][cout]. */
+int hit3d_conv_forward(const hit3d_conv* conv, const void* x, const void* wp, const float* bias,
+                       int32_t gelu, float* pre, void* act, float* out, void* stream);
+
+/* Gradients of one layer from grad_out (gelu = 1: fp32 [V][cout_pad] w.r.t.
+   act; gelu = 0: fp32 [V][cout] w.r.t. out): grad_x fp32 [V][cin_pad] (NULL
+   to skip), grad_w fp32 [cout][27][cin], grad_b fp32 [cout]. Deterministic
+   (fixed-order reductions, no atomics). */
+int hit3d_conv_backward(const hit3d_conv* conv, const void* x, const void* wt, const float* pre,
+                        const float* grad_out, int32_t gelu, float* grad_x, float* grad_w,
+                        float* grad_b, void* work, size_t work_bytes, void* stream);
+
+/* One Adam step (Kingma & Ba) on n fp32 parameters, step count t >= 1. */
+int hit3d_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float beta1,
+               float beta2, float eps, int64_t t, void* stream);
+
+/* aos[v][ch] = bf16(soa[ch][v] * scale[ch]) for ch < c, 0 up to cpad. */
+int hit3d_channels_pack(const float* soa, int32_t c, int64_t nvox, const float* scale, int32_t cpad,
+                        void* aos, void* stream);
+
+/* soa[ch][v] = aos[v*ld + ch] * scale[ch] for ch < c (fp32). */
+int hit3d_channels_unpack(const float* aos, int32_t c, int32_t ld, int64_t nvox, const float* scale,
+                          float* soa, void* stream);
+
+/* Sharp spherical spectral filter onto a coarser grid of the same cubic box:
+   u_coarse = (Ncoarse/Nfine) * u_fine on the modes with |k| <= kcut that the
+   coarse grid holds without its Nyquist planes, 0 elsewhere. */
+int hit3d_spectral_truncate(const hit3d_grid* fine, const void* u_fine, const hit3d_grid* coarse,
+                            void* u_coarse, int64_t ncomp, double kcut, void* stream);
+
+/* uu[6] = (u0u0, u0u1, u0u2, u1u1, u1u2, u2u2) from physical u[3]. */
+int hit3d_velocity_products(const hit3d_grid* grid, const void* u, void* uu, void* stream);
+
+/* adjoint = 0: out[3] = F_i = -sum_j i k_j tau_ij from the spectral stress
+   in[6] (order of hit3d_velocity_products); adjoint = 1: its adjoint, in[3]
+   -> out[6], in the convention dL = Re sum conj(g) dx. No aliasing. */
+int hit3d_stress_divergence(const hit3d_grid* grid, const void* in, void* out, int32_t adjoint,
+                            void* stream);
+
 #ifdef __cplusplus
 }
 #endif

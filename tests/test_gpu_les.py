@@ -1,0 +1,143 @@
+"""The LES pieces on the GPU: exact adjoints of the differentiable spectral
+operators (dot-product identities in float64), the RHS adjoint and a whole
+LES step against central differences, and the filtered-DNS subgrid stress
+against a NumPy evaluation on the fine grid."""
+
+import numpy as np
+import pytest
+import torch
+
+import hit3d_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+h3 = pytest.importorskip("hit3d")
+from hit3d import les
+from hit3d.closure import ClosureNet
+
+
+def rdot(a, b):
+    """Re sum conj(a) b (torch's pairing for complex tensors)."""
+    if a.is_complex():
+        return float(torch.sum((a.conj() * b).real))
+    return float(torch.sum(a * b))
+
+
+def crand(shape, gen, dtype=torch.complex128):
+    return torch.complex(torch.randn(shape, generator=gen, dtype=torch.float64),
+                         torch.randn(shape, generator=gen, dtype=torch.float64)).to(dtype).cuda()
+
+
+OPS = ["to_physical", "to_spectral", "grad9", "stress_div", "dealias_project"]
+
+
+@pytest.mark.parametrize("op", OPS)
+def test_linear_operator_adjoints(op):
+    g = h3.make_grid(16, 8, 16, dtype=torch.float64)
+    gen = torch.Generator().manual_seed(OPS.index(op))
+    sh = (g.Nx, g.Ny, g.Nz // 2 + 1)
+    ph = (g.Nx, g.Ny, g.Nz)
+    if op == "to_physical":
+        f, x = (lambda x: les._ToPhysical.apply(x, g)), crand((4,) + sh, gen)
+    elif op == "to_spectral":
+        f, x = (lambda x: les._ToSpectral.apply(x, g)), torch.randn((4,) + ph, generator=gen,
+                                                                     dtype=torch.float64).cuda()
+    elif op == "grad9":
+        f, x = (lambda x: les._Grad9.apply(x, g)), crand((3,) + sh, gen)
+    elif op == "stress_div":
+        f, x = (lambda x: les._StressDiv.apply(x, g)), crand((6,) + sh, gen)
+    else:
+        f, x = (lambda x: les._DealiasProject.apply(x, g)), crand((3,) + sh, gen)
+    x = x.requires_grad_(True)
+    y = f(x)
+    w = crand(y.shape, gen) if y.is_complex() else torch.randn(y.shape, generator=gen,
+                                                               dtype=torch.float64).cuda()
+    (gx,) = torch.autograd.grad(y, x, w)
+    lhs, rhs = rdot(w, y.detach()), rdot(gx, x.detach())
+    scale = float(torch.sum(w.abs() * y.detach().abs()))
+    assert abs(lhs - rhs) <= 1e-13 * scale
+
+
+def test_ns_rhs_backward_matches_central_differences():
+    g = h3.make_grid(16, dtype=torch.float64)
+    rhs = h3.make_ns_rhs(g)
+    params = h3.NSParams(nu=0.02, forcing=h3.LinearForcing(A=0.1))
+    u = h3.random_field(1, g, h3.config_spectrum).requires_grad_(True)
+    gen = torch.Generator().manual_seed(2)
+    W = crand(u.shape, gen)
+    y = les._NSRhs.apply(u, rhs, params, 0.0)
+    (gu,) = torch.autograd.grad(y, u, W)
+    d = h3.project(h3.dealias(crand(u.shape, gen), g), g)
+    eps = 1e-4 * float(u.detach().abs().max() / d.abs().max())
+
+    def J(v):
+        return rdot(W, rhs(v, params, 0.0))
+    fd = (J(u.detach() + eps * d) - J(u.detach() - eps * d)) / (2 * eps)
+    assert abs(rdot(gu, d) - fd) < 1e-7 * abs(fd)
+
+
+def test_les_step_gradient_without_closure_matches_central_differences():
+    g = h3.make_grid(16, dtype=torch.float64)
+    rhs = h3.make_ns_rhs(g)
+    params = h3.NSParams(nu=0.02, forcing=h3.LinearForcing(A=0.1))
+    u0 = h3.random_field(3, g, h3.config_spectrum)
+    gen = torch.Generator().manual_seed(4)
+    W = crand(u0.shape, gen)
+
+    def J(v):
+        u = v
+        for n in range(3):
+            u = les.les_step(u, rhs, params, None, 5e-3, n * 5e-3)
+        return torch.sum((W.conj() * u).real)
+    x = u0.clone().requires_grad_(True)
+    (gx,) = torch.autograd.grad(J(x), x)
+    d = h3.project(h3.dealias(crand(u0.shape, gen), g), g)
+    eps = 1e-4 * float(u0.abs().max() / d.abs().max())
+    fd = (float(J(u0 + eps * d)) - float(J(u0 - eps * d))) / (2 * eps)
+    assert abs(rdot(gx, d) - fd) < 1e-7 * abs(fd)
+
+
+def test_les_step_with_closure_gradient_matches_central_differences():
+    """float32 LES with the bf16 net: the autograd gradient of a functional of
+    two steps against central differences (float32 + bf16 noise: 2%)."""
+    g = h3.make_grid(16, dtype=torch.float32)
+    rhs = h3.make_ns_rhs(g)
+    params = h3.NSParams(nu=0.02, forcing=h3.LinearForcing(A=0.1))
+    net = ClosureNet(g, channels=16, seed=1)
+    u0 = h3.random_field(5, g, h3.config_spectrum)
+    gg = les.velocity_gradient(u0, g).reshape(9, -1)
+    net.set_scales(1.0 / gg.std(dim=1), torch.full((6,), 0.05))
+    gen = torch.Generator().manual_seed(6)
+    W = crand(u0.shape, gen, torch.complex64)
+
+    def J(v):
+        u = v
+        for n in range(2):
+            u = les.les_step(u, rhs, params, net, 5e-3, n * 5e-3)
+        return torch.sum((W.conj() * u).real.double())
+    x = u0.clone().requires_grad_(True)
+    net.zero_grad()
+    (gx,) = torch.autograd.grad(J(x), x)
+    d = h3.project(h3.dealias(crand(u0.shape, gen, torch.complex64), g), g)
+    eps = 3e-3 * float(u0.abs().max() / d.abs().max())
+    with torch.no_grad():
+        fd = (float(J(u0 + eps * d)) - float(J(u0 - eps * d))) / (2 * eps)
+    assert abs(rdot(gx.to(torch.complex128), d.to(torch.complex128)) - fd) < 2e-2 * abs(fd)
+    assert float(net.layers[2].gw.abs().max()) > 0        # the closure received gradient
+
+
+def test_subgrid_stress_matches_numpy():
+    fine, coarse = h3.make_grid(32, dtype=torch.float64), h3.make_grid(16, dtype=torch.float64)
+    ofine = O.make_grid(32, dtype=np.float64)
+    u = O.random_field(7, ofine, O.config_spectrum)
+    ubar, tau_hat = les.subgrid_stress(torch.from_numpy(u).cuda(), fine, coarse, 5.0)
+    # NumPy on the fine grid: G = sharp filter |k| <= 5, then sample every 2nd point
+    mask = ofine.k2 <= 25.0
+    G = lambda f: O.to_physical(O.to_spectral(f[None], ofine) * mask, ofine)[0]
+    up = O.to_physical(u, ofine)
+    ub = O.to_physical(u * mask, ofine)
+    pairs = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    tau = np.stack([G(up[i] * up[j]) - G(ub[i] * ub[j]) for i, j in pairs])[:, ::2, ::2, ::2]
+    got = h3.to_physical(tau_hat, coarse).cpu().numpy()
+    assert np.abs(got - tau).max() < 1e-12 * np.abs(tau).max()
+    assert np.abs(h3.to_physical(ubar, coarse).cpu().numpy() - ub[:, ::2, ::2, ::2]).max() < 1e-12
