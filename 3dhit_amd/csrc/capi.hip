@@ -1205,6 +1205,18 @@ int hit3d_channels_unpack(const float* aos, int32_t c, int32_t ld, int64_t nvox,
   return HIT3D_OK;
 }
 
+int hit3d_stress_unpack_clip(const float* aos, int64_t nvox, const float* scale, const float* grad9,
+                             float* tau, uint8_t* keep, void* stream) {
+  H3_TRY(check_ptr(aos, "aos"));
+  H3_TRY(check_ptr(scale, "scale"));
+  H3_TRY(check_ptr(grad9, "grad9"));
+  H3_TRY(check_ptr(tau, "tau"));
+  if (!keep) return fail(HIT3D_ERR_INVALID, "keep is NULL");
+  if (nvox < 1) return fail(HIT3D_ERR_INVALID, "nvox must be >= 1");
+  H3_HIP(h3::launch_unpack_clip(aos, nvox, scale, grad9, tau, keep, static_cast<hipStream_t>(stream)));
+  return HIT3D_OK;
+}
+
 int hit3d_spectral_truncate(const hit3d_grid* fine, const void* u_fine, const hit3d_grid* coarse,
                             void* u_coarse, int64_t ncomp, double kcut, void* stream) {
   H3_TRY(check_grid(fine));

@@ -77,14 +77,17 @@ __global__ __launch_bounds__(64) void k_conv_fwd(const __bf16* __restrict__ X,
   for (int r = 0; r < RB; ++r)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[r][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const int ksteps = KP / 32;
-  for (int s = 0; s < ksteps; ++s) {
+  // K-steps are a compile-time count (KP == conv_kp(CIN)); the loop is
+  // unrolled and double-buffered: step s + 1's operands are loaded before
+  // step s's MFMAs issue
+  constexpr int KS = (27 * CIN + 31) / 32;
+  (void)KP;
+  auto load = [&](int s, bf16x8* A, bf16x8* B) {
     const int k = 32 * s + 8 * lg;          // this lane's first k
     const int o = k / CIN, ci = k % CIN;    // offset and first channel
-    bf16x8 A[RB], B[4];
 #pragma unroll
     for (int r = 0; r < RB; ++r)
-      A[r] = *reinterpret_cast<const bf16x8*>(Wp + int64_t(16 * r + li) * KP + k);
+      A[r] = *reinterpret_cast<const bf16x8*>(Wp + int64_t(16 * r + li) * (32 * KS) + k);
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       if (o < 27) {
@@ -94,6 +97,14 @@ __global__ __launch_bounds__(64) void k_conv_fwd(const __bf16* __restrict__ X,
         B[b] = bf16x8{};  // K padding (the packed weights are zero there too)
       }
     }
+  };
+  bf16x8 A0[RB], B0[4], A1[RB], B1[4];
+  load(0, A0, B0);
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    bf16x8* A = (s & 1) ? A1 : A0;
+    bf16x8* B = (s & 1) ? B1 : B0;
+    if (s + 1 < KS) load(s + 1, (s & 1) ? A0 : A1, (s & 1) ? B0 : B1);
 #pragma unroll
     for (int r = 0; r < RB; ++r)
 #pragma unroll
@@ -148,14 +159,16 @@ __global__ __launch_bounds__(64) void k_conv_bwd_data(const __bf16* __restrict__
   for (int r = 0; r < RB; ++r)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[r][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const int ksteps = KT / 32;
-  for (int s = 0; s < ksteps; ++s) {
+  constexpr int KS = (27 * COUT + 31) / 32;  // KT == conv_kt(COUT)
+  (void)KT;
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
     const int k = 32 * s + 8 * lg;
     const int o = k / COUT, co = k % COUT;
     bf16x8 A[RB], B[4];
 #pragma unroll
     for (int r = 0; r < RB; ++r)
-      A[r] = *reinterpret_cast<const bf16x8*>(WT + int64_t(16 * r + li) * KT + k);
+      A[r] = *reinterpret_cast<const bf16x8*>(WT + int64_t(16 * r + li) * (32 * KS) + k);
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       if (o < 27) {
@@ -350,6 +363,31 @@ __global__ void k_soa_aos(const float* __restrict__ soa, int C, int64_t V, const
   }
 }
 
+// tau[c][v] = aos[v][c] * s[c] (6 stress channels), set to 0 where the
+// modelled transfer Pi = -tau_ij S_ij is negative, S_ij = (d_j u_i + d_i u_j)/2
+// from the gradient channels g[3 j + i][v]; keep[v] = (Pi >= 0).
+__global__ void k_unpack_clip(const float* __restrict__ aos, int64_t V, const float* __restrict__ s,
+                              const float* __restrict__ g, float* __restrict__ tau,
+                              uint8_t* __restrict__ keep) {
+  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V;
+       v += int64_t(gridDim.x) * blockDim.x) {
+    float t[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) t[c] = aos[v * 6 + c] * s[c];
+    float d[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) d[c] = g[c * V + v];
+    const float s00 = d[0], s11 = d[4], s22 = d[8];
+    const float s01 = 0.5f * (d[3] + d[1]), s02 = 0.5f * (d[6] + d[2]), s12 = 0.5f * (d[7] + d[5]);
+    const float pi = -(t[0] * s00 + 2.f * t[1] * s01 + 2.f * t[2] * s02 + t[3] * s11 + 2.f * t[4] * s12 +
+                       t[5] * s22);
+    const bool k = pi >= 0.f;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) tau[c * V + v] = k ? t[c] : 0.f;
+    keep[v] = k ? 1 : 0;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Spectral pieces of the filtered-DNS pipeline and the closure force.
 // ---------------------------------------------------------------------------
@@ -531,6 +569,12 @@ hipError_t launch_soa_aos(const float* soa, int C, int64_t V, const float* scale
                           __bf16* aos_bf, const float* aos_f, float* soa_out, hipStream_t s) {
   hipLaunchKernelGGL(k_soa_aos, dim3(grid_for(V * CP)), dim3(256), 0, s, soa, C, V, scale, CP, dir, aos_bf,
                      aos_f, soa_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_unpack_clip(const float* aos, int64_t V, const float* scale, const float* g, float* tau,
+                              uint8_t* keep, hipStream_t s) {
+  hipLaunchKernelGGL(k_unpack_clip, dim3(grid_for(V)), dim3(256), 0, s, aos, V, scale, g, tau, keep);
   return hipGetLastError();
 }
 

@@ -36,6 +36,8 @@ hipError_t launch_adam(float* p, const float* gr, float* m, float* v, int64_t n,
                        float b2, float eps, float c1, float c2, hipStream_t s);
 hipError_t launch_soa_aos(const float* soa, int C, int64_t V, const float* scale, int CP, int dir,
                           __bf16* aos_bf, const float* aos_f, float* soa_out, hipStream_t s);
+hipError_t launch_unpack_clip(const float* aos, int64_t V, const float* scale, const float* g, float* tau,
+                              uint8_t* keep, hipStream_t s);
 template <class T>
 hipError_t launch_truncate(const cpx<T>* fine, int fx, int fy, int fz, cpx<T>* coarse, int cx, int cy,
                            int cz, int ncomp, double kc2, T scale, hipStream_t s);

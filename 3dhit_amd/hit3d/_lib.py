@@ -77,6 +77,7 @@ SIGNATURES = {
                    ctypes.c_float, _I64, _P],
     "hit3d_channels_pack": [_P, _I32, _I64, _P, _I32, _P, _P],
     "hit3d_channels_unpack": [_P, _I32, _I32, _I64, _P, _P, _P],
+    "hit3d_stress_unpack_clip": [_P, _I64, _P, _P, _P, _P, _P],
     "hit3d_spectral_truncate": [_G, _P, _G, _P, _I64, _D, _P],
     "hit3d_velocity_products": [_G, _P, _P, _P],
     "hit3d_stress_divergence": [_G, _P, _P, _I32, _P],

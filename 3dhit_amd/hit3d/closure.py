@@ -113,7 +113,9 @@ class ConvLayer:
 
 class ClosureNet:
     """tau = out_scale * net(in_scale * g): g fp32 [9][V] -> tau fp32 [6][V]
-    on the LES grid (V = Nx*Ny*Nz voxels, physical layout)."""
+    on the LES grid (V = Nx*Ny*Nz voxels, physical layout), with tau set to
+    0 where it would transfer energy to the resolved scales (-tau_ij S_ij <
+    0) while `clip_backscatter` is on (hit3d_stress_unpack_clip)."""
 
     def __init__(self, grid, channels=32, seed=0, in_scale=None, out_scale=None):
         dev = grid.device
@@ -127,6 +129,15 @@ class ClosureNet:
         self.in_scale = torch.ones(9, device=dev) if in_scale is None else in_scale.float().to(dev)
         self.out_scale = torch.ones(6, device=dev) if out_scale is None else out_scale.float().to(dev)
         self.t = 0
+        # stands for the parameters in torch's graph: the net's output depends
+        # on it, so losses reach the net's backward even when the input state
+        # does not require grad (the parameters themselves live in the layers)
+        self.token = torch.zeros(1, device=dev, requires_grad=True)
+        # scalar gain on the closure force, trained a posteriori; backscatter
+        # clipping (tau set to 0 where -tau_ij S_ij < 0) keeps the modelled
+        # subgrid transfer dissipative
+        self.gain = torch.ones(1, device=dev, requires_grad=True)
+        self.clip_backscatter = True
 
     def set_scales(self, in_scale, out_scale):
         self.in_scale = in_scale.float().to(self.in_scale.device).contiguous()
@@ -142,16 +153,24 @@ class ClosureNet:
         p2, a2 = self.layers[1].forward(a1)
         out = self.layers[2].forward(a2)
         tau = torch.empty(6, self.V, device=g.device)
-        _lib.call("hit3d_channels_unpack", out.data_ptr(), 6, 6, self.V, self.out_scale.data_ptr(),
-                  tau.data_ptr(), stream())
-        return tau, (x0, p1, a1, p2, a2)
+        keep = None
+        if self.clip_backscatter:
+            keep = torch.empty(self.V, dtype=torch.uint8, device=g.device)
+            _lib.call("hit3d_stress_unpack_clip", out.data_ptr(), self.V, self.out_scale.data_ptr(),
+                      g.data_ptr(), tau.data_ptr(), keep.data_ptr(), stream())
+        else:
+            _lib.call("hit3d_channels_unpack", out.data_ptr(), 6, 6, self.V, self.out_scale.data_ptr(),
+                      tau.data_ptr(), stream())
+        return tau, (x0, p1, a1, p2, a2, keep)
 
     def __call__(self, g):
         return self.forward_saved(g)[0]
 
     def backward_saved(self, saved, grad_tau, need_grad_g=True):
         """Accumulates the parameter gradients; returns d/dg (fp32 [9][V])."""
-        x0, p1, a1, p2, a2 = saved
+        x0, p1, a1, p2, a2, keep = saved
+        if keep is not None:   # clipped entries are constant (zero): no gradient flows back
+            grad_tau = grad_tau * keep.to(grad_tau.dtype)[None, :]
         gout = (grad_tau * self.out_scale[:, None]).t().contiguous()      # [V][6]
         ga2 = self.layers[2].backward(a2, None, gout)
         ga1 = self.layers[1].backward(a1, p2, ga2)
@@ -177,12 +196,16 @@ class ClosureNet:
 
     def state(self):
         return {"layers": [L.state() for L in self.layers], "in_scale": self.in_scale.clone(),
-                "out_scale": self.out_scale.clone()}
+                "out_scale": self.out_scale.clone(), "gain": self.gain.detach().clone(),
+                "clip_backscatter": self.clip_backscatter}
 
     def load(self, st):
         for L, s in zip(self.layers, st["layers"]):
             L.load(s)
         self.set_scales(st["in_scale"], st["out_scale"])
+        with torch.no_grad():
+            self.gain.copy_(st["gain"])
+        self.clip_backscatter = st["clip_backscatter"]
 
 
 class _NetFn(torch.autograd.Function):
@@ -190,7 +213,7 @@ class _NetFn(torch.autograd.Function):
     accumulate in the net's layers (ClosureNet.zero_grad / adam_step)."""
 
     @staticmethod
-    def forward(ctx, g, net):
+    def forward(ctx, g, token, net):
         tau, saved = net.forward_saved(g)
         ctx.net, ctx.saved = net, saved
         return tau
@@ -199,9 +222,11 @@ class _NetFn(torch.autograd.Function):
     def backward(ctx, grad_tau):
         gg = ctx.net.backward_saved(ctx.saved, grad_tau.contiguous(), need_grad_g=ctx.needs_input_grad[0])
         ctx.saved = None
-        return gg, None
+        return gg, torch.zeros_like(ctx.net.token), None
 
 
 def apply_net(net, g):
-    """Differentiable tau = net(g) (autograd through the library kernels)."""
-    return _NetFn.apply(g, net)
+    """Differentiable tau = net(g) (autograd through the library kernels).
+    Outside torch.no_grad() the result always carries a graph, since the
+    parameters stand behind `net.token`."""
+    return _NetFn.apply(g, net.token, net)

@@ -37,7 +37,7 @@ from .grid import check_spectral, make_grid, stream, to_physical, to_spectral
 from .navierstokes import NSParams
 
 __all__ = ["filter_state", "subgrid_stress", "velocity_gradient", "les_rhs", "les_step",
-           "les_rollout", "closure_force", "ClosureNet"]
+           "les_rollout", "closure_force", "sphere_mask", "LesStepGraph", "ClosureNet"]
 
 
 # ---------------------------------------------------------------------------
@@ -97,7 +97,11 @@ class _Grad9(torch.autograd.Function):
     def forward(ctx, u, grid):
         ctx.grid = grid
         u = u.contiguous()
-        return torch.cat([_deriv(u, grid, j) for j in range(3)], 0)
+        out = torch.empty((9,) + tuple(u.shape[1:]), dtype=u.dtype, device=u.device)
+        for j in range(3):
+            _lib.call("hit3d_spectral_derivative", grid.desc, u.data_ptr(), out[3 * j].data_ptr(), 3, j,
+                      stream())
+        return out
 
     @staticmethod
     def backward(ctx, g):
@@ -196,28 +200,81 @@ class _NSRhs(torch.autograd.Function):
 # LES right-hand side and time stepping
 # ---------------------------------------------------------------------------
 
+_PAIRS = ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))
+
+
+def clip_backscatter(tau, g):
+    """tau (6 x V) set to 0 where the modelled transfer -tau_ij S_ij is
+    negative; S_ij = (d_j u_i + d_i u_j)/2 from g (channel 3 j + i). Torch
+    statement of what hit3d_stress_unpack_clip does inside the net."""
+    pi = 0.0
+    for s, (i, j) in enumerate(_PAIRS):
+        sij = 0.5 * (g[3 * j + i] + g[3 * i + j])
+        pi = pi - (1.0 if i == j else 2.0) * tau[s] * sij
+    return tau * (pi >= 0).to(tau.dtype)
+
+
 def closure_force(u, grid, net):
-    """P M [-i k_j tau_hat_ij], tau = net(grad u) (differentiable)."""
+    """gain * P M [-i k_j tau_hat_ij], tau = net(grad u), with backscatter
+    clipping when the net asks for it (differentiable)."""
+    if grid.dtype != torch.float32:
+        raise ValueError("the closure runs on float32 LES grids")
     g = velocity_gradient(u, grid)
     V = grid.Nx * grid.Ny * grid.Nz
-    tau = apply_net(net, g.reshape(9, V)).reshape(6, grid.Nx, grid.Ny, grid.Nz)
-    return _DealiasProject.apply(_StressDiv.apply(_ToSpectral.apply(tau, grid), grid), grid)
+    tau = apply_net(net, g.reshape(9, V)).reshape(6, grid.Nx, grid.Ny, grid.Nz)  # clipping inside
+    f = _DealiasProject.apply(_StressDiv.apply(_ToSpectral.apply(tau, grid), grid), grid)
+    return net.gain * f
 
 
-def les_rhs(u, rhs, params, net, t=0.0):
+def sphere_mask(grid, kcut):
+    """0/1 mask of the modes with |k| <= kcut: the LES resolves exactly the
+    sharp spherical filter's modes (the solver's 2/3 box is a cube, whose
+    corners reach sqrt(3) kcut)."""
+    return (grid.k2 <= float(kcut) ** 2).to(grid.dtype)
+
+
+def les_rhs(u, rhs, params, net, t=0.0, mask=None):
     du = _NSRhs.apply(u, rhs, params, t)
-    if net is None:
-        return du
-    return du + closure_force(u, rhs.grid, net)
+    if net is not None:
+        du = du + closure_force(u, rhs.grid, net)
+    return du if mask is None else du * mask
 
 
-def les_step(u, rhs, params, net, dt, t=0.0):
+def les_step(u, rhs, params, net, dt, t=0.0, mask=None):
     """Classic RK4 (integrators.py:32-38 structure) on the LES RHS."""
-    k1 = les_rhs(u, rhs, params, net, t)
-    k2 = les_rhs(u + (0.5 * dt) * k1, rhs, params, net, t + 0.5 * dt)
-    k3 = les_rhs(u + (0.5 * dt) * k2, rhs, params, net, t + 0.5 * dt)
-    k4 = les_rhs(u + dt * k3, rhs, params, net, t + dt)
-    return u + (dt / 6.0) * (k1 + 2.0 * k2 + 2.0 * k3 + k4)
+    k1 = les_rhs(u, rhs, params, net, t, mask)
+    k2 = les_rhs(torch.add(u, k1, alpha=0.5 * dt), rhs, params, net, t + 0.5 * dt, mask)
+    acc = torch.add(u, k1, alpha=dt / 6.0).add(k2, alpha=dt / 3.0)
+    k3 = les_rhs(torch.add(u, k2, alpha=0.5 * dt), rhs, params, net, t + 0.5 * dt, mask)
+    acc = acc.add(k3, alpha=dt / 3.0)
+    k4 = les_rhs(torch.add(u, k3, alpha=dt), rhs, params, net, t + dt, mask)
+    return acc.add(k4, alpha=dt / 6.0)
+
+
+class LesStepGraph:
+    """One LES step without gradients, captured as a HIP graph and replayed
+    (the 64^3 step is a hundred small launches; replaying the graph removes
+    their host cost). The captured kernels issue no memsets and no host
+    synchronisation; workspaces are allocated by the warm-up steps."""
+
+    def __init__(self, u, rhs, params, net, dt, mask=None):
+        check_les_state(u, rhs.grid)
+        self.u = u.clone()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.no_grad(), torch.cuda.stream(side):
+            w = self.u.clone()
+            for _ in range(2):
+                w = les_step(w, rhs, params, net, dt, 0.0, mask)
+        torch.cuda.current_stream().wait_stream(side)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.no_grad(), torch.cuda.graph(self.graph):
+            self.out = les_step(self.u, rhs, params, net, dt, 0.0, mask)
+
+    def step(self):
+        self.graph.replay()
+        self.u.copy_(self.out)
+        return self.u
 
 
 def check_les_state(u, grid):
@@ -227,12 +284,21 @@ def check_les_state(u, grid):
         raise ValueError("LES state has content outside the 2/3 box; filter it first")
 
 
-def les_rollout(u, rhs, params, net, dt, nsteps, t0=0.0, observe=None, every=1):
-    """nsteps LES steps without gradients; observe(n, u) every `every` steps."""
+def les_rollout(u, rhs, params, net, dt, nsteps, t0=0.0, observe=None, every=1, mask=None,
+                graph=True):
+    """nsteps LES steps without gradients (replayed from a HIP graph unless
+    graph=False); observe(n, u) every `every` steps."""
     check_les_state(u, rhs.grid)
+    if graph and nsteps > 0:
+        g = LesStepGraph(u, rhs, params, net, dt, mask)
+        for n in range(int(nsteps)):
+            u = g.step()
+            if observe is not None and (n + 1) % every == 0:
+                observe(n + 1, u)
+        return u.clone()
     with torch.no_grad():
         for n in range(int(nsteps)):
-            u = les_step(u, rhs, params, net, dt, t0 + n * dt)
+            u = les_step(u, rhs, params, net, dt, t0 + n * dt, mask)
             if observe is not None and (n + 1) % every == 0:
                 observe(n + 1, u)
     return u

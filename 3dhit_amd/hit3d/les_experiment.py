@@ -1,0 +1,297 @@
+"""The learned-closure LES experiment, end to end on one GPU:
+
+1. Lundgren-forced DNS on the fine grid (energy held at 0.5), spun up, then
+   `nsnap` snapshots `stride` DNS steps apart, each filtered to |k| <= kcut
+   on the LES grid: the filtered state ubar and the exact subgrid stress
+   tau_hat (kept in device memory).
+2. A-priori training of the closure net (MSE on the deviatoric tau,
+   channels normalised)
+   on the first `train_frac` of the snapshots; the loss on the rest is
+   reported.
+3. A-posteriori training: `window`-step LES rollouts from filtered DNS
+   states, loss = mean over the steps of the squared log error of the shell
+   spectrum against the filtered DNS, plus a small relative-L2 trajectory
+   term; gradients through the LES steps (the solver's hand-written RHS
+   adjoint plus the net's backward kernels) reach the net and the scalar
+   gain of the closure force. The LES evolves exactly the filter's modes
+   (|k| <= kcut, a sphere inside the solver's 2/3 cube).
+4. Evaluation: an LES of `eval_steps` steps (LES dt = stride * DNS dt) from
+   the first snapshot, with and without the closure; the time-averaged
+   shell spectrum E(k), k = 1..kcut, against the filtered DNS spectrum over
+   the same times. The closure is accepted when max_k |E_les/E_dns - 1| <=
+   tol; otherwise the result falls back to the plain DNS. GPU time of the
+   DNS and of the LES over the same simulated time is measured with HIP
+   events.
+
+The LES forcing is LinearForcing with the DNS's mean Lundgren coefficient
+A: the filter commutes with f = A u, so this is the forcing of the filtered
+equations. Run: python -m hit3d.les_experiment [--quick] [--out FILE]
+(with 3dhit_amd on sys.path)."""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+import torch
+
+from .closure import ClosureNet
+from .diagnostics import energy_spectrum, quadratic_sums
+from .fields import config_spectrum, random_field
+from .forcing import LinearForcing, LundgrenForcing
+from .grid import make_grid, to_physical
+from .integrators import rollout
+from .les import (check_les_state, les_rollout, les_step, sphere_mask, subgrid_stress,
+                  velocity_gradient)
+from .navierstokes import NSParams, make_ns_rhs
+
+
+def _log(msg):
+    print(f"[les {time.strftime('%H:%M:%S')}] {msg}", flush=True)
+
+
+def _timed(fn):
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    out = fn()
+    e1.record()
+    e1.synchronize()
+    return out, e0.elapsed_time(e1) / 1e3
+
+
+def _spectrum(u, grid, kmax):
+    """Energy of shells 1..kmax."""
+    return energy_spectrum(u, grid)[1][:kmax].astype(np.float64)
+
+
+def _e_omega(u, grid):
+    s = quadratic_sums(u, grid)
+    return float(s[0]), float(s[2])
+
+
+def generate(fine_n, les_n, kcut, nu, dt_dns, stride, spinup_time, nsnap, seed):
+    fine = make_grid(fine_n, dtype=torch.float32)
+    coarse = make_grid(les_n, dtype=torch.float32)
+    rhs = make_ns_rhs(fine)
+    params = NSParams(nu=nu, forcing=LundgrenForcing(nu=nu))
+    u = random_field(seed, fine, config_spectrum)
+    nspin = int(round(spinup_time / dt_dns))
+    _log(f"DNS {fine_n}^3 spin-up: {nspin} steps (dt {dt_dns})")
+    dns_s, done = 0.0, 0
+    chunk = max(1, min(nspin, 500))
+    while done < nspin:
+        n = min(chunk, nspin - done)
+        u, s = _timed(lambda: rollout(rhs, u, params, dt_dns, n, done * dt_dns))
+        dns_s += s
+        done += n
+        E, Om = _e_omega(u, fine)
+        _log(f"  spin-up step {done}/{nspin}: E = {E:.4f}, Omega = {Om:.2f}, {1e3 * dns_s / done:.2f} ms/step")
+    shape = (nsnap, 3, les_n, les_n, les_n // 2 + 1)
+    ubar = torch.empty(shape, dtype=coarse.complex_dtype, device=coarse.device)
+    tau = torch.empty((nsnap, 6) + shape[2:], dtype=coarse.complex_dtype, device=coarse.device)
+    coef = np.empty(nsnap)
+    t0 = nspin * dt_dns
+    dns_steps_s, stress_s = 0.0, 0.0
+    for s in range(nsnap):
+        u, sec = _timed(lambda: rollout(rhs, u, params, dt_dns, stride, t0 + s * stride * dt_dns))
+        dns_steps_s += sec
+        (ub, th), sec = _timed(lambda: subgrid_stress(u, fine, coarse, kcut))
+        stress_s += sec
+        ubar[s], tau[s] = ub, th
+        E, Om = _e_omega(u, fine)
+        coef[s] = nu * Om / E                     # Lundgren A = 2 nu Omega / (2 E)
+        if (s + 1) % max(1, nsnap // 20) == 0:
+            _log(f"  snapshot {s + 1}/{nsnap}: E = {E:.4f}, A = {coef[s]:.4f}, "
+                 f"DNS {1e3 * dns_steps_s / ((s + 1) * stride):.2f} ms/step, stress {1e3 * stress_s / (s + 1):.1f} ms")
+    del u
+    return {"fine": fine, "coarse": coarse, "ubar": ubar, "tau": tau, "coef": coef,
+            "dns_ms_per_step": 1e3 * dns_steps_s / (nsnap * stride),
+            "stress_ms": 1e3 * stress_s / nsnap, "t_start": t0}
+
+
+def _deviatoric(tau):
+    """tau_ij - delta_ij tr(tau)/3 (components 00, 01, 02, 11, 12, 22): the
+    trace enters the LES only as a gradient, which the projection removes."""
+    tr = (tau[0] + tau[3] + tau[5]) / 3.0
+    out = tau.clone()
+    out[0] -= tr
+    out[3] -= tr
+    out[5] -= tr
+    return out
+
+
+def train_apriori(net, data, train, test, epochs, lr, seed):
+    coarse = data["coarse"]
+    V = coarse.Nx * coarse.Ny * coarse.Nz
+    rng = np.random.default_rng(seed)
+    sig2 = (net.out_scale ** 2)[:, None]
+
+    def loss_of(s, train_step):
+        g = velocity_gradient(data["ubar"][s], coarse).reshape(9, V)
+        tau = _deviatoric(to_physical(data["tau"][s], coarse).reshape(6, V))
+        pred, saved = net.forward_saved(g)
+        err = pred - tau
+        loss = float(torch.mean(err * err / sig2))
+        if train_step:
+            net.zero_grad()
+            net.backward_saved(saved, (2.0 / (6 * V)) * err / sig2, need_grad_g=False)
+            net.adam_step(lr)
+        return loss
+
+    hist = []
+    for ep in range(epochs):
+        tr = [loss_of(int(s), True) for s in rng.permutation(train)]
+        te = [loss_of(int(s), False) for s in test]
+        hist.append({"epoch": ep + 1, "train_mse": float(np.mean(tr)), "test_mse": float(np.mean(te))})
+        _log(f"  a-priori epoch {ep + 1}/{epochs}: normalised MSE train {hist[-1]['train_mse']:.4f}, "
+             f"held-out {hist[-1]['test_mse']:.4f}")
+    return hist
+
+
+def _shell_energy(u, grid, kmax):
+    """Differentiable energy of shells 1..kmax (torch ops on the state)."""
+    shell = grid.shell.reshape(-1).long()
+    e = (grid.weights * (u.real ** 2 + u.imag ** 2)).sum(0).reshape(-1)
+    out = torch.zeros(int(shell.max()) + 1, dtype=e.dtype, device=e.device).index_add_(0, shell, e)
+    return out[1:kmax + 1] / (2.0 * float(grid.Nx * grid.Ny * grid.Nz) ** 2)
+
+
+def train_aposteriori(net, data, params, dt, train, windows, window, lr, seed, kmax, mask,
+                      traj_weight=0.1, lr_gain=1e-2):
+    """Loss per window: mean over its steps of mean_k log(E_les(k)/E_dns(k))^2
+    plus traj_weight times the relative L2^2 distance to the filtered DNS
+    state; the gradient reaches the net's parameters (library backward
+    kernels, Adam on the device) and the closure gain."""
+    coarse = data["coarse"]
+    rhs = make_ns_rhs(coarse)
+    w = coarse.weights
+    rng = np.random.default_rng(seed + 1)
+    starts = [int(s) for s in rng.choice(len(train) - window, size=windows)]
+    opt_gain = torch.optim.Adam([net.gain], lr=lr_gain)
+    hist = []
+    for i, k0 in enumerate(starts):
+        u = data["ubar"][k0].clone()
+        loss = 0.0
+        for m in range(1, window + 1):
+            u = les_step(u, rhs, params, net, dt, (k0 + m - 1) * dt, mask)
+            ref = data["ubar"][k0 + m]
+            num = torch.sum(w * (u - ref).abs() ** 2)
+            den = torch.sum(w * ref.abs() ** 2)
+            lspec = torch.mean(torch.log(_shell_energy(u, coarse, kmax) / _shell_energy(ref, coarse, kmax)) ** 2)
+            loss = loss + (lspec + traj_weight * num / den) / window
+        net.zero_grad()
+        opt_gain.zero_grad()
+        loss.backward()
+        net.adam_step(lr)
+        opt_gain.step()
+        hist.append(float(loss))
+        if (i + 1) % max(1, windows // 10) == 0:
+            _log(f"  a-posteriori window {i + 1}/{windows}: loss over {window} steps "
+                 f"{np.mean(hist[-10:]):.4e}, gain {float(net.gain):.3f}")
+    return hist
+
+
+def evaluate(net, data, params, dt, steps, kmax, mask, every=10):
+    coarse = data["coarse"]
+    rhs = make_ns_rhs(coarse)
+    u0 = data["ubar"][0].clone()
+    check_les_state(u0, coarse)
+    spectra = []
+
+    def obs(n, u):
+        spectra.append(_spectrum(u, coarse, kmax))
+    torch.cuda.synchronize()
+    _, sec = _timed(lambda: les_rollout(u0, rhs, params, net, dt, steps, observe=obs, every=every,
+                                        mask=mask))
+    ref = np.mean([_spectrum(data["ubar"][s], coarse, kmax)
+                   for s in range(every, steps + 1, every) if s < data["ubar"].shape[0]], axis=0)
+    E = np.mean(spectra, axis=0)
+    err = np.abs(E / ref - 1.0)
+    # step cost without the observations (each spectrum synchronises the host)
+    nt = min(100, steps)
+    _, sec_pure = _timed(lambda: les_rollout(u0, rhs, params, net, dt, nt, mask=mask))
+    return {"E_les": E.tolist(), "E_dns_filtered": ref.tolist(), "rel_err": err.tolist(),
+            "max_rel_err": float(err.max()), "seconds_observed": sec,
+            "ms_per_step": 1e3 * sec_pure / nt, "seconds": sec_pure / nt * steps}
+
+
+def run(fine_n=512, les_n=64, kcut=21.0, nu=1 / 2000, dt_dns=1e-3, stride=10, spinup_time=10.0,
+        nsnap=2000, train_frac=0.8, epochs=12, lr=1e-3, windows=60, window=50, lr_post=3e-4,
+        eval_steps=2000, tol=0.05, channels=32, seed=0):
+    wall0 = time.time()
+    data = generate(fine_n, les_n, kcut, nu, dt_dns, stride, spinup_time, nsnap, seed)
+    coarse = data["coarse"]
+    V = coarse.Nx * coarse.Ny * coarse.Nz
+    ntrain = int(train_frac * nsnap)
+    train, test = np.arange(ntrain), np.arange(ntrain, nsnap)
+    net = ClosureNet(coarse, channels=channels, seed=seed)
+    sub = np.linspace(0, ntrain - 1, min(ntrain, 50)).astype(int)
+    gs = torch.stack([velocity_gradient(data["ubar"][s], coarse).reshape(9, V).std(dim=1) for s in sub]).mean(0)
+    ts = torch.stack([_deviatoric(to_physical(data["tau"][s], coarse).reshape(6, V)).std(dim=1)
+                      for s in sub]).mean(0)
+    net.set_scales(1.0 / gs, ts)
+    _log(f"closure net: {net.num_parameters()} parameters, a-priori training on {ntrain} snapshots")
+    apriori = train_apriori(net, data, train, test, epochs, lr, seed)
+    dt = stride * dt_dns
+    A = float(np.mean(data["coef"]))
+    params = NSParams(nu=nu, forcing=LinearForcing(A=A))
+    _log(f"a-posteriori training: {windows} windows of {window} LES steps (dt {dt}, LinearForcing A = {A:.4f})")
+    kmax = int(kcut)
+    mask = sphere_mask(coarse, kcut)
+    apost = train_aposteriori(net, data, params, dt, train, windows, window, lr_post, seed, kmax, mask)
+    steps = min(eval_steps, nsnap - 1)   # LES step n pairs with snapshot n
+    _log(f"evaluation: {steps} LES steps with and without the closure")
+    with_cl = evaluate(net, data, params, dt, steps, kmax, mask)
+    without = evaluate(None, data, params, dt, steps, kmax, mask)
+    dns_s = data["dns_ms_per_step"] * 1e-3 * steps * stride
+    res = {
+        "config": {"fine_n": fine_n, "les_n": les_n, "kcut": kcut, "nu": nu, "dt_dns": dt_dns,
+                   "dt_les": dt, "stride": stride, "spinup_time": spinup_time, "nsnap": nsnap,
+                   "train_snapshots": ntrain, "epochs": epochs, "windows": windows, "window": window,
+                   "eval_steps": steps, "channels": channels, "parameters": net.num_parameters(),
+                   "forcing_les": f"LinearForcing(A={A:.5f})", "tolerance": tol},
+        "dns_ms_per_step": data["dns_ms_per_step"], "stress_ms_per_snapshot": data["stress_ms"],
+        "apriori": apriori, "aposteriori_loss_first_last": [apost[0], float(np.mean(apost[-10:]))],
+        "closure_gain": float(net.gain),
+        "les": with_cl, "les_no_closure": without,
+        "gpu_seconds": {"dns": dns_s, "les": with_cl["seconds"],
+                        "speedup": dns_s / with_cl["seconds"]},
+        "accepted": with_cl["max_rel_err"] <= tol,
+        "fallback": "none" if with_cl["max_rel_err"] <= tol else "plain DNS (closure error above tolerance)",
+        "wall_s": time.time() - wall0,
+    }
+    _log(f"E(k) max relative error, k = 1..{kmax}: closure {with_cl['max_rel_err']:.3f}, "
+         f"no closure {without['max_rel_err']:.3f}; GPU seconds DNS {dns_s:.1f} vs LES "
+         f"{with_cl['seconds']:.2f} ({res['gpu_seconds']['speedup']:.0f}x)")
+    return res
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--quick", action="store_true", help="small smoke configuration (128^3 -> 32^3)")
+    ap.add_argument("--out", default=None, help="write the result JSON here")
+    ap.add_argument("--nsnap", type=int, default=None)
+    ap.add_argument("--epochs", type=int, default=None)
+    ap.add_argument("--windows", type=int, default=None)
+    a = ap.parse_args(argv)
+    kw = {}
+    if a.quick:
+        kw = dict(fine_n=128, les_n=32, kcut=10.0, nu=1 / 500, dt_dns=2e-3, stride=4, spinup_time=4.0,
+                  nsnap=200, epochs=2, windows=10, window=10, eval_steps=200)
+    for k in ("nsnap", "epochs", "windows"):
+        if getattr(a, k) is not None:
+            kw[k] = getattr(a, k)
+    res = run(**kw)
+    line = json.dumps(res)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(line + "\n")
+    print(line, flush=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

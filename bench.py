@@ -109,6 +109,51 @@ def adjoint_timing(torch, h3, N=128, steps=200, checkpoint=20):
             "backward_over_forward": round((el - fw) / fw, 2)}
 
 
+def les_speed(torch, h3, fine_n=512, les_n=64, stride=10, steps=40):
+    """GPU cost of the learned-closure LES against the plain DNS it replaces
+    over the same simulated time: one 64^3 LES step (RK4, closure on every
+    stage, dt = stride * DNS dt) against `stride` 512^3 DNS steps (nu =
+    1/2000, Lundgren). The closure's weights do not change its cost; its
+    accuracy comes from the full training experiment (hit3d.les_experiment)."""
+    from hit3d import les
+    from hit3d.closure import ClosureNet
+
+    def timed(fn, n):
+        fn(2)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn(n)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    nu = 1 / 2000
+    fine = h3.make_grid(fine_n, dtype=torch.float32)
+    uf = h3.random_field(0, fine, h3.config_spectrum)
+    rhs_f = h3.make_ns_rhs(fine)
+    pf = h3.NSParams(nu=nu, forcing=h3.LundgrenForcing(nu=nu))
+    holder = [uf]
+
+    def dns(n):
+        holder[0] = h3.rollout(rhs_f, holder[0], pf, 1e-3, n)
+    dns_ms = timed(dns, 10)
+    del holder[0], uf
+    coarse = h3.make_grid(les_n, dtype=torch.float32)
+    ul = les.filter_state(h3.random_field(1, fine, h3.config_spectrum), fine, coarse, les_n // 3)
+    net = ClosureNet(coarse, channels=32, seed=0)
+    rhs_c = h3.make_ns_rhs(coarse)
+    pc = h3.NSParams(nu=nu, forcing=h3.LinearForcing(A=0.15))
+    lh = [ul]
+
+    def les_run(n):
+        lh[0] = les.les_rollout(lh[0], rhs_c, pc, net, stride * 1e-3, n)
+    les_ms = timed(les_run, steps)
+    return {"dns_grid": fine_n, "les_grid": les_n, "dns_ms_per_step": round(dns_ms, 4),
+            "les_ms_per_step": round(les_ms, 4), "dns_steps_per_les_step": stride,
+            "gpu_seconds_ratio": round(stride * dns_ms / les_ms, 2)}
+
+
 def cpu_baseline(u0, N, nu, dt, budget_s=15.0):
     """Oracle (float32 NumPy/SciPy, all host cores) on the same initial field:
     one warm-up step, then steps until `budget_s` (at least two)."""
@@ -139,6 +184,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the 128^3 adjoint (gradient) timing")
+    ap.add_argument("--no-les", action="store_true",
+                    help="skip the LES-versus-DNS step timing (512^3 DNS, 64^3 LES)")
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run 3 device copies of 1 GiB (reference kernel for the "
@@ -224,6 +271,7 @@ def main():
 
     copy_gbs = hbm_copy_gbs(torch)
     adj = None if args.no_adjoint else adjoint_timing(torch, h3)
+    les_t = None if args.no_les else les_speed(torch, h3)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -254,6 +302,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "hbm_copy_gbs": copy_gbs,
         "adjoint": adj,
+        "les": les_t,
         "passes": passes,
         "spectrum_ms": round(spec_ms, 4),
         "energy": {"start": E0, "end": E1},

@@ -228,6 +228,13 @@ int hit3d_channels_pack(const float* soa, int32_t c, int64_t nvox, const float* 
 int hit3d_channels_unpack(const float* aos, int32_t c, int32_t ld, int64_t nvox, const float* scale,
                           float* soa, void* stream);
 
+/* Unpacks the net's 6 stress channels (aos fp32 [V][6], times scale[c]) into
+   tau fp32 [6][V] with backscatter clipping: tau = 0 where the modelled
+   transfer -tau_ij S_ij < 0, S_ij = (d_j u_i + d_i u_j)/2 from grad9 fp32
+   [3 j + i][V]; keep[v] (uint8) records where tau was kept. */
+int hit3d_stress_unpack_clip(const float* aos, int64_t nvox, const float* scale, const float* grad9,
+                             float* tau, uint8_t* keep, void* stream);
+
 /* Sharp spherical spectral filter onto a coarser grid of the same cubic box:
    u_coarse = (Ncoarse/Nfine) * u_fine on the modes with |k| <= kcut that the
    coarse grid holds without its Nyquist planes, 0 elsewhere. */

@@ -132,6 +132,7 @@ def test_net_autograd_matches_layerwise_torch():
     torch float32 (bf16 rounding where the kernels round)."""
     grid = h3.make_grid(16, dtype=torch.float32)
     net = ClosureNet(grid, channels=32, seed=3)
+    net.clip_backscatter = False          # the raw net against torch (clipping is tested in test_gpu_les)
     net.set_scales(torch.full((9,), 0.5), torch.full((6,), 2.0))
     gen = torch.Generator().manual_seed(11)
     g = torch.randn(9, V, generator=gen).cuda().requires_grad_(True)

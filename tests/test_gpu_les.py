@@ -104,6 +104,7 @@ def test_les_step_with_closure_gradient_matches_central_differences():
     rhs = h3.make_ns_rhs(g)
     params = h3.NSParams(nu=0.02, forcing=h3.LinearForcing(A=0.1))
     net = ClosureNet(g, channels=16, seed=1)
+    net.clip_backscatter = False      # the clipping mask is not differentiable across its edge
     u0 = h3.random_field(5, g, h3.config_spectrum)
     gg = les.velocity_gradient(u0, g).reshape(9, -1)
     net.set_scales(1.0 / gg.std(dim=1), torch.full((6,), 0.05))
@@ -141,3 +142,72 @@ def test_subgrid_stress_matches_numpy():
     got = h3.to_physical(tau_hat, coarse).cpu().numpy()
     assert np.abs(got - tau).max() < 1e-12 * np.abs(tau).max()
     assert np.abs(h3.to_physical(ubar, coarse).cpu().numpy() - ub[:, ::2, ::2, ::2]).max() < 1e-12
+
+
+def test_les_experiment_driver_end_to_end():
+    """The whole pipeline at toy size: DNS 64^3 -> LES 16^3 (kcut 5), data,
+    a-priori and a-posteriori training, evaluation and timing."""
+    from hit3d import les_experiment
+    res = les_experiment.run(fine_n=64, les_n=16, kcut=5.0, nu=1 / 200, dt_dns=4e-3, stride=2,
+                             spinup_time=0.2, nsnap=24, epochs=1, windows=2, window=3,
+                             eval_steps=20, channels=16)
+    assert res["config"]["eval_steps"] == 20 and len(res["les"]["E_les"]) == 5
+    assert all(np.isfinite(res["les"]["rel_err"])) and all(np.isfinite(res["les_no_closure"]["rel_err"]))
+    assert res["gpu_seconds"]["dns"] > 0 and res["gpu_seconds"]["les"] > 0
+    assert isinstance(res["accepted"], bool)
+    assert res["fallback"] == ("none" if res["accepted"] else "plain DNS (closure error above tolerance)")
+
+
+def test_backscatter_clipping_keeps_transfer_dissipative():
+    gen = torch.Generator().manual_seed(9)
+    g = torch.randn(9, 4096, generator=gen).cuda()
+    tau = torch.randn(6, 4096, generator=gen).cuda()
+    out = les.clip_backscatter(tau, g)
+    pairs = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+
+    def transfer(t):
+        return -sum((1.0 if i == j else 2.0) * t[s] * 0.5 * (g[3 * j + i] + g[3 * i + j])
+                    for s, (i, j) in enumerate(pairs))
+    pi_in, pi_out = transfer(tau), transfer(out)
+    assert float(pi_out.min()) >= 0.0
+    keep = pi_in >= 0
+    assert torch.equal(out[:, keep], tau[:, keep]) and float(out[:, ~keep].abs().max()) == 0.0
+    # the library's fused unpack + clip against this statement (raw net output
+    # aos[v][c] scaled by s[c]); the kept/zeroed decision may differ only where
+    # Pi is at rounding level
+    s = torch.linspace(0.5, 1.5, 6).cuda()
+    aos = (tau / s[:, None]).t().contiguous()
+    got = torch.empty(6, 4096, device="cuda")
+    kept = torch.empty(4096, dtype=torch.uint8, device="cuda")
+    h3._lib.call("hit3d_stress_unpack_clip", aos.data_ptr(), 4096, s.data_ptr(), g.data_ptr(),
+                 got.data_ptr(), kept.data_ptr(), h3.grid.stream())
+    ref = les.clip_backscatter((aos * s[None, :]).t().contiguous(), g)
+    diff = (kept.bool() != (transfer((aos * s[None, :]).t()) >= 0))
+    scale = float(transfer((aos * s[None, :]).t()).abs().max())
+    assert float(transfer((aos * s[None, :]).t())[diff].abs().max() if diff.any() else 0.0) <= 1e-5 * scale
+    same = ~diff
+    assert float((got[:, same] - ref[:, same]).abs().max()) <= 1e-6 * float(ref.abs().max())
+
+
+def test_les_graph_replays_match_eager_steps():
+    """A captured LES step (closure, spherical mask) replayed several times,
+    with eager work and synchronisation between replays; every replay equals
+    the eager step from the same state."""
+    g = h3.make_grid(32, dtype=torch.float32)
+    rhs = h3.make_ns_rhs(g)
+    params = h3.NSParams(nu=0.01, forcing=h3.LinearForcing(A=0.1))
+    net = ClosureNet(g, channels=32, seed=2)
+    u0 = les.filter_state(h3.random_field(4, g, h3.config_spectrum), g, g, 10.0)
+    gg = les.velocity_gradient(u0, g).reshape(9, -1)
+    net.set_scales(1.0 / gg.std(dim=1), torch.full((6,), 0.05))
+    mask = les.sphere_mask(g, 10.0)
+    G = les.LesStepGraph(u0, rhs, params, net, 4e-3, mask)
+    u_eager = u0.clone()
+    for r in range(4):
+        with torch.no_grad():
+            u_eager = les.les_step(u_eager, rhs, params, net, 4e-3, 0.0, mask)
+        h3.energy(u_eager, g)                     # eager work and a host sync between replays
+        torch.cuda.synchronize()
+        out = G.step().clone()
+        assert torch.equal(out, u_eager), r
+    assert float((u_eager * (1 - mask)).abs().max()) == 0.0   # stays inside the sphere
