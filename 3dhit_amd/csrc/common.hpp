@@ -260,6 +260,11 @@ template <int N>
 struct FftSel<N, true> {
   using type = Fft3<N>;
 };
+// 256 points: the three-level transform over 32 threads (8 values per thread)
+template <>
+struct FftSel<256, false> {
+  using type = Fft3<256, 32>;
+};
 template <int N>
 using FftFor = typename FftSel<N>::type;
 

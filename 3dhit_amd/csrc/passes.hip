@@ -16,7 +16,7 @@ namespace h3 {
 // Plain strided pass (any axis, either direction)
 // ---------------------------------------------------------------------------
 template <int N, int DIR, class T>
-__global__ __launch_bounds__(StridedShape<N>::THREADS) void k_strided(StridedArgs<T> a) {
+__global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(MinWaves<N, T>::value))) void k_strided(StridedArgs<T> a) {
   using S = StridedShape<N>;
   using F = typename S::F;
   constexpr int R = S::R, Q = S::Q, TC = S::TC;
@@ -64,7 +64,7 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) void k_strided(StridedArg
 // out fields: [u0, u1, u2, w1, w2]
 // ---------------------------------------------------------------------------
 template <int N, class T>
-__global__ __launch_bounds__(StridedShape<N>::THREADS) void k_x_inverse_curl(const cpx<T>* __restrict__ us,
+__global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(MinWaves<N, T>::value))) void k_x_inverse_curl(const cpx<T>* __restrict__ us,
                                                             int64_t us_fs,
                                                             cpx<T>* __restrict__ out,
                                                             int64_t out_fs, Geo g) {
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) void k_x_inverse_curl(con
 // out [6][Nx][Ny][Kz] fields (u0, u1, u2, w0, w1, w2)
 // ---------------------------------------------------------------------------
 template <int N, class T>
-__global__ __launch_bounds__(StridedShape<N>::THREADS) void k_y_inverse_curl(const cpx<T>* __restrict__ in,
+__global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(MinWaves<N, T>::value))) void k_y_inverse_curl(const cpx<T>* __restrict__ in,
                                                             int64_t in_fs,
                                                             cpx<T>* __restrict__ out,
                                                             int64_t out_fs, Geo g) {
@@ -169,9 +169,9 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) void k_y_inverse_curl(con
   };
   // each field's loads are issued one transform ahead of its use (pf holds
   // the prefetched field), so the barriers inside run() hide their latency
-  // At N >= 512 the three-level transform needs more registers, so field 0
+  // The three-level transforms (N >= 256) need more registers, so field 0
   // is fetched after w's transform (three arrays live instead of four).
-  constexpr bool kEarly = N < 512;
+  constexpr bool kEarly = N < 256;
   cpx<T> a[R], b[R], pf[R], w[R];
   load(1, a);
   load(2, b);
@@ -404,7 +404,7 @@ __device__ __forceinline__ void block_sum3(double a, double b, double c, double*
 // and the RK4 stage update (integrators.py:34-38).
 // ---------------------------------------------------------------------------
 template <int N, int MODE, class T>
-__global__ __launch_bounds__(StridedShape<N>::THREADS) void k_x_forward_stage(StageArgs<T> a) {
+__global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(MinWaves<N, T>::value))) void k_x_forward_stage(StageArgs<T> a) {
   using S = StridedShape<N>;
   using F = typename S::F;
   constexpr int R = S::R, Q = S::Q, TC = S::TC;

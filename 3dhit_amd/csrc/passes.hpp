@@ -8,11 +8,19 @@ namespace h3 {
 constexpr int kThreads = 256;   // strided passes: 4 waves
 constexpr int kZThreads = 64;   // z passes: one wave, so the pencil exchanges need no cross-wave barrier
 
+// Minimum waves per SIMD the compiler must allow: the 256-point float
+// strided passes run 8-wave workgroups, which fit twice per CU only at 4
+// waves per SIMD (<= 128 VGPRs); double needs twice the registers.
+template <int N, class T>
+struct MinWaves {
+  static constexpr int value = (N == 256 && sizeof(T) == 4) ? 4 : 1;
+};
+
 template <int N>
 struct StridedShape {
   using F = FftFor<N>;
   static constexpr int R = F::R, Q = F::Q;
-  static constexpr int TC = (kThreads / Q) > 8 ? kThreads / Q : 8;  // columns per workgroup
+  static constexpr int TC = N == 256 ? 16 : ((kThreads / Q) > 8 ? kThreads / Q : 8);  // columns per workgroup
   static constexpr int THREADS = TC * Q;
   static constexpr int LDS = TC * F::kSlots;
 };

@@ -161,7 +161,7 @@ def _shell_energy(u, grid, kmax):
 
 
 def train_aposteriori(net, data, params, dt, train, windows, window, lr, seed, kmax, mask,
-                      traj_weight=0.1, lr_gain=2e-2):
+                      traj_weight=0.1, lr_gain=1e-2):
     """Loss per window: mean over its steps of mean_k log(E_les(k)/E_dns(k))^2
     plus traj_weight times the relative L2^2 distance to the filtered DNS
     state; the gradient reaches the net's parameters (library backward
@@ -220,7 +220,7 @@ def evaluate(net, data, params, dt, steps, kmax, mask, every=10):
 
 
 def run(fine_n=512, les_n=64, kcut=21.0, nu=1 / 2000, dt_dns=1e-3, stride=10, spinup_time=10.0,
-        nsnap=2000, train_frac=0.8, epochs=12, lr=1e-3, windows=150, window=50, lr_post=1e-3,
+        nsnap=2000, train_frac=0.8, epochs=12, lr=1e-3, windows=60, window=50, lr_post=3e-4,
         eval_steps=2000, tol=0.05, channels=32, seed=0):
     wall0 = time.time()
     data = generate(fine_n, les_n, kcut, nu, dt_dns, stride, spinup_time, nsnap, seed)
