@@ -109,6 +109,28 @@ def adjoint_timing(torch, h3, N=128, steps=200, checkpoint=20):
             "backward_over_forward": round((el - fw) / fw, 2)}
 
 
+def profiled_traffic(N, kernel_key):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE + WRITE_SIZE summary of this workload (counters
+    cannot be read inside this process), with the file it came from; None
+    when there is no summary for N."""
+    path = os.path.join(ROOT, "profiles", f"rk4_{N}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None, None
+    pat = {"z_c2r_cross_r2c": f"k_z_fused<{N},", "y_inverse_curl": f"k_y_inverse_curl<{N},",
+           "x_inverse_curl": f"k_x_inverse_curl<{N},", "y_forward": f"k_strided<{N}, -1,"}.get(kernel_key)
+    if pat is None:
+        return None, None
+    for line in open(path):
+        if pat in line:
+            cols = line.split()
+            try:
+                return int(float(cols[-3]) * 1e6), os.path.relpath(path, ROOT)  # "total MB" column
+            except ValueError:
+                return None, None
+    return None, None
+
+
 def les_speed(torch, h3, fine_n=512, les_n=64, stride=10, steps=40):
     """GPU cost of the learned-closure LES against the plain DNS it replaces
     over the same simulated time: one 64^3 LES step (RK4, closure on every
@@ -257,6 +279,7 @@ def main():
         passes[name] = d
     dom = max((k for k in passes if k in per_bytes), key=lambda k: passes[k]["avg_ms"] * passes[k]["launches"])
     achieved = passes[dom]["gbs"]
+    traffic, traffic_src = profiled_traffic(N, dom)
 
     # spectrum cost (not part of the step)
     spec = h3.diagnostics.shell_spectrum(u, g)
@@ -299,7 +322,9 @@ def main():
         "step_bytes": step_bytes,
         "hbm_frac_step": round(step_bytes * sps / (HBM_PEAK_GBS * 1e9), 4),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled)",
+                     "traffic_source": traffic_src, "algorithmic_bytes": per_bytes[dom]},
         "hbm_copy_gbs": copy_gbs,
         "adjoint": adj,
         "les": les_t,
