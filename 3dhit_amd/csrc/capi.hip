@@ -232,6 +232,37 @@ __global__ __launch_bounds__(256) void k_spectrum_final(const double* __restrict
   out[b] = s.value() * scale;
 }
 
+// coef = [A, D, 2E] from k_hold_partials' sums: D = dE/dt of du, 2E = sum w|u|^2 / Ntot^2,
+// A = (-D - (E - energy)/tau) / (2E) (no relaxation term when tau <= 0 or infinite), 0 when
+// 2E <= tiny. Same fixed-order Neumaier reduction as k_finalize.
+__global__ __launch_bounds__(256) void k_hold_coef(const double* __restrict__ partials, int nblocks,
+                                                   double inv_ntot2, double energy, double tau,
+                                                   double tiny, double* __restrict__ coef) {
+  __shared__ double sh[2][256];
+  h3::KSum s[2];
+  for (int i = threadIdx.x; i < nblocks; i += 256)
+    for (int c = 0; c < 2; ++c) s[c].add(partials[3 * i + c]);
+  for (int c = 0; c < 2; ++c) sh[c][threadIdx.x] = s[c].value();
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int c = 0; c < 2; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double D = sh[0][0] * inv_ntot2, twoE = sh[1][0] * inv_ntot2;
+    double a = 0.0;
+    if (twoE > tiny) {
+      double rate = -D;
+      if (tau > 0.0 && isfinite(tau)) rate -= (0.5 * twoE - energy) / tau;
+      a = rate / twoE;
+    }
+    coef[0] = a;
+    coef[1] = D;
+    coef[2] = twoE;
+  }
+}
+
 __global__ void k_energy_out(const double* __restrict__ scalars, double nu,
                              double* __restrict__ out) {
   // [E, eps, Omega, E_band]
@@ -1274,6 +1305,45 @@ int hit3d_stress_divergence(const hit3d_grid* grid, const void* in, void* out, i
   else
     H3_HIP(h3::launch_stress_div<double>(static_cast<const cpx<double>*>(in), static_cast<cpx<double>*>(out),
                                          g, adjoint ? 1 : 0, s));
+  return HIT3D_OK;
+}
+
+int hit3d_hold_energy(const hit3d_grid* grid, const void* u_hat, const void* du_hat, void* out,
+                      double energy, double tau, double* coef, void* work, size_t work_bytes,
+                      void* stream) {
+  H3_TRY(check_grid(grid));
+  H3_TRY(check_ptr(u_hat, "u_hat"));
+  H3_TRY(check_ptr(du_hat, "du_hat"));
+  H3_TRY(check_ptr(out, "out"));
+  H3_TRY(check_ptr(coef, "coef"));
+  if (!(energy >= 0.0) || std::isnan(tau))
+    return fail(HIT3D_ERR_INVALID, "energy must be >= 0 and tau a number");
+  H3_TRY(check_work(work, work_bytes, reduce_bytes(*grid, nullptr, nullptr, nullptr)));
+  double* partials;
+  reduce_bytes(*grid, &partials, nullptr, work);
+  const Geo g = make_geo(*grid, false);
+  const int64_t fs = int64_t(g.nx) * g.ny * g.nzh;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (grid->precision == 4)
+    H3_HIP(h3::launch_hold_partials<float>(static_cast<const cpx<float>*>(u_hat),
+                                           static_cast<const cpx<float>*>(du_hat), fs, g, partials,
+                                           s));
+  else
+    H3_HIP(h3::launch_hold_partials<double>(static_cast<const cpx<double>*>(u_hat),
+                                            static_cast<const cpx<double>*>(du_hat), fs, g,
+                                            partials, s));
+  const double tiny = grid->precision == 4 ? double(FLT_EPSILON) : DBL_EPSILON;
+  hipLaunchKernelGGL(k_hold_coef, dim3(1), dim3(256), 0, s, partials, h3::kReduceBlocks,
+                     ntot2_inv(*grid), energy, tau, tiny, coef);
+  H3_HIP(hipGetLastError());
+  if (grid->precision == 4)
+    H3_HIP(h3::launch_hold_apply<float>(static_cast<const cpx<float>*>(u_hat),
+                                        static_cast<const cpx<float>*>(du_hat),
+                                        static_cast<cpx<float>*>(out), 3 * fs, coef, s));
+  else
+    H3_HIP(h3::launch_hold_apply<double>(static_cast<const cpx<double>*>(u_hat),
+                                         static_cast<const cpx<double>*>(du_hat),
+                                         static_cast<cpx<double>*>(out), 3 * fs, coef, s));
   return HIT3D_OK;
 }
 

@@ -381,6 +381,43 @@ __global__ __launch_bounds__(kBlock) void k_dot_partials(const cpx<T>* __restric
 }
 
 template <class T>
+__global__ __launch_bounds__(kBlock) void k_hold_partials(const cpx<T>* __restrict__ u,
+                                                          const cpx<T>* __restrict__ du,
+                                                          int64_t fs, Geo g,
+                                                          double* __restrict__ partials) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  double s1 = 0.0, s2 = 0.0;
+  for (int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x; idx < n;
+       idx += int64_t(gridDim.x) * kBlock) {
+    double a = 0.0, b = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      const cpx<T> x = u[c * fs + idx], y = du[c * fs + idx];
+      a += double(x.x) * y.x + double(x.y) * y.y;
+      b += double(x.x) * x.x + double(x.y) * x.y;
+    }
+    const int w = hermitian_weight(int(idx % g.nzh), g.nz);
+    s1 += w * a;
+    s2 += w * b;
+  }
+  block_sum3m(s1, s2, 0.0, partials + 3 * blockIdx.x);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_hold_apply(const cpx<T>* __restrict__ u,
+                                                       const cpx<T>* __restrict__ du,
+                                                       cpx<T>* __restrict__ out, int64_t n,
+                                                       const double* __restrict__ coef) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const T a = T(coef[0]);
+  const cpx<T> x = u[i], y = du[i];
+  cpx<T> r;
+  r.x = y.x + a * x.x;
+  r.y = y.y + a * x.y;
+  out[i] = r;
+}
+
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_adj_combine(const cpx<T>* __restrict__ AB,
                                                         const cpx<T>* __restrict__ v,
                                                         const cpx<T>* __restrict__ vp,
@@ -550,6 +587,22 @@ hipError_t launch_dot_partials(const cpx<T>* x, const cpx<T>* y, int64_t fs, con
 }
 
 template <class T>
+hipError_t launch_hold_partials(const cpx<T>* u, const cpx<T>* du, int64_t fs, const Geo& g,
+                                double* partials, hipStream_t s) {
+  hipLaunchKernelGGL((k_hold_partials<T>), dim3(kReduceBlocks), dim3(kBlock), 0, s, u, du, fs, g,
+                     partials);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_hold_apply(const cpx<T>* u, const cpx<T>* du, cpx<T>* out, int64_t n,
+                             const double* coef, hipStream_t s) {
+  hipLaunchKernelGGL((k_hold_apply<T>), dim3(nblocks_for(n)), dim3(kBlock), 0, s, u, du, out, n,
+                     coef);
+  return hipGetLastError();
+}
+
+template <class T>
 hipError_t launch_adj_combine(const cpx<T>* AB, const cpx<T>* v, const cpx<T>* vp,
                               const cpx<T>* u, cpx<T>* out, int64_t fs, const Geo& g, T kmin2,
                               T kmax2, const AdjCombine& c, hipStream_t s) {
@@ -576,7 +629,11 @@ hipError_t launch_axpby(cpx<T>* out, T a, const cpx<T>* x, T b, const cpx<T>* y,
                                             const cpx<T>*, cpx<T>*, int64_t, const Geo&, T, T,     \
                                             const AdjCombine&, hipStream_t);                       \
   template hipError_t launch_axpby<T>(cpx<T>*, T, const cpx<T>*, T, const cpx<T>*, int64_t,       \
-                                      hipStream_t);
+                                      hipStream_t);                                                       \
+  template hipError_t launch_hold_partials<T>(const cpx<T>*, const cpx<T>*, int64_t, const Geo&,  \
+                                              double*, hipStream_t);                               \
+  template hipError_t launch_hold_apply<T>(const cpx<T>*, const cpx<T>*, cpx<T>*, int64_t,        \
+                                           const double*, hipStream_t);
 
 H3_INSTANTIATE_ADJ(float)
 H3_INSTANTIATE_ADJ(double)

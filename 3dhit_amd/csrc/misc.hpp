@@ -56,6 +56,15 @@ hipError_t launch_adj_cross(const T* ph, int64_t pfs, T* out, int64_t ofs, int64
 template <class T>
 hipError_t launch_dot_partials(const cpx<T>* x, const cpx<T>* y, int64_t fs, const Geo& g,
                                T kmin2, T kmax2, int use_band, double* partials, hipStream_t s);
+// partials[kReduceBlocks][3]: (sum w Re(conj(u) du), sum w |u|^2, 0) over full-layout states,
+// w the Hermitian weight of the mode's kz column
+template <class T>
+hipError_t launch_hold_partials(const cpx<T>* u, const cpx<T>* du, int64_t fs, const Geo& g,
+                                double* partials, hipStream_t s);
+// out = du + coef[0] u over n complex values (coef on the device)
+template <class T>
+hipError_t launch_hold_apply(const cpx<T>* u, const cpx<T>* du, cpx<T>* out, int64_t n,
+                             const double* coef, hipStream_t s);
 struct AdjCombine {
   int fkind;          // HIT3D_FORCING_*
   double a;           // LINEAR coefficient

@@ -186,6 +186,13 @@ class ClosureNet:
         for L in self.layers:
             L.zero_grad()
 
+    def reset_optimizer(self):
+        """Zero the Adam moments and step count (a new training phase)."""
+        self.t = 0
+        for L in self.layers:
+            for m in (L.mw, L.vw, L.mb, L.vb):
+                m.zero_()
+
     def adam_step(self, lr):
         self.t += 1
         for L in self.layers:

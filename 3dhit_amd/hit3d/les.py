@@ -26,6 +26,7 @@ from __future__ import annotations
 
 import math
 import time
+from dataclasses import dataclass
 
 import numpy as np
 import torch
@@ -33,11 +34,13 @@ import torch
 from . import _lib
 from .closure import ClosureNet, apply_net
 from .diagnostics import energy_spectrum
+from .forcing import NoForcing
 from .grid import check_spectral, make_grid, stream, to_physical, to_spectral
 from .navierstokes import NSParams
 
 __all__ = ["filter_state", "subgrid_stress", "velocity_gradient", "les_rhs", "les_step",
-           "les_rollout", "closure_force", "sphere_mask", "LesStepGraph", "ClosureNet"]
+           "les_rollout", "closure_force", "sphere_mask", "LesStepGraph", "ClosureNet",
+           "HoldResolvedEnergy", "hold_energy"]
 
 
 # ---------------------------------------------------------------------------
@@ -233,11 +236,88 @@ def sphere_mask(grid, kcut):
     return (grid.k2 <= float(kcut) ** 2).to(grid.dtype)
 
 
+@dataclass(frozen=True)
+class HoldResolvedEnergy:
+    """LES forcing f = A(u) u that holds the resolved energy: A is set from
+    every other term of the LES RHS (closure included) so that
+    dE/dt = -(E - energy)/tau, or dE/dt = 0 with tau=None. It is the LES
+    counterpart of the DNS's constant-energy forcing (forcing.py:5-20 for
+    the form f = A u): the DNS holds its total energy, the LES holds its
+    resolved energy at the filtered DNS's mean. Only les_rhs knows it; the
+    solver's own RHS runs unforced underneath (hit3d_hold_energy)."""
+
+    energy: float
+    tau: float | None = 1.0
+
+    @property
+    def label(self):
+        return f"HoldResolvedEnergy_E{float(self.energy):g}_tau{self.tau}"
+
+
+def _hold_raw(u, du, grid, energy, tau):
+    u = check_spectral(u, grid, 3)
+    if du.shape != u.shape or du.dtype != u.dtype or du.device != u.device:
+        raise ValueError("du must match u in shape, dtype and device")
+    out = torch.empty_like(du)
+    coef = torch.empty(3, dtype=torch.float64, device=du.device)
+    ws = grid.workspace(_lib.OP_REDUCE)
+    _lib.call("hit3d_hold_energy", grid.desc, u.contiguous().data_ptr(), du.contiguous().data_ptr(),
+              out.data_ptr(), float(energy), 0.0 if tau is None else float(tau), coef.data_ptr(),
+              ws.data_ptr(), ws.numel(), stream())
+    return out, coef
+
+
+class _HoldEnergy(torch.autograd.Function):
+    """out = du + A(u, du) u (hit3d_hold_energy); the backward differentiates
+    A through its two FP64 sums s1 = sum w Re(conj u du)/Ntot^2 and
+    s2 = sum w |u|^2/Ntot^2: dA/ds1 = -1/s2, dA/ds2 = (-r/2 - A)/s2 with
+    r = 1/tau (0 without relaxation)."""
+
+    @staticmethod
+    def forward(ctx, u, du, grid, energy, tau):
+        u, du = u.contiguous(), du.contiguous()
+        out, coef = _hold_raw(u, du, grid, energy, tau)
+        ctx.save_for_backward(u, du, coef)
+        ctx.grid = grid
+        ctx.r = 1.0 / float(tau) if tau is not None and 0 < float(tau) < math.inf else 0.0
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        u, du, coef = ctx.saved_tensors
+        grid = ctx.grid
+        A, s2 = coef[0], coef[2]
+        tiny = float(torch.finfo(grid.dtype).eps)
+        live = (s2 > tiny).to(coef.dtype)               # A = 0 (and constant) when 2E <= tiny
+        c = torch.sum(g.real.double() * u.real + g.imag.double() * u.imag)   # Re sum conj(g) u
+        safe = torch.where(s2 > 0, s2, torch.ones_like(s2))
+        k1 = (live * c * (-1.0 / safe)).to(grid.dtype)
+        k2 = (live * c * ((-0.5 * ctx.r - A) / safe)).to(grid.dtype)
+        wn = grid.weights / float(grid.Nx * grid.Ny * grid.Nz) ** 2
+        gdu = g + (k1 * wn) * u
+        gu = A.to(grid.dtype) * g + (k1 * wn) * du + (2.0 * k2 * wn) * u
+        return gu, gdu, None, None, None
+
+
+def hold_energy(u, du, grid, energy, tau=1.0):
+    """du + A u with A from hit3d_hold_energy (differentiable in u and du)."""
+    return _HoldEnergy.apply(u, du, grid, energy, tau)
+
+
 def les_rhs(u, rhs, params, net, t=0.0, mask=None):
-    du = _NSRhs.apply(u, rhs, params, t)
+    """NS RHS + closure force, restricted to `mask`; with a HoldResolvedEnergy
+    forcing the NS part runs unforced and the energy-holding term is added
+    last, from the masked total."""
+    hold = params.forcing if isinstance(params.forcing, HoldResolvedEnergy) else None
+    p = NSParams(nu=params.nu, forcing=NoForcing()) if hold is not None else params
+    du = _NSRhs.apply(u, rhs, p, t)
     if net is not None:
         du = du + closure_force(u, rhs.grid, net)
-    return du if mask is None else du * mask
+    if mask is not None:
+        du = du * mask
+    if hold is not None:
+        du = hold_energy(u, du, rhs.grid, hold.energy, hold.tau)
+    return du
 
 
 def les_step(u, rhs, params, net, dt, t=0.0, mask=None):

@@ -8,24 +8,33 @@
    channels normalised)
    on the first `train_frac` of the snapshots; the loss on the rest is
    reported.
-3. A-posteriori training: `window`-step LES rollouts from filtered DNS
-   states, loss = mean over the steps of the squared log error of the shell
-   spectrum against the filtered DNS, plus a small relative-L2 trajectory
-   term; gradients through the LES steps (the solver's hand-written RHS
-   adjoint plus the net's backward kernels) reach the net and the scalar
-   gain of the closure force. The LES evolves exactly the filter's modes
-   (|k| <= kcut, a sphere inside the solver's 2/3 cube).
-4. Evaluation: an LES of `eval_steps` steps (LES dt = stride * DNS dt) from
-   the first snapshot, with and without the closure; the time-averaged
-   shell spectrum E(k), k = 1..kcut, against the filtered DNS spectrum over
-   the same times. The closure is accepted when max_k |E_les/E_dns - 1| <=
-   tol; otherwise the result falls back to the plain DNS. GPU time of the
-   DNS and of the LES over the same simulated time is measured with HIP
-   events.
+3. A-posteriori training, gradients through the LES steps (the solver's
+   hand-written RHS adjoint plus the net's backward kernels) reaching the
+   net and the scalar gain of the closure force: first `windows`
+   trajectory windows of `window` steps from filtered DNS states (squared
+   log error of the shell spectrum against the filtered DNS plus a small
+   relative-L2 trajectory term), then `attractor_windows` statistical
+   windows that start after `spin` gradient-free LES steps, on the LES's
+   own attractor, and match the window-averaged spectrum to the filtered
+   DNS's mean spectrum. Last, the closure gain is calibrated by a few
+   secant steps so that the upper shells' mean energy over a `calib_steps`
+   LES matches the training snapshots' mean. The LES evolves exactly the
+   filter's modes (|k| <= kcut, a sphere inside the solver's 2/3 cube).
+4. Evaluation: an LES from the first snapshot, with and without the
+   closure; the time-averaged shell spectrum E(k), k = 1..kcut, over the
+   DNS record's time window against the filtered DNS spectrum over the
+   same times. The closure is accepted when max_k |E_les/E_dns - 1| <=
+   tol; otherwise the result falls back to the plain DNS. Reported beside
+   it: the same statistic over `long_factor` times the record, and the DNS
+   record's own half-versus-half spectrum difference (its sampling noise).
+   GPU time of the DNS and of the LES over the same simulated time is
+   measured with HIP events.
 
-The LES forcing is LinearForcing with the DNS's mean Lundgren coefficient
-A: the filter commutes with f = A u, so this is the forcing of the filtered
-equations. Run: python -m hit3d.les_experiment [--quick] [--out FILE]
+The LES forcing (HoldResolvedEnergy) is f = A(u) u with A set at every
+stage so that the resolved energy relaxes to the filtered DNS's mean
+(time scale `relax`), counting the closure's transfer: the DNS holds its
+total energy with the same form of forcing, and the filter commutes with
+f = A u. Run: python -m hit3d.les_experiment [--quick] [--out FILE]
 (with 3dhit_amd on sys.path)."""
 
 from __future__ import annotations
@@ -42,11 +51,11 @@ import torch
 from .closure import ClosureNet
 from .diagnostics import energy_spectrum, quadratic_sums
 from .fields import config_spectrum, random_field
-from .forcing import LinearForcing, LundgrenForcing
+from .forcing import LundgrenForcing
 from .grid import make_grid, to_physical
 from .integrators import rollout
-from .les import (check_les_state, les_rollout, les_step, sphere_mask, subgrid_stress,
-                  velocity_gradient)
+from .les import (HoldResolvedEnergy, check_les_state, les_rollout, les_step, sphere_mask,
+                  subgrid_stress, velocity_gradient)
 from .navierstokes import NSParams, make_ns_rhs
 
 
@@ -94,6 +103,7 @@ def generate(fine_n, les_n, kcut, nu, dt_dns, stride, spinup_time, nsnap, seed):
     ubar = torch.empty(shape, dtype=coarse.complex_dtype, device=coarse.device)
     tau = torch.empty((nsnap, 6) + shape[2:], dtype=coarse.complex_dtype, device=coarse.device)
     coef = np.empty(nsnap)
+    e_res = np.empty(nsnap)
     t0 = nspin * dt_dns
     dns_steps_s, stress_s = 0.0, 0.0
     for s in range(nsnap):
@@ -104,11 +114,12 @@ def generate(fine_n, les_n, kcut, nu, dt_dns, stride, spinup_time, nsnap, seed):
         ubar[s], tau[s] = ub, th
         E, Om = _e_omega(u, fine)
         coef[s] = nu * Om / E                     # Lundgren A = 2 nu Omega / (2 E)
+        e_res[s] = _e_omega(ub, coarse)[0]        # resolved (filtered) energy
         if (s + 1) % max(1, nsnap // 20) == 0:
             _log(f"  snapshot {s + 1}/{nsnap}: E = {E:.4f}, A = {coef[s]:.4f}, "
                  f"DNS {1e3 * dns_steps_s / ((s + 1) * stride):.2f} ms/step, stress {1e3 * stress_s / (s + 1):.1f} ms")
     del u
-    return {"fine": fine, "coarse": coarse, "ubar": ubar, "tau": tau, "coef": coef,
+    return {"fine": fine, "coarse": coarse, "ubar": ubar, "tau": tau, "coef": coef, "e_res": e_res,
             "dns_ms_per_step": 1e3 * dns_steps_s / (nsnap * stride),
             "stress_ms": 1e3 * stress_s / nsnap, "t_start": t0}
 
@@ -161,67 +172,147 @@ def _shell_energy(u, grid, kmax):
 
 
 def train_aposteriori(net, data, params, dt, train, windows, window, lr, seed, kmax, mask,
-                      traj_weight=0.1, lr_gain=1e-2):
-    """Loss per window: mean over its steps of mean_k log(E_les(k)/E_dns(k))^2
-    plus traj_weight times the relative L2^2 distance to the filtered DNS
-    state; the gradient reaches the net's parameters (library backward
-    kernels, Adam on the device) and the closure gain."""
+                      traj_weight=0.1, lr_gain=1e-2, attractor_windows=0, spin=150, target=None):
+    """Two kinds of window, each ending in one Adam step of the net's
+    parameters (library backward kernels) and of the closure gain:
+
+    - `windows` trajectory windows from filtered DNS states: loss = mean
+      over the window's steps of mean_k log(E_les(k)/E_dns(k))^2 plus
+      traj_weight times the relative L2^2 distance to the filtered DNS;
+    - then `attractor_windows` statistical windows: `spin` LES steps without
+      gradient from a filtered DNS state (so that the window starts on the
+      LES's own attractor), then `window` steps with gradient; loss =
+      mean_k log(<E_les(k)>_window / target(k))^2, target the filtered DNS
+      spectrum averaged over the training snapshots. These correct the
+      long-time statistics that short trajectory windows cannot see."""
     coarse = data["coarse"]
     rhs = make_ns_rhs(coarse)
     w = coarse.weights
     rng = np.random.default_rng(seed + 1)
-    starts = [int(s) for s in rng.choice(len(train) - window, size=windows)]
+    net.reset_optimizer()                 # fresh Adam moments: the a-priori ones are for another loss
     opt_gain = torch.optim.Adam([net.gain], lr=lr_gain)
     hist = []
-    for i, k0 in enumerate(starts):
-        u = data["ubar"][k0].clone()
-        loss = 0.0
+    total = windows + attractor_windows
+    for i in range(total):
+        k0 = int(rng.integers(0, len(train) - window))
+        u = data["ubar"][int(train[k0])].clone()
+        statistical = i >= windows
+        if statistical:
+            with torch.no_grad():
+                for m in range(spin):
+                    u = les_step(u, rhs, params, net, dt, 0.0, mask)
+        loss, acc = 0.0, 0.0
         for m in range(1, window + 1):
             u = les_step(u, rhs, params, net, dt, (k0 + m - 1) * dt, mask)
-            ref = data["ubar"][k0 + m]
+            e_les = _shell_energy(u, coarse, kmax)
+            if statistical:
+                acc = acc + e_les / window
+                continue
+            ref = data["ubar"][int(train[k0]) + m]
             num = torch.sum(w * (u - ref).abs() ** 2)
             den = torch.sum(w * ref.abs() ** 2)
-            lspec = torch.mean(torch.log(_shell_energy(u, coarse, kmax) / _shell_energy(ref, coarse, kmax)) ** 2)
+            lspec = torch.mean(torch.log(e_les / _shell_energy(ref, coarse, kmax)) ** 2)
             loss = loss + (lspec + traj_weight * num / den) / window
+        if statistical:
+            loss = torch.mean(torch.log(acc / target) ** 2)
         net.zero_grad()
         opt_gain.zero_grad()
         loss.backward()
         net.adam_step(lr)
         opt_gain.step()
-        hist.append(float(loss))
-        if (i + 1) % max(1, windows // 10) == 0:
-            _log(f"  a-posteriori window {i + 1}/{windows}: loss over {window} steps "
-                 f"{np.mean(hist[-10:]):.4e}, gain {float(net.gain):.3f}")
+        hist.append(float(loss.detach()))
+        if (i + 1) % 10 == 0 or i + 1 == total:
+            kind = "attractor" if statistical else "trajectory"
+            _log(f"  a-posteriori window {i + 1}/{total} ({kind}): loss {np.mean(hist[-10:]):.4e}, "
+                 f"gain {float(net.gain.detach()):.3f}")
     return hist
 
 
-def evaluate(net, data, params, dt, steps, kmax, mask, every=10):
+def calibrate_gain(net, data, params, dt, kmax, mask, target, steps=2000, iters=4, every=10,
+                   slope=-1.3):
+    """Secant iterations on the closure gain g so that the LES's mean
+    spectrum over the upper shells (k >= kmax/2) matches the training
+    snapshots' mean: f(g) = mean_k log(<E_les(k)>/target(k)) over those
+    shells, <.> over the last three quarters of a `steps`-step LES from the
+    first snapshot. The upper shells' energy falls as the gain rises (the
+    closure drains them); `slope` (d f / d log g) seeds the first step and
+    every step is clamped to a factor of 1.2. Uses training data only."""
+    coarse = data["coarse"]
+    rhs = make_ns_rhs(coarse)
+    lo = kmax // 2
+    tgt = target.double().cpu().numpy()
+
+    def f(g):
+        with torch.no_grad():
+            net.gain.fill_(g)
+        spectra = []
+        les_rollout(data["ubar"][0].clone(), rhs, params, net, dt, steps, mask=mask, every=every,
+                    observe=lambda n, u: spectra.append(_spectrum(u, coarse, kmax)))
+        E = np.mean(spectra[len(spectra) // 4:], axis=0)
+        return float(np.mean(np.log(E[lo - 1:] / tgt[lo - 1:])))
+
+    g0 = float(net.gain.detach())
+    f0 = f(g0)
+    hist = [(g0, f0)]
+    _log(f"  gain calibration: gain {g0:.4f}, upper-shell mean log error {f0:+.4f}")
+    g1 = g0 * math.exp(min(max(-f0 / slope, -math.log(1.2)), math.log(1.2)))
+    for _ in range(iters if abs(f0) >= 2e-3 else 0):
+        f1 = f(g1)
+        hist.append((g1, f1))
+        _log(f"  gain calibration: gain {g1:.4f}, upper-shell mean log error {f1:+.4f}")
+        if abs(f1) < 2e-3 or f1 == f0:
+            break
+        d = (f1 - f0) / (math.log(g1) - math.log(g0))
+        if d >= 0:                        # noise: fall back to the seed slope
+            d = slope
+        step = min(max(-f1 / d, -math.log(1.2)), math.log(1.2))
+        g0, f0, g1 = g1, f1, g1 * math.exp(step)
+    best = min(hist, key=lambda gf: abs(gf[1]))
+    with torch.no_grad():
+        net.gain.fill_(best[0])
+    return [{"gain": g, "upper_shell_log_error": e} for g, e in hist]
+
+
+def evaluate(net, data, params, dt, steps, kmax, mask, every=10, long_steps=0):
+    """LES of max(steps, long_steps) steps from the first snapshot. The
+    acceptance statistic pairs LES step n with snapshot n over the first
+    `steps` steps (the DNS record's time window); the long run's mean
+    spectrum against the whole record's mean is reported beside it."""
     coarse = data["coarse"]
     rhs = make_ns_rhs(coarse)
     u0 = data["ubar"][0].clone()
     check_les_state(u0, coarse)
     spectra = []
+    nrun = max(steps, long_steps)
 
     def obs(n, u):
         spectra.append(_spectrum(u, coarse, kmax))
     torch.cuda.synchronize()
-    _, sec = _timed(lambda: les_rollout(u0, rhs, params, net, dt, steps, observe=obs, every=every,
+    _, sec = _timed(lambda: les_rollout(u0, rhs, params, net, dt, nrun, observe=obs, every=every,
                                         mask=mask))
-    ref = np.mean([_spectrum(data["ubar"][s], coarse, kmax)
-                   for s in range(every, steps + 1, every) if s < data["ubar"].shape[0]], axis=0)
-    E = np.mean(spectra, axis=0)
+    dns = data["spectra"]
+    npair = min(steps // every, (dns.shape[0] - 1) // every)
+    ref = dns[every:every * npair + 1:every].mean(0)
+    E = np.mean(spectra[:npair], axis=0)
     err = np.abs(E / ref - 1.0)
+    out = {"E_les": E.tolist(), "E_dns_filtered": ref.tolist(), "rel_err": err.tolist(),
+           "max_rel_err": float(err.max()), "seconds_observed": sec}
+    if long_steps > steps:
+        E_long = np.mean(spectra, axis=0)
+        err_long = np.abs(E_long / dns.mean(0) - 1.0)
+        out.update({"long_steps": nrun, "E_les_long": E_long.tolist(), "rel_err_long": err_long.tolist(),
+                    "max_rel_err_long": float(err_long.max())})
     # step cost without the observations (each spectrum synchronises the host)
     nt = min(100, steps)
     _, sec_pure = _timed(lambda: les_rollout(u0, rhs, params, net, dt, nt, mask=mask))
-    return {"E_les": E.tolist(), "E_dns_filtered": ref.tolist(), "rel_err": err.tolist(),
-            "max_rel_err": float(err.max()), "seconds_observed": sec,
-            "ms_per_step": 1e3 * sec_pure / nt, "seconds": sec_pure / nt * steps}
+    out.update({"ms_per_step": 1e3 * sec_pure / nt, "seconds": sec_pure / nt * steps})
+    return out
 
 
 def run(fine_n=512, les_n=64, kcut=21.0, nu=1 / 2000, dt_dns=1e-3, stride=10, spinup_time=10.0,
-        nsnap=2000, train_frac=0.8, epochs=12, lr=1e-3, windows=60, window=50, lr_post=3e-4,
-        eval_steps=2000, tol=0.05, channels=32, seed=0):
+        nsnap=2000, train_frac=0.8, epochs=12, lr=1e-3, windows=30, window=50, lr_post=3e-4,
+        attractor_windows=100, spin=150, calib_steps=2000, eval_steps=2000, long_factor=3, tol=0.05,
+        channels=32, relax=1.0, seed=0):
     wall0 = time.time()
     data = generate(fine_n, les_n, kcut, nu, dt_dns, stride, spinup_time, nsnap, seed)
     coarse = data["coarse"]
@@ -237,26 +328,39 @@ def run(fine_n=512, les_n=64, kcut=21.0, nu=1 / 2000, dt_dns=1e-3, stride=10, sp
     _log(f"closure net: {net.num_parameters()} parameters, a-priori training on {ntrain} snapshots")
     apriori = train_apriori(net, data, train, test, epochs, lr, seed)
     dt = stride * dt_dns
-    A = float(np.mean(data["coef"]))
-    params = NSParams(nu=nu, forcing=LinearForcing(A=A))
-    _log(f"a-posteriori training: {windows} windows of {window} LES steps (dt {dt}, LinearForcing A = {A:.4f})")
     kmax = int(kcut)
+    data["spectra"] = np.stack([_spectrum(data["ubar"][s], coarse, kmax) for s in range(nsnap)])
+    half = nsnap // 2
+    floor = np.abs(data["spectra"][:half].mean(0) / data["spectra"][half:].mean(0) - 1.0)
+    e_target = float(np.mean(data["e_res"][:ntrain]))
+    params = NSParams(nu=nu, forcing=HoldResolvedEnergy(e_target, relax))
+    target = torch.tensor(data["spectra"][:ntrain].mean(0), dtype=torch.float64, device=coarse.device)
+    _log(f"a-posteriori training: {windows} trajectory + {attractor_windows} attractor windows of "
+         f"{window} LES steps (dt {dt}, resolved energy held at {e_target:.4f}); DNS record's own "
+         f"half-vs-half spectrum difference: max {floor.max():.3f}")
     mask = sphere_mask(coarse, kcut)
-    apost = train_aposteriori(net, data, params, dt, train, windows, window, lr_post, seed, kmax, mask)
+    apost = train_aposteriori(net, data, params, dt, train, windows, window, lr_post, seed, kmax, mask,
+                              attractor_windows=attractor_windows, spin=spin, target=target)
+    calib = calibrate_gain(net, data, params, dt, kmax, mask, target,
+                           steps=min(calib_steps, nsnap - 1)) if calib_steps > 0 else []
     steps = min(eval_steps, nsnap - 1)   # LES step n pairs with snapshot n
-    _log(f"evaluation: {steps} LES steps with and without the closure")
-    with_cl = evaluate(net, data, params, dt, steps, kmax, mask)
+    _log(f"evaluation: {steps} LES steps paired with the DNS record ({long_factor * steps} in all) "
+         f"with and without the closure")
+    with_cl = evaluate(net, data, params, dt, steps, kmax, mask, long_steps=long_factor * steps)
     without = evaluate(None, data, params, dt, steps, kmax, mask)
     dns_s = data["dns_ms_per_step"] * 1e-3 * steps * stride
     res = {
         "config": {"fine_n": fine_n, "les_n": les_n, "kcut": kcut, "nu": nu, "dt_dns": dt_dns,
                    "dt_les": dt, "stride": stride, "spinup_time": spinup_time, "nsnap": nsnap,
-                   "train_snapshots": ntrain, "epochs": epochs, "windows": windows, "window": window,
+                   "train_snapshots": ntrain, "epochs": epochs, "windows": windows,
+                   "attractor_windows": attractor_windows, "spin": spin, "window": window,
+                   "calib_steps": calib_steps,
                    "eval_steps": steps, "channels": channels, "parameters": net.num_parameters(),
-                   "forcing_les": f"LinearForcing(A={A:.5f})", "tolerance": tol},
+                   "forcing_les": params.forcing.label, "tolerance": tol},
+        "dns_half_vs_half_rel_diff": floor.tolist(), "dns_half_vs_half_max": float(floor.max()),
         "dns_ms_per_step": data["dns_ms_per_step"], "stress_ms_per_snapshot": data["stress_ms"],
         "apriori": apriori, "aposteriori_loss_first_last": [apost[0], float(np.mean(apost[-10:]))],
-        "closure_gain": float(net.gain),
+        "closure_gain": float(net.gain.detach()), "gain_calibration": calib,
         "les": with_cl, "les_no_closure": without,
         "gpu_seconds": {"dns": dns_s, "les": with_cl["seconds"],
                         "speedup": dns_s / with_cl["seconds"]},
@@ -264,7 +368,8 @@ def run(fine_n=512, les_n=64, kcut=21.0, nu=1 / 2000, dt_dns=1e-3, stride=10, sp
         "fallback": "none" if with_cl["max_rel_err"] <= tol else "plain DNS (closure error above tolerance)",
         "wall_s": time.time() - wall0,
     }
-    _log(f"E(k) max relative error, k = 1..{kmax}: closure {with_cl['max_rel_err']:.3f}, "
+    _log(f"E(k) max relative error, k = 1..{kmax}: closure {with_cl['max_rel_err']:.3f} "
+         f"({with_cl.get('max_rel_err_long', float('nan')):.3f} over {with_cl.get('long_steps', steps)} steps), "
          f"no closure {without['max_rel_err']:.3f}; GPU seconds DNS {dns_s:.1f} vs LES "
          f"{with_cl['seconds']:.2f} ({res['gpu_seconds']['speedup']:.0f}x)")
     return res
@@ -277,12 +382,14 @@ def main(argv=None):
     ap.add_argument("--nsnap", type=int, default=None)
     ap.add_argument("--epochs", type=int, default=None)
     ap.add_argument("--windows", type=int, default=None)
+    ap.add_argument("--attractor-windows", dest="attractor_windows", type=int, default=None)
     a = ap.parse_args(argv)
     kw = {}
     if a.quick:
         kw = dict(fine_n=128, les_n=32, kcut=10.0, nu=1 / 500, dt_dns=2e-3, stride=4, spinup_time=4.0,
-                  nsnap=200, epochs=2, windows=10, window=10, eval_steps=200)
-    for k in ("nsnap", "epochs", "windows"):
+                  nsnap=200, epochs=2, windows=6, attractor_windows=4, spin=20, window=10,
+                  calib_steps=100, eval_steps=200)
+    for k in ("nsnap", "epochs", "windows", "attractor_windows"):
         if getattr(a, k) is not None:
             kw[k] = getattr(a, k)
     res = run(**kw)

@@ -135,8 +135,10 @@ def les_speed(torch, h3, fine_n=512, les_n=64, stride=10, steps=40):
     """GPU cost of the learned-closure LES against the plain DNS it replaces
     over the same simulated time: one 64^3 LES step (RK4, closure on every
     stage, dt = stride * DNS dt) against `stride` 512^3 DNS steps (nu =
-    1/2000, Lundgren). The closure's weights do not change its cost; its
-    accuracy comes from the full training experiment (hit3d.les_experiment)."""
+    1/2000, Lundgren), with the experiment's LES forcing (resolved energy
+    held) and spherical mask. The closure's weights do not change its cost;
+    its accuracy comes from the full training experiment
+    (hit3d.les_experiment)."""
     from hit3d import les
     from hit3d.closure import ClosureNet
 
@@ -165,11 +167,12 @@ def les_speed(torch, h3, fine_n=512, les_n=64, stride=10, steps=40):
     ul = les.filter_state(h3.random_field(1, fine, h3.config_spectrum), fine, coarse, les_n // 3)
     net = ClosureNet(coarse, channels=32, seed=0)
     rhs_c = h3.make_ns_rhs(coarse)
-    pc = h3.NSParams(nu=nu, forcing=h3.LinearForcing(A=0.15))
+    pc = h3.NSParams(nu=nu, forcing=les.HoldResolvedEnergy(h3.energy(ul, coarse), 1.0))
+    mask = les.sphere_mask(coarse, les_n // 3)
     lh = [ul]
 
     def les_run(n):
-        lh[0] = les.les_rollout(lh[0], rhs_c, pc, net, stride * 1e-3, n)
+        lh[0] = les.les_rollout(lh[0], rhs_c, pc, net, stride * 1e-3, n, mask=mask)
     les_ms = timed(les_run, steps)
     return {"dns_grid": fine_n, "les_grid": les_n, "dns_ms_per_step": round(dns_ms, 4),
             "les_ms_per_step": round(les_ms, 4), "dns_steps_per_les_step": stride,

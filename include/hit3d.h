@@ -250,6 +250,19 @@ int hit3d_velocity_products(const hit3d_grid* grid, const void* u, void* uu, voi
 int hit3d_stress_divergence(const hit3d_grid* grid, const void* in, void* out, int32_t adjoint,
                             void* stream);
 
+/* Forcing that holds the resolved energy of an LES (no reference
+   counterpart; the LES analogue of forcing.py's constant-energy forcing, with
+   the closure's transfer counted in the balance): out = du + A u, with
+   D = sum w Re(conj(u) du) / Ntot^2 (the rate of change of E that du causes),
+   2E = sum w |u|^2 / Ntot^2 and A = (-D - (E - energy)/tau) / (2E), so that
+   dE/dt = -(E - energy)/tau; tau <= 0 or infinite drops the relaxation term
+   (dE/dt = 0), A = 0 when 2E <= finfo(dtype).eps. Full-layout states [3];
+   coef (device, 3 doubles) receives [A, D, 2E]; FP64 fixed-order sums;
+   workspace of HIT3D_OP_REDUCE. out may alias du_hat or u_hat. */
+int hit3d_hold_energy(const hit3d_grid* grid, const void* u_hat, const void* du_hat, void* out,
+                      double energy, double tau, double* coef, void* work, size_t work_bytes,
+                      void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -150,8 +150,13 @@ def test_les_experiment_driver_end_to_end():
     from hit3d import les_experiment
     res = les_experiment.run(fine_n=64, les_n=16, kcut=5.0, nu=1 / 200, dt_dns=4e-3, stride=2,
                              spinup_time=0.2, nsnap=24, epochs=1, windows=2, window=3,
-                             eval_steps=20, channels=16)
+                             attractor_windows=2, spin=3, eval_steps=20, channels=16)
     assert res["config"]["eval_steps"] == 20 and len(res["les"]["E_les"]) == 5
+    assert res["les"]["long_steps"] == 60 and np.isfinite(res["les"]["max_rel_err_long"])
+    assert np.isfinite(res["dns_half_vs_half_max"])
+    assert res["config"]["forcing_les"].startswith("HoldResolvedEnergy")
+    assert len(res["gain_calibration"]) >= 1
+    assert min(abs(res["closure_gain"] - c["gain"]) for c in res["gain_calibration"]) <= 1e-6
     assert all(np.isfinite(res["les"]["rel_err"])) and all(np.isfinite(res["les_no_closure"]["rel_err"]))
     assert res["gpu_seconds"]["dns"] > 0 and res["gpu_seconds"]["les"] > 0
     assert isinstance(res["accepted"], bool)
@@ -211,3 +216,81 @@ def test_les_graph_replays_match_eager_steps():
         out = G.step().clone()
         assert torch.equal(out, u_eager), r
     assert float((u_eager * (1 - mask)).abs().max()) == 0.0   # stays inside the sphere
+
+
+def _hold_reference(u, du, g, energy, tau):
+    """float64 torch statement of hit3d_hold_energy."""
+    w = g.weights.double() / float(g.Nx * g.Ny * g.Nz) ** 2
+    u, du = u.to(torch.complex128), du.to(torch.complex128)
+    s1 = torch.sum(w * (u.real * du.real + u.imag * du.imag))
+    s2 = torch.sum(w * (u.real ** 2 + u.imag ** 2))
+    rate = -s1 - ((0.5 * s2 - energy) / tau if tau else 0.0)
+    A = rate / s2
+    return du + A * u, torch.stack([A, s1, s2])
+
+
+@pytest.mark.parametrize("dtype,tau", [(torch.float64, 0.5), (torch.float64, None), (torch.float32, 1.0)])
+def test_hold_energy_matches_formula_and_sets_the_energy_rate(dtype, tau):
+    g = h3.make_grid(16, 8, 16, dtype=dtype)
+    gen = torch.Generator().manual_seed(11)
+    cd = torch.complex128 if dtype == torch.float64 else torch.complex64
+    u = h3.dealias(crand((3, g.Nx, g.Ny, g.Nz // 2 + 1), gen, cd), g)
+    du = crand(u.shape, gen, cd)
+    out, coef = les._hold_raw(u, du, g, 0.3, tau)
+    ref, rc = _hold_reference(u, du, g, 0.3, tau)
+    tol = 1e-12 if dtype == torch.float64 else 2e-6
+    assert torch.allclose(coef, rc, rtol=tol, atol=0), (coef, rc)
+    assert float((out.to(torch.complex128) - ref).abs().max()) <= tol * float(ref.abs().max())
+    # the held rate: dE/dt of the output = -(E - energy)/tau (0 without relaxation)
+    w = g.weights.double() / float(g.Nx * g.Ny * g.Nz) ** 2
+    o, uu = out.to(torch.complex128), u.to(torch.complex128)
+    dE = float(torch.sum(w * (uu.real * o.real + uu.imag * o.imag)))
+    E = 0.5 * float(rc[2])
+    want = -(E - 0.3) / tau if tau else 0.0
+    scale = float(torch.sum(w * uu.abs() * o.abs()))
+    assert abs(dE - want) <= (1e-13 if dtype == torch.float64 else 1e-6) * scale
+
+
+def test_hold_energy_backward_matches_autograd_of_the_formula():
+    g = h3.make_grid(16, dtype=torch.float64)
+    gen = torch.Generator().manual_seed(12)
+    u = h3.dealias(crand((3, g.Nx, g.Ny, g.Nz // 2 + 1), gen), g).requires_grad_(True)
+    du = crand(u.shape, gen).requires_grad_(True)
+    W = crand(u.shape, gen)
+    for tau in (0.7, None):
+        gu, gdu = torch.autograd.grad(les.hold_energy(u, du, g, 0.2, tau), (u, du), W)
+        ru, rdu = torch.autograd.grad(_hold_reference(u, du, g, 0.2, tau)[0], (u, du), W)
+        assert float((gu - ru).abs().max()) <= 1e-12 * float(ru.abs().max())
+        assert float((gdu - rdu).abs().max()) <= 1e-12 * float(rdu.abs().max())
+
+
+def test_hold_energy_rejects_bad_arguments():
+    g = h3.make_grid(8, dtype=torch.float32)
+    u = torch.zeros((3, 8, 8, 5), dtype=torch.complex64, device="cuda")
+    with pytest.raises(ValueError):
+        les._hold_raw(u, u, g, -1.0, 1.0)
+
+
+def test_les_graph_with_hold_energy_matches_eager_and_holds_the_energy():
+    """The energy-holding forcing inside a captured LES step: replays equal
+    eager steps, and without relaxation the resolved energy stays at its
+    start value up to the RK4 error of a conserved quadratic."""
+    g = h3.make_grid(32, dtype=torch.float32)
+    rhs = h3.make_ns_rhs(g)
+    net = ClosureNet(g, channels=32, seed=3)
+    u0 = les.filter_state(h3.random_field(6, g, h3.config_spectrum), g, g, 10.0)
+    gg = les.velocity_gradient(u0, g).reshape(9, -1)
+    net.set_scales(1.0 / gg.std(dim=1), torch.full((6,), 0.05))
+    mask = les.sphere_mask(g, 10.0)
+    E0 = h3.energy(u0, g)
+    params = h3.NSParams(nu=0.01, forcing=les.HoldResolvedEnergy(E0, None))
+    G = les.LesStepGraph(u0, rhs, params, net, 4e-3, mask)
+    u_eager = u0.clone()
+    for r in range(4):
+        with torch.no_grad():
+            u_eager = les.les_step(u_eager, rhs, params, net, 4e-3, 0.0, mask)
+        h3.energy(u_eager, g)
+        torch.cuda.synchronize()
+        out = G.step().clone()
+        assert torch.equal(out, u_eager), r
+    assert abs(h3.energy(u_eager, g) / E0 - 1) < 1e-5
