@@ -310,8 +310,8 @@ def evaluate(net, data, params, dt, steps, kmax, mask, every=10, long_steps=0):
 
 
 def run(fine_n=512, les_n=64, kcut=21.0, nu=1 / 2000, dt_dns=1e-3, stride=10, spinup_time=10.0,
-        nsnap=2000, train_frac=0.8, epochs=12, lr=1e-3, windows=30, window=50, lr_post=3e-4,
-        attractor_windows=100, spin=150, calib_steps=2000, eval_steps=2000, long_factor=3, tol=0.05,
+        nsnap=4000, train_frac=0.8, epochs=6, lr=1e-3, windows=30, window=50, lr_post=3e-4,
+        attractor_windows=100, spin=150, calib_steps=2000, eval_steps=4000, long_factor=3, tol=0.05,
         channels=32, relax=1.0, seed=0):
     wall0 = time.time()
     data = generate(fine_n, les_n, kcut, nu, dt_dns, stride, spinup_time, nsnap, seed)
