@@ -181,13 +181,7 @@ size_t transform_bytes(const hit3d_grid& gd, int64_t ncomp) {
 
 // ---- FP64 finalize: partial sums -> totals and the forcing coefficient ----
 
-struct FinalizeArgs {
-  int kind;        // HIT3D_FORCING_*
-  double eps;      // band injection
-  double nu, energy, tau;
-  double tiny;     // finfo(dtype).eps
-  double inv_ntot2;
-};
+using h3::FinalizeArgs;
 
 __global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ partials, int nblocks,
                                                   FinalizeArgs fa, double* __restrict__ scalars) {
@@ -202,24 +196,7 @@ __global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ par
       for (int c = 0; c < 3; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    const double twoE = sh[0][0] * fa.inv_ntot2, twoO = sh[1][0] * fa.inv_ntot2,
-                 twoB = sh[2][0] * fa.inv_ntot2;
-    double coef = 0.0;
-    if (fa.kind == HIT3D_FORCING_BAND) {
-      coef = twoB > fa.tiny ? fa.eps / twoB : 0.0;
-    } else if (fa.kind == HIT3D_FORCING_LUNDGREN) {
-      if (twoE > fa.tiny) {
-        double rate = fa.nu * twoO;
-        if (fa.tau > 0.0 && isfinite(fa.tau)) rate -= (0.5 * twoE - fa.energy) / fa.tau;
-        coef = rate / twoE;
-      }
-    }
-    scalars[0] = coef;
-    scalars[1] = twoE;
-    scalars[2] = twoO;
-    scalars[3] = twoB;
-  }
+  if (threadIdx.x == 0) h3::finalize_write(fa, sh[0][0], sh[1][0], sh[2][0], scalars);
 }
 
 __global__ __launch_bounds__(256) void k_spectrum_final(const double* __restrict__ partials,
@@ -306,11 +283,11 @@ bool needs_sums(const hit3d_forcing* f) {
 // P1-P4 of one RHS evaluation on the compact stage input `us`.
 template <class T>
 int rhs_passes(const hit3d_grid& gd, const Geo& g, const cpx<T>* us, cpx<T>* bufA, cpx<T>* bufB,
-               hipStream_t s, PassTimer* tm = nullptr) {
+               hipStream_t s, PassTimer* tm = nullptr, const h3::FinalizeJob* job = nullptr) {
   const int64_t st_fs = int64_t(g.kx) * g.jp;
   const int64_t a_fs = int64_t(g.nx) * g.jp;
   const int64_t b_fs = int64_t(g.nx) * g.ny * g.kz;
-  H3_HIP(h3::launch_x_inverse_curl<T>(g.nx, us, st_fs, bufA, a_fs, g, s));
+  H3_HIP(h3::launch_x_inverse_curl<T>(g.nx, us, st_fs, bufA, a_fs, g, s, job));
   mark(tm, kP1);
   H3_HIP(h3::launch_y_inverse_curl<T>(g.ny, bufA, a_fs, bufB, b_fs, g, s));
   mark(tm, kP2);
@@ -447,23 +424,33 @@ int rk4_impl(const hit3d_grid& gd, cpx<T>* u_hat, double nu, double dt, int64_t 
   a.partials = sums ? b.partials : nullptr;
   a.acc = acc;
   a.u = u;
-  const FinalizeArgs fa = finalize_args(gd, f);
+  // The sums a stage kernel writes set the forcing coefficient of the next
+  // stage's P5; block 0 of the next stage's P1 reduces them (FinalizeJob),
+  // and only the last stage's sums need a launch of their own.
+  h3::FinalizeJob job{b.partials, 0, finalize_args(gd, f), b.scalars};
+  bool pending = false;
   for (int64_t n = 0; n < nsteps; ++n) {
     for (int stage = 1; stage <= 4; ++stage) {
       const cpx<T>* us_in = stage == 1 ? u : us;
-      int rc = rhs_passes<T>(gd, g, us_in, bufA, bufB, s, tm);
+      int rc = rhs_passes<T>(gd, g, us_in, bufA, bufB, s, tm, pending ? &job : nullptr);
       if (rc) return rc;
+      pending = false;
       a.us = us_in;
       a.out = stage == 4 ? u : us;
       int nb = 0;
       H3_HIP(h3::launch_x_forward_stage<T>(g.nx, stage, a, s, &nb));
       mark(tm, kP5);
       if (sums) {
-        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, b.partials, nb, fa, b.scalars);
-        H3_HIP(hipGetLastError());
+        job.nblocks = nb;
+        pending = true;
         mark(tm, kFinalize);
       }
     }
+  }
+  if (pending) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, b.partials, job.nblocks, job.fa,
+                       b.scalars);
+    H3_HIP(hipGetLastError());
   }
   if (diag) {
     int rc = reduce_state<T>(gd, g, u, f, b.partials, b.scalars, s);

@@ -67,10 +67,13 @@ template <int N, class T>
 __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(MinWaves<N, T>::value))) void k_x_inverse_curl(const cpx<T>* __restrict__ us,
                                                             int64_t us_fs,
                                                             cpx<T>* __restrict__ out,
-                                                            int64_t out_fs, Geo g) {
+                                                            int64_t out_fs, Geo g,
+                                                            FinalizeJob job) {
   using S = StridedShape<N>;
   using F = typename S::F;
   constexpr int R = S::R, Q = S::Q, TC = S::TC;
+  static_assert(S::THREADS % 64 == 0 && S::THREADS <= 1024, "finalize_in_block needs whole waves");
+  if (job.partials != nullptr && blockIdx.x == 0) finalize_in_block(job);   // uniform per block
   __shared__ cpx<T> lds[S::LDS];
   const int c = threadIdx.x % TC, q = threadIdx.x / TC;
   const int j = xcd_tile(blockIdx.x, gridDim.x) * TC + c;
@@ -527,11 +530,14 @@ hipError_t launch_strided(int N, int dir, const StridedArgs<T>& a, hipStream_t s
 
 template <class T>
 hipError_t launch_x_inverse_curl(int N, const cpx<T>* us, int64_t us_fs, cpx<T>* out,
-                                 int64_t out_fs, const Geo& g, hipStream_t s) {
+                                 int64_t out_fs, const Geo& g, hipStream_t s,
+                                 const FinalizeJob* job) {
+  FinalizeJob j{};
+  if (job) j = *job;
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(int64_t(g.ky) * g.kz, StridedShape<NN>::TC);
     hipLaunchKernelGGL((k_x_inverse_curl<NN, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, us, us_fs, out,
-                       out_fs, g);
+                       out_fs, g, j);
   });
   return hipGetLastError();
 }
@@ -602,7 +608,7 @@ hipError_t launch_x_forward_stage(int N, int mode, const StageArgs<T>& a, hipStr
 #define H3_INSTANTIATE(T)                                                                        \
   template hipError_t launch_strided<T>(int, int, const StridedArgs<T>&, hipStream_t);          \
   template hipError_t launch_x_inverse_curl<T>(int, const cpx<T>*, int64_t, cpx<T>*, int64_t,   \
-                                               const Geo&, hipStream_t);                         \
+                                               const Geo&, hipStream_t, const FinalizeJob*);     \
   template hipError_t launch_y_inverse_curl<T>(int, const cpx<T>*, int64_t, cpx<T>*, int64_t,   \
                                                const Geo&, hipStream_t);                         \
   template hipError_t launch_z_fused<T>(int, cpx<T>*, int64_t, int64_t, int, T, hipStream_t);   \

@@ -1,6 +1,7 @@
 // Kernel argument blocks and host launcher declarations shared by the HIP
 // translation units and the C ABI.
 #pragma once
+#include "../../include/hit3d.h"
 #include "common.hpp"
 
 namespace h3 {
@@ -87,11 +88,80 @@ struct StageArgs {
   Geo g;
 };
 
+// FP64 finalize: per-block partial sums [nblocks][3] of (w|u|^2, w k2|u|^2,
+// band w|u|^2) -> scalars [coef, 2E, 2 Omega, 2 E_band] (normalised by
+// 1/Ntot^2) and the forcing coefficient of the summed state.
+struct FinalizeArgs {
+  int kind;        // HIT3D_FORCING_*
+  double eps;      // band injection
+  double nu, energy, tau;
+  double tiny;     // finfo(dtype).eps
+  double inv_ntot2;
+};
+
+// coefficient rule of the forcings (forcing.py:109-163 band, forcing.py:5-20
+// constant energy) from the three totals
+__device__ __forceinline__ void finalize_write(const FinalizeArgs& fa, double sE, double sO,
+                                               double sB, double* __restrict__ scalars) {
+  const double twoE = sE * fa.inv_ntot2, twoO = sO * fa.inv_ntot2, twoB = sB * fa.inv_ntot2;
+  double coef = 0.0;
+  if (fa.kind == HIT3D_FORCING_BAND) {
+    coef = twoB > fa.tiny ? fa.eps / twoB : 0.0;
+  } else if (fa.kind == HIT3D_FORCING_LUNDGREN) {
+    if (twoE > fa.tiny) {
+      double rate = fa.nu * twoO;
+      if (fa.tau > 0.0 && isfinite(fa.tau)) rate -= (0.5 * twoE - fa.energy) / fa.tau;
+      coef = rate / twoE;
+    }
+  }
+  scalars[0] = coef;
+  scalars[1] = twoE;
+  scalars[2] = twoO;
+  scalars[3] = twoB;
+}
+
+// A finalize carried out by block 0 of the next pass (P1 of the next RK4
+// stage), which starts after the stage kernel that wrote the partials has
+// finished: no separate one-block launch between the passes.
+struct FinalizeJob {
+  const double* partials;  // null: nothing to do
+  int nblocks;
+  FinalizeArgs fa;
+  double* scalars;
+};
+
+// Whole-block fixed-order reduction (blockDim a multiple of 64, at most 1024):
+// Neumaier sums per thread over a strided set of blocks, a shuffle tree per
+// wave, then the waves in order.
+__device__ __forceinline__ void finalize_in_block(const FinalizeJob& j) {
+  __shared__ double wsum[3][16];
+  KSum acc[3];
+  for (int i = threadIdx.x; i < j.nblocks; i += blockDim.x)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[c].add(j.partials[3 * i + c]);
+  double v[3] = {acc[0].value(), acc[1].value(), acc[2].value()};
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] += __shfl_xor(v[c], off);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) wsum[c][threadIdx.x / 64] = v[c];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t[3] = {0.0, 0.0, 0.0};
+    for (int w = 0; w < int(blockDim.x / 64); ++w)
+      for (int c = 0; c < 3; ++c) t[c] += wsum[c][w];
+    finalize_write(j.fa, t[0], t[1], t[2], j.scalars);
+  }
+}
+
 template <class T>
 hipError_t launch_strided(int N, int dir, const StridedArgs<T>& a, hipStream_t s);
 template <class T>
 hipError_t launch_x_inverse_curl(int N, const cpx<T>* us, int64_t us_fs, cpx<T>* out,
-                                 int64_t out_fs, const Geo& g, hipStream_t s);
+                                 int64_t out_fs, const Geo& g, hipStream_t s,
+                                 const FinalizeJob* job = nullptr);
 template <class T>
 hipError_t launch_y_inverse_curl(int N, const cpx<T>* in, int64_t in_fs, cpx<T>* out,
                                  int64_t out_fs, const Geo& g, hipStream_t s);
