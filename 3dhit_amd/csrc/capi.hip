@@ -91,11 +91,16 @@ Geo make_geo(const hit3d_grid& gd, bool pruned) {
     g.kx = 2 * g.cx + 1;
     g.ky = 2 * g.cy + 1;
     g.kz = g.cz + 1;
-    g.jp = int(round_up(int64_t(g.ky) * g.kz, 16));
+    // rows on 128-byte lines when that pads by at most 5 %, else on 64-byte
+    // halves (86 -> 88 at N = 256, 171 -> 176 at 512, 342 -> 352 at 1024)
+    g.kzs = int(round_up(g.kz, 16));
+    if (20 * (g.kzs - g.kz) > g.kz) g.kzs = int(round_up(g.kz, 8));
+    g.jp = int(round_up(int64_t(g.ky) * g.kzs, 16));
   } else {
     g.kx = g.nx;
     g.ky = g.ny;
     g.kz = g.nzh;
+    g.kzs = g.kz;
     g.jp = g.ny * g.nzh;
   }
   g.px = (g.kx + 1) / 2;
@@ -146,8 +151,8 @@ StepBuffers carve_step(const hit3d_grid& gd, const Geo& g, bool pruned, bool rk4
   b.us = c.take(st);
   if (rk4) b.acc = c.take(st);
   b.bufA = c.take(size_t(5) * g.nx * g.jp * cb);
-  b.bufB = c.take(size_t(6) * g.nx * g.ny * g.kz * cb);
-  const int nb = std::max(h3::x_stage_blocks(g.nx, g.ky * g.kz), h3::kReduceBlocks);
+  b.bufB = c.take(size_t(6) * g.nx * g.ny * g.kzs * cb);
+  const int nb = std::max(h3::x_stage_blocks(g.nx, g.ky * g.kzs), h3::kReduceBlocks);
   b.partials = static_cast<double*>(c.take(size_t(3) * nb * sizeof(double)));
   b.scalars = static_cast<double*>(c.take(16 * sizeof(double)));
   *total = c.off;
@@ -286,23 +291,24 @@ int rhs_passes(const hit3d_grid& gd, const Geo& g, const cpx<T>* us, cpx<T>* buf
                hipStream_t s, PassTimer* tm = nullptr, const h3::FinalizeJob* job = nullptr) {
   const int64_t st_fs = int64_t(g.kx) * g.jp;
   const int64_t a_fs = int64_t(g.nx) * g.jp;
-  const int64_t b_fs = int64_t(g.nx) * g.ny * g.kz;
+  const int64_t b_fs = int64_t(g.nx) * g.ny * g.kzs;
   H3_HIP(h3::launch_x_inverse_curl<T>(g.nx, us, st_fs, bufA, a_fs, g, s, job));
   mark(tm, kP1);
   H3_HIP(h3::launch_y_inverse_curl<T>(g.ny, bufA, a_fs, bufB, b_fs, g, s));
   mark(tm, kP2);
   const double n = double(gd.nx) * double(gd.ny) * double(gd.nz);
-  H3_HIP(h3::launch_z_fused<T>(g.nz, bufB, b_fs, int64_t(g.nx) * g.ny, g.kz, T(1.0 / (n * n)), s));
+  H3_HIP(h3::launch_z_fused<T>(g.nz, bufB, b_fs, int64_t(g.nx) * g.ny, g.kz, g.kzs,
+                               T(1.0 / (n * n)), s));
   mark(tm, kP3);
   h3::StridedArgs<T> y{};
   y.in = bufB;
   y.out = bufA;
   y.in_fs = b_fs;
   y.out_fs = a_fs;
-  y.in_slab = int64_t(g.ny) * g.kz;
+  y.in_slab = int64_t(g.ny) * g.kzs;
   y.out_slab = g.jp;
-  y.B = g.kz;
-  y.J = g.nx * g.kz;
+  y.B = g.kzs;
+  y.J = g.nx * g.kzs;
   y.nf = 3;
   y.in_k = g.ny;
   y.in_p = (g.ny + 1) / 2;

@@ -323,9 +323,13 @@ __device__ __forceinline__ int xcd_tile_rev(int b, int nb) {
 struct Geo {
   int nx, ny, nz, nzh;  // grid points; nzh = nz/2 + 1
   int kx, ky, kz;       // retained modes per axis (compact extents)
+  int kzs;              // row stride of the kz rows in the compact, x-pass and pencil arrays:
+                        // kz rounded up to 16 (or 8, see make_geo) in the pruned layout, so
+                        // rows start on 128- (64-) byte boundaries for complex64; the padding
+                        // slots hold zeros. kz when dense
   int px, py;           // positive-frequency counts (kx+1)/2, (ky+1)/2
   int cx, cy, cz;       // 2/3-rule cut N_i // 3
-  int jp;               // row stride of x-pass arrays (>= ky*kz)
+  int jp;               // row stride of x-pass arrays (ky*kzs rounded up to 16)
   double k0x, k0y, k0z; // 2 pi / L_i
 };
 

@@ -37,17 +37,17 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_gather(const cpx<T>* __restrict__ full, int64_t full_fs,
                                                    cpx<T>* __restrict__ comp, int64_t comp_fs,
                                                    int nf, Geo g) {
-  const int64_t J = int64_t(g.ky) * g.kz;
-  const int64_t n = int64_t(g.kx) * J;
+  // every compact slot, the padding (iz >= kz) included: padding gets zeros
+  const int64_t n = int64_t(g.kx) * g.jp;
   const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (idx >= n) return;
-  const int ix = int(idx / J);
-  const int j = int(idx - int64_t(ix) * J);
-  const int iy = j / g.kz, iz = j - iy * g.kz;
+  const int ix = int(idx / g.jp);
+  const int j = int(idx - int64_t(ix) * g.jp);
+  const int iy = j / g.kzs, iz = j - iy * g.kzs;
+  const bool mode = iy < g.ky && iz < g.kz;   // not a padding slot
   const int x = row_of_compact(ix, g.nx, g.kx, g.px), y = row_of_compact(iy, g.ny, g.ky, g.py);
-  const int64_t src = (int64_t(x) * g.ny + y) * g.nzh + iz;
-  const int64_t dst = int64_t(ix) * g.jp + j;
-  for (int f = 0; f < nf; ++f) comp[f * comp_fs + dst] = full[f * full_fs + src];
+  const int64_t src = mode ? (int64_t(x) * g.ny + y) * g.nzh + iz : 0;   // in range either way
+  for (int f = 0; f < nf; ++f) comp[f * comp_fs + idx] = mode ? full[f * full_fs + src] : czero<T>();
 }
 
 template <class T>
@@ -62,7 +62,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const cpx<T>* __restrict__ c
   const int y = int(xy % g.ny), x = int(xy / g.ny);
   const int ix = compact_of_row(x, g.nx, g.kx, g.px), iy = compact_of_row(y, g.ny, g.ky, g.py);
   const bool kept = ix >= 0 && iy >= 0 && iz < g.kz;
-  const int64_t src = int64_t(ix) * g.jp + int64_t(iy) * g.kz + iz;
+  const int64_t src = int64_t(ix) * g.jp + int64_t(iy) * g.kzs + iz;
   for (int f = 0; f < nf; ++f) full[f * full_fs + idx] = kept ? comp[f * comp_fs + src] : czero<T>();
 }
 
@@ -80,7 +80,7 @@ __global__ __launch_bounds__(kBlock) void k_quad_partials(const cpx<T>* __restri
     const int ix = int(idx / J);
     const int j = int(idx - int64_t(ix) * J);
     const int iy = j / g.kz, iz = j - iy * g.kz;
-    const int64_t o = int64_t(ix) * g.jp + j;
+    const int64_t o = int64_t(ix) * g.jp + int64_t(iy) * g.kzs + iz;
     double e = 0.0;
     for (int f = 0; f < 3; ++f) {
       const cpx<T> v = st[f * fs + o];
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kBlock) void k_spectrum_partials(const cpx<T>* __re
     const int ix = int(l / g.ky), iy = int(l - int64_t(ix) * g.ky);
     const T kxv = wavenumber<T>(g.k0x, freq_of(ix, g.kx, g.px));
     const T kyv = wavenumber<T>(g.k0y, freq_of(iy, g.ky, g.py));
-    const int64_t base = int64_t(ix) * g.jp + int64_t(iy) * g.kz;
+    const int64_t base = int64_t(ix) * g.jp + int64_t(iy) * g.kzs;
     for (int z0 = 0; z0 < g.kz; z0 += 64) {
       const int iz = z0 + lane;
       double v = 0.0;
@@ -475,7 +475,7 @@ __global__ __launch_bounds__(kBlock) void k_axpby(cpx<T>* out, T a, const cpx<T>
 template <class T>
 hipError_t launch_gather(const cpx<T>* full, int64_t full_fs, cpx<T>* comp, int64_t comp_fs,
                          int nf, const Geo& g, hipStream_t s) {
-  const int64_t n = int64_t(g.kx) * g.ky * g.kz;
+  const int64_t n = int64_t(g.kx) * g.jp;
   hipLaunchKernelGGL((k_gather<T>), dim3(nblocks_for(n)), dim3(kBlock), 0, s, full, full_fs, comp,
                      comp_fs, nf, g);
   return hipGetLastError();

@@ -77,9 +77,9 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
   __shared__ cpx<T> lds[S::LDS];
   const int c = threadIdx.x % TC, q = threadIdx.x / TC;
   const int j = xcd_tile(blockIdx.x, gridDim.x) * TC + c;
-  const bool active = j < g.ky * g.kz;
+  const bool active = j < g.ky * g.kzs;   // padding columns (iz >= kz) carry zeros
   const int jj = active ? j : 0;
-  const int iy = jj / g.kz, iz = jj - iy * g.kz;
+  const int iy = jj / g.kzs, iz = jj - iy * g.kzs;
   const T kyv = wavenumber<T>(g.k0y, freq_of(iy, g.ky, g.py));
   const T kzv = wavenumber<T>(g.k0z, iz);
   SlotStrided<TC, T> L{lds + c};
@@ -143,9 +143,9 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
   __shared__ cpx<T> lds[S::LDS];
   const int c = threadIdx.x % TC, q = threadIdx.x / TC;
   const int j = xcd_tile(blockIdx.x, gridDim.x) * TC + c;
-  const bool active = j < g.nx * g.kz;
+  const bool active = j < g.nx * g.kzs;   // padding columns (iz >= kz) carry zeros
   const int jj = active ? j : 0;
-  const int x = jj / g.kz, iz = jj - x * g.kz;
+  const int x = jj / g.kzs, iz = jj - x * g.kzs;
   const T kzv = wavenumber<T>(g.k0z, iz);
   SlotStrided<TC, T> L{lds + c};
   cpx<T> tw[R];
@@ -158,17 +158,17 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
     kyv[r] = wavenumber<T>(g.k0y, irow[r] >= 0 ? freq_of(irow[r], g.ky, g.py) : 0);
   }
   const cpx<T>* src = in + int64_t(x) * g.jp + iz;
-  cpx<T>* dst0 = out + int64_t(x) * g.ny * g.kz + iz;
+  cpx<T>* dst0 = out + int64_t(x) * g.ny * g.kzs + iz;
   auto load = [&](int f, cpx<T>* v) {
     const cpx<T>* p = src + f * in_fs;
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = irow[r] >= 0 ? p[irow[r] * g.kz] : czero<T>();
+    for (int r = 0; r < R; ++r) v[r] = irow[r] >= 0 ? p[irow[r] * g.kzs] : czero<T>();
   };
   auto store = [&](int f, const cpx<T>* v) {
     if (!active) return;
     cpx<T>* p = dst0 + f * out_fs;
 #pragma unroll
-    for (int s = 0; s < R; ++s) p[F::out_index(q, s) * g.kz] = v[s];
+    for (int s = 0; s < R; ++s) p[F::out_index(q, s) * g.kzs] = v[s];
   };
   // each field's loads are issued one transform ahead of its use (pf holds
   // the prefetched field), so the barriers inside run() hide their latency
@@ -232,11 +232,13 @@ __device__ __forceinline__ void split_pair(cpx<T> ak, cpx<T> bk, cpx<T>& X, cpx<
 // ---------------------------------------------------------------------------
 // P3: per z pencil, c2r of (u_j + i w_j) for j = 0..2, u x w scaled by
 // `scale` (= 1/Ntot^2, a power of two), then r2c; output (nl0, nl1, nl2)
-// overwrites fields 0..2 of the same pencils.
+// overwrites fields 0..2 of the same pencils. Pencils are kzs apart; their
+// kz modes are followed by kzs - kz padding slots, written as zeros so that
+// every line of the output is written whole.
 // ---------------------------------------------------------------------------
 template <int N, class T>
 __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_fused(cpx<T>* buf, int64_t fs, int64_t npencils,
-                                                     int kz, T scale) {
+                                                     int kz, int kzs, T scale) {
   using B = PencilBlock<N>;
   using F = typename B::F;
   static_assert(B::THREADS == 64, "WaveSync exchanges need one-wave blocks");
@@ -245,7 +247,7 @@ __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_fused(cpx<T>* buf
   const int q = threadIdx.x % Q, p = threadIdx.x / Q;
   const int64_t pp = int64_t(xcd_tile_rev(blockIdx.x, gridDim.x)) * PW + p;
   const bool active = pp < npencils;
-  const int64_t base = (active ? pp : 0) * kz;
+  const int64_t base = (active ? pp : 0) * kzs;
   cpx<T>* lp = lds + p * F::kSlots;
   SlotPencil<T> L{lp};
   const int kzl = active ? kz : 0;  // inactive pencils load nothing
@@ -278,15 +280,16 @@ __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_fused(cpx<T>* buf
       const int k = F::out_index(q, s);
       if (k < kz) buf[2 * fs + base + k] = Bv[s];
     }
+    for (int k = kz + q; k < kzs; k += Q) buf[2 * fs + base + k] = czero<T>();
   }
   WaveSync::sync();
 #pragma unroll
   for (int s = 0; s < R; ++s) lp[F::out_index(q, s)] = A[s];
   WaveSync::sync();
   if (active) {
-    for (int k = q; k < kz; k += Q) {
-      cpx<T> X, Y;
-      split_pair(lp[k], lp[(N - k) & (N - 1)], X, Y);
+    for (int k = q; k < kzs; k += Q) {
+      cpx<T> X = czero<T>(), Y = czero<T>();
+      if (k < kz) split_pair(lp[k], lp[(N - k) & (N - 1)], X, Y);
       buf[0 * fs + base + k] = X;
       buf[1 * fs + base + k] = Y;
     }
@@ -415,9 +418,9 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
   const Geo& g = a.g;
   const int c = threadIdx.x % TC, q = threadIdx.x / TC;
   const int j = xcd_tile(blockIdx.x, gridDim.x) * TC + c;
-  const bool active = j < g.ky * g.kz;
+  const bool active = j < g.ky * g.kzs;   // padding columns: in_box is false, zeros stay zeros
   const int jj = active ? j : 0;
-  const int iy = jj / g.kz, iz = jj - iy * g.kz;
+  const int iy = jj / g.kzs, iz = jj - iy * g.kzs;
   const int ny = freq_of(iy, g.ky, g.py);
   const T kyv = wavenumber<T>(g.k0y, ny);
   const T kzv = wavenumber<T>(g.k0z, iz);
@@ -535,7 +538,7 @@ hipError_t launch_x_inverse_curl(int N, const cpx<T>* us, int64_t us_fs, cpx<T>*
   FinalizeJob j{};
   if (job) j = *job;
   H3_DISPATCH_N(N, {
-    const unsigned nb = ceil_div(int64_t(g.ky) * g.kz, StridedShape<NN>::TC);
+    const unsigned nb = ceil_div(int64_t(g.ky) * g.kzs, StridedShape<NN>::TC);
     hipLaunchKernelGGL((k_x_inverse_curl<NN, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, us, us_fs, out,
                        out_fs, g, j);
   });
@@ -546,7 +549,7 @@ template <class T>
 hipError_t launch_y_inverse_curl(int N, const cpx<T>* in, int64_t in_fs, cpx<T>* out,
                                  int64_t out_fs, const Geo& g, hipStream_t s) {
   H3_DISPATCH_N(N, {
-    const unsigned nb = ceil_div(int64_t(g.nx) * g.kz, StridedShape<NN>::TC);
+    const unsigned nb = ceil_div(int64_t(g.nx) * g.kzs, StridedShape<NN>::TC);
     hipLaunchKernelGGL((k_y_inverse_curl<NN, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, in, in_fs, out,
                        out_fs, g);
   });
@@ -554,12 +557,12 @@ hipError_t launch_y_inverse_curl(int N, const cpx<T>* in, int64_t in_fs, cpx<T>*
 }
 
 template <class T>
-hipError_t launch_z_fused(int N, cpx<T>* buf, int64_t fs, int64_t npencils, int kz, T scale,
-                          hipStream_t s) {
+hipError_t launch_z_fused(int N, cpx<T>* buf, int64_t fs, int64_t npencils, int kz, int kzs,
+                          T scale, hipStream_t s) {
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(npencils, PencilBlock<NN>::PW);
     hipLaunchKernelGGL((k_z_fused<NN, T>), dim3(nb), dim3(PencilBlock<NN>::THREADS), 0, s, buf, fs, npencils, kz,
-                       scale);
+                       kzs, scale);
   });
   return hipGetLastError();
 }
@@ -589,7 +592,7 @@ hipError_t launch_z_r2c(int N, const T* in, int64_t in_fs, cpx<T>* out, int64_t 
 template <class T>
 hipError_t launch_x_forward_stage(int N, int mode, const StageArgs<T>& a, hipStream_t s,
                                   int* nblocks_out) {
-  const int64_t J = int64_t(a.g.ky) * a.g.kz;
+  const int64_t J = int64_t(a.g.ky) * a.g.kzs;
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(J, StridedShape<NN>::TC);
     if (nblocks_out) *nblocks_out = int(nb);
@@ -611,7 +614,7 @@ hipError_t launch_x_forward_stage(int N, int mode, const StageArgs<T>& a, hipStr
                                                const Geo&, hipStream_t, const FinalizeJob*);     \
   template hipError_t launch_y_inverse_curl<T>(int, const cpx<T>*, int64_t, cpx<T>*, int64_t,   \
                                                const Geo&, hipStream_t);                         \
-  template hipError_t launch_z_fused<T>(int, cpx<T>*, int64_t, int64_t, int, T, hipStream_t);   \
+  template hipError_t launch_z_fused<T>(int, cpx<T>*, int64_t, int64_t, int, int, T, hipStream_t); \
   template hipError_t launch_z_c2r<T>(int, const cpx<T>*, int64_t, T*, int64_t, int, int64_t,   \
                                       int, T, hipStream_t);                                      \
   template hipError_t launch_z_r2c<T>(int, const T*, int64_t, cpx<T>*, int64_t, int, int64_t,   \

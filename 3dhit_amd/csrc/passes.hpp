@@ -166,8 +166,8 @@ template <class T>
 hipError_t launch_y_inverse_curl(int N, const cpx<T>* in, int64_t in_fs, cpx<T>* out,
                                  int64_t out_fs, const Geo& g, hipStream_t s);
 template <class T>
-hipError_t launch_z_fused(int N, cpx<T>* buf, int64_t fs, int64_t npencils, int kz, T scale,
-                          hipStream_t s);
+hipError_t launch_z_fused(int N, cpx<T>* buf, int64_t fs, int64_t npencils, int kz, int kzs,
+                          T scale, hipStream_t s);
 template <class T>
 hipError_t launch_z_c2r(int N, const cpx<T>* in, int64_t in_fs, T* out, int64_t out_fs, int nf,
                         int64_t npencils, int kz, T scale, hipStream_t s);
