@@ -148,6 +148,36 @@ def test_rhs_anisotropic(shape):
     assert rel(h3.make_ns_rhs(g)(dev(uh, g), h3.NSParams(nu=0.01), 0.0), ref) < 1e-12
 
 
+# The fused pruned passes at the large transform sizes (256-, 512- and
+# 1024-point kernels, padded kz rows) along each axis in turn, on grids small
+# enough for the oracle.
+LARGE_AXIS = [(256, 8, 8), (8, 256, 8), (8, 8, 256), (512, 8, 4), (1024, 8, 4), (4, 1024, 8),
+              (8, 4, 1024)]
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 5e-6)])
+@pytest.mark.parametrize("shape", LARGE_AXIS)
+def test_rhs_large_axis_pruned(shape, dtype, tol):
+    og, g = _grids(shape, dtype)
+    uh = O.project(random_spectral(12, og), og)
+    params = (O.NSParams(nu=0.01, forcing=O.LundgrenForcing(nu=0.01, energy=0.5, tau=0.3)),
+              h3.NSParams(nu=0.01, forcing=h3.LundgrenForcing(nu=0.01, energy=0.5, tau=0.3)))
+    ref = O.make_ns_rhs(og)(uh, params[0], 0.0)
+    assert rel(h3.make_ns_rhs(g)(dev(uh, g), params[1], 0.0), ref) < tol
+
+
+@pytest.mark.parametrize("shape", [(1024, 8, 4), (8, 8, 256)])
+def test_rollout_large_axis_pruned(shape):
+    """RK4 stage kernels and the per-stage FP64 coefficient at large N."""
+    og, g = _grids(shape, np.float64)
+    uh = O.project(random_spectral(13, og), og)
+    op = O.NSParams(nu=0.01, forcing=O.LundgrenForcing(nu=0.01, energy=0.5))
+    hp = h3.NSParams(nu=0.01, forcing=h3.LundgrenForcing(nu=0.01, energy=0.5))
+    ref = O.rollout(O.make_ns_rhs(og), uh, op, 1e-3, 3)
+    got = h3.rollout(h3.make_ns_rhs(g), dev(uh, g), hp, 1e-3, 3)
+    assert rel(got, ref) < 1e-12
+
+
 def test_beltrami_rhs_is_pure_viscous_decay():
     """reference tests/test_navierstokes.py:10-19 on the GPU (float64, dense path)."""
     og, g = _grids((32, 32, 32), np.float64)
