@@ -138,6 +138,28 @@ class ClosureNet:
         # subgrid transfer dissipative
         self.gain = torch.ones(1, device=dev, requires_grad=True)
         self.clip_backscatter = True
+        # spectral shape of the closure force, exp(tilt (|k|/kc - 1/2)) per
+        # mode, kc = N/3; a device tensor updated in place, so captured LES
+        # steps see the current tilt
+        self.tilt = 0.0
+        self._shape = None
+
+    def spectral_shape(self):
+        """exp(tilt (|k|/kc - 1/2)) on the net's grid (real, [Nx][Ny][Nz/2+1])."""
+        if self._shape is None:
+            self._shape = torch.empty(self.grid.k2.shape, dtype=self.grid.dtype, device=self.grid.device)
+            self._fill_shape()
+        return self._shape
+
+    def _fill_shape(self):
+        kc = float(min(self.grid.Nx, self.grid.Ny, self.grid.Nz) // 3)
+        with torch.no_grad():
+            self._shape.copy_(torch.exp(self.tilt * (torch.sqrt(self.grid.k2) / kc - 0.5)))
+
+    def set_tilt(self, tilt):
+        self.tilt = float(tilt)
+        if self._shape is not None:
+            self._fill_shape()
 
     def set_scales(self, in_scale, out_scale):
         self.in_scale = in_scale.float().to(self.in_scale.device).contiguous()
@@ -204,7 +226,7 @@ class ClosureNet:
     def state(self):
         return {"layers": [L.state() for L in self.layers], "in_scale": self.in_scale.clone(),
                 "out_scale": self.out_scale.clone(), "gain": self.gain.detach().clone(),
-                "clip_backscatter": self.clip_backscatter}
+                "clip_backscatter": self.clip_backscatter, "tilt": self.tilt}
 
     def load(self, st):
         for L, s in zip(self.layers, st["layers"]):
@@ -213,6 +235,7 @@ class ClosureNet:
         with torch.no_grad():
             self.gain.copy_(st["gain"])
         self.clip_backscatter = st["clip_backscatter"]
+        self.set_tilt(st.get("tilt", 0.0))
 
 
 class _NetFn(torch.autograd.Function):

@@ -218,15 +218,16 @@ def clip_backscatter(tau, g):
 
 
 def closure_force(u, grid, net):
-    """gain * P M [-i k_j tau_hat_ij], tau = net(grad u), with backscatter
-    clipping when the net asks for it (differentiable)."""
+    """gain * S(k) * P M [-i k_j tau_hat_ij], tau = net(grad u), with
+    backscatter clipping when the net asks for it and S the net's spectral
+    shape exp(tilt (|k|/kc - 1/2)) (differentiable)."""
     if grid.dtype != torch.float32:
         raise ValueError("the closure runs on float32 LES grids")
     g = velocity_gradient(u, grid)
     V = grid.Nx * grid.Ny * grid.Nz
     tau = apply_net(net, g.reshape(9, V)).reshape(6, grid.Nx, grid.Ny, grid.Nz)  # clipping inside
     f = _DealiasProject.apply(_StressDiv.apply(_ToSpectral.apply(tau, grid), grid), grid)
-    return net.gain * f
+    return net.gain * (net.spectral_shape() * f)
 
 
 def sphere_mask(grid, kcut):

@@ -16,9 +16,10 @@
    relative-L2 trajectory term), then `attractor_windows` statistical
    windows that start after `spin` gradient-free LES steps, on the LES's
    own attractor, and match the window-averaged spectrum to the filtered
-   DNS's mean spectrum. Last, the closure gain is calibrated by a few
-   secant steps so that the upper shells' mean energy over a `calib_steps`
-   LES matches the training snapshots' mean. The LES evolves exactly the
+   DNS's mean spectrum. Last, the closure force's gain and spectral tilt
+   (g exp(tilt (|k|/kc - 1/2))) are calibrated by two Gauss-Newton steps so
+   that the mean spectrum of a `calib_steps` LES matches the training
+   snapshots' mean on shells 5..kcut. The LES evolves exactly the
    filter's modes (|k| <= kcut, a sphere inside the solver's 2/3 cube).
 4. Evaluation: an LES from the first snapshot, with and without the
    closure; the time-averaged shell spectrum E(k), k = 1..kcut, over the
@@ -228,49 +229,59 @@ def train_aposteriori(net, data, params, dt, train, windows, window, lr, seed, k
     return hist
 
 
-def calibrate_gain(net, data, params, dt, kmax, mask, target, steps=2000, iters=4, every=10,
-                   slope=-1.3):
-    """Secant iterations on the closure gain g so that the LES's mean
-    spectrum over the upper shells (k >= kmax/2) matches the training
-    snapshots' mean: f(g) = mean_k log(<E_les(k)>/target(k)) over those
-    shells, <.> over the last three quarters of a `steps`-step LES from the
-    first snapshot. The upper shells' energy falls as the gain rises (the
-    closure drains them); `slope` (d f / d log g) seeds the first step and
-    every step is clamped to a factor of 1.2. Uses training data only."""
+def calibrate_closure(net, data, params, dt, kmax, mask, target, steps=2000, iters=2, every=10,
+                      kfit=5):
+    """Gauss-Newton on (log gain, tilt) of the closure force g exp(tilt
+    (|k|/kc - 1/2)) so that the LES's mean spectrum matches the training
+    snapshots' mean on shells kfit..kmax: residual r_k = log(<E_les(k)>/
+    target(k)), <.> over the last three quarters of a `steps`-step LES from
+    the first snapshot. The gain sets the overall drain, the tilt how it is
+    shared between the middle and the top shells. Jacobian by forward
+    differences (0.1 in log gain, 0.3 in tilt), steps clamped to 0.2 and
+    0.6; the best evaluated point is kept. Uses training data only."""
     coarse = data["coarse"]
     rhs = make_ns_rhs(coarse)
-    lo = kmax // 2
-    tgt = target.double().cpu().numpy()
+    kfit = max(1, min(kfit, kmax))
+    tgt = target.double().cpu().numpy()[kfit - 1:]
 
-    def f(g):
+    def resid(x):
         with torch.no_grad():
-            net.gain.fill_(g)
+            net.gain.fill_(math.exp(x[0]))
+        net.set_tilt(x[1])
         spectra = []
         les_rollout(data["ubar"][0].clone(), rhs, params, net, dt, steps, mask=mask, every=every,
                     observe=lambda n, u: spectra.append(_spectrum(u, coarse, kmax)))
-        E = np.mean(spectra[len(spectra) // 4:], axis=0)
-        return float(np.mean(np.log(E[lo - 1:] / tgt[lo - 1:])))
+        E = np.mean(spectra[len(spectra) // 4:], axis=0)[kfit - 1:]
+        return np.log(E / tgt)
 
-    g0 = float(net.gain.detach())
-    f0 = f(g0)
-    hist = [(g0, f0)]
-    _log(f"  gain calibration: gain {g0:.4f}, upper-shell mean log error {f0:+.4f}")
-    g1 = g0 * math.exp(min(max(-f0 / slope, -math.log(1.2)), math.log(1.2)))
-    for _ in range(iters if abs(f0) >= 2e-3 else 0):
-        f1 = f(g1)
-        hist.append((g1, f1))
-        _log(f"  gain calibration: gain {g1:.4f}, upper-shell mean log error {f1:+.4f}")
-        if abs(f1) < 2e-3 or f1 == f0:
-            break
-        d = (f1 - f0) / (math.log(g1) - math.log(g0))
-        if d >= 0:                        # noise: fall back to the seed slope
-            d = slope
-        step = min(max(-f1 / d, -math.log(1.2)), math.log(1.2))
-        g0, f0, g1 = g1, f1, g1 * math.exp(step)
-    best = min(hist, key=lambda gf: abs(gf[1]))
+    hist = []
+
+    def record(x, r):
+        hist.append({"gain": math.exp(x[0]), "tilt": float(x[1]), "rms_log_error": float(np.sqrt(np.mean(r * r))),
+                     "max_log_error": float(np.abs(r).max())})
+        _log(f"  closure calibration: gain {math.exp(x[0]):.4f}, tilt {x[1]:+.3f}, "
+             f"rms log error {hist[-1]['rms_log_error']:.4f}, max {hist[-1]['max_log_error']:.4f}")
+
+    x = np.array([math.log(float(net.gain.detach())), net.tilt])
+    r = resid(x)
+    record(x, r)
+    for _ in range(iters):
+        J = np.empty((r.size, 2))
+        for j, d in enumerate((0.1, 0.3)):
+            xp = x.copy()
+            xp[j] += d
+            rp = resid(xp)
+            record(xp, rp)
+            J[:, j] = (rp - r) / d
+        dx = np.clip(-np.linalg.lstsq(J, r, rcond=None)[0], [-0.2, -0.6], [0.2, 0.6])
+        x = x + dx
+        r = resid(x)
+        record(x, r)
+    best = min(hist, key=lambda h: h["rms_log_error"])
     with torch.no_grad():
-        net.gain.fill_(best[0])
-    return [{"gain": g, "upper_shell_log_error": e} for g, e in hist]
+        net.gain.fill_(best["gain"])
+    net.set_tilt(best["tilt"])
+    return hist
 
 
 def evaluate(net, data, params, dt, steps, kmax, mask, every=10, long_steps=0):
@@ -347,8 +358,8 @@ def run(fine_n=512, les_n=64, kcut=21.0, nu=1 / 2000, dt_dns=1e-3, stride=10, sp
     mask = sphere_mask(coarse, kcut)
     apost = train_aposteriori(net, data, params, dt, train, windows, window, lr_post, seed, kmax, mask,
                               attractor_windows=attractor_windows, spin=spin, target=target)
-    calib = calibrate_gain(net, data, params, dt, kmax, mask, target,
-                           steps=min(calib_steps, nsnap - 1)) if calib_steps > 0 else []
+    calib = calibrate_closure(net, data, params, dt, kmax, mask, target,
+                              steps=min(calib_steps, nsnap - 1)) if calib_steps > 0 else []
     steps = min(eval_steps, nsnap - 1)   # LES step n pairs with snapshot n
     _log(f"evaluation: {steps} LES steps paired with the DNS record ({long_factor * steps} in all) "
          f"with and without the closure")
@@ -366,7 +377,7 @@ def run(fine_n=512, les_n=64, kcut=21.0, nu=1 / 2000, dt_dns=1e-3, stride=10, sp
         "dns_half_vs_half_rel_diff": floor.tolist(), "dns_half_vs_half_max": float(floor.max()),
         "dns_ms_per_step": data["dns_ms_per_step"], "stress_ms_per_snapshot": data["stress_ms"],
         "apriori": apriori, "aposteriori_loss_first_last": [apost[0], float(np.mean(apost[-10:]))],
-        "closure_gain": float(net.gain.detach()), "gain_calibration": calib,
+        "closure_gain": float(net.gain.detach()), "closure_tilt": net.tilt, "closure_calibration": calib,
         "les": with_cl, "les_no_closure": without,
         "gpu_seconds": {"dns": dns_s, "les": with_cl["seconds"],
                         "speedup": dns_s / with_cl["seconds"]},

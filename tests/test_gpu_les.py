@@ -155,8 +155,9 @@ def test_les_experiment_driver_end_to_end():
     assert res["les"]["long_steps"] == 60 and np.isfinite(res["les"]["max_rel_err_long"])
     assert np.isfinite(res["dns_half_vs_half_max"])
     assert res["config"]["forcing_les"].startswith("HoldResolvedEnergy")
-    assert len(res["gain_calibration"]) >= 1
-    assert min(abs(res["closure_gain"] - c["gain"]) for c in res["gain_calibration"]) <= 1e-6
+    assert len(res["closure_calibration"]) == 7          # start + 2 x (2 differences + step)
+    best = min(res["closure_calibration"], key=lambda c: c["rms_log_error"])
+    assert abs(res["closure_gain"] - best["gain"]) <= 1e-6 and res["closure_tilt"] == best["tilt"]
     assert all(np.isfinite(res["les"]["rel_err"])) and all(np.isfinite(res["les_no_closure"]["rel_err"]))
     assert res["gpu_seconds"]["dns"] > 0 and res["gpu_seconds"]["les"] > 0
     assert isinstance(res["accepted"], bool)
