@@ -229,16 +229,45 @@ def train_aposteriori(net, data, params, dt, train, windows, window, lr, seed, k
     return hist
 
 
+def gauss_newton(resid, x0, deltas, clamp, iters, on_eval=None):
+    """Gauss-Newton on a vector residual with a forward-difference Jacobian:
+    per iteration one evaluation per coordinate (x + deltas[j] e_j), then the
+    least-squares step, clamped to +-clamp. Every evaluated point is kept;
+    returns (best x by rms residual, [(x, r), ...] in evaluation order)."""
+    x = np.asarray(x0, dtype=np.float64).copy()
+    pts = []
+
+    def ev(z):
+        r = np.asarray(resid(z), dtype=np.float64)
+        pts.append((z.copy(), r))
+        if on_eval is not None:
+            on_eval(z, r)
+        return r
+
+    r = ev(x)
+    for _ in range(iters):
+        J = np.empty((r.size, x.size))
+        for j, d in enumerate(deltas):
+            xp = x.copy()
+            xp[j] += d
+            J[:, j] = (ev(xp) - r) / d
+        dx = np.clip(-np.linalg.lstsq(J, r, rcond=None)[0], -np.asarray(clamp), np.asarray(clamp))
+        x = x + dx
+        r = ev(x)
+    best = min(pts, key=lambda p: float(np.sqrt(np.mean(p[1] ** 2))))
+    return best[0], pts
+
+
 def calibrate_closure(net, data, params, dt, kmax, mask, target, steps=2000, iters=2, every=10,
                       kfit=5):
-    """Gauss-Newton on (log gain, tilt) of the closure force g exp(tilt
-    (|k|/kc - 1/2)) so that the LES's mean spectrum matches the training
-    snapshots' mean on shells kfit..kmax: residual r_k = log(<E_les(k)>/
-    target(k)), <.> over the last three quarters of a `steps`-step LES from
-    the first snapshot. The gain sets the overall drain, the tilt how it is
-    shared between the middle and the top shells. Jacobian by forward
-    differences (0.1 in log gain, 0.3 in tilt), steps clamped to 0.2 and
-    0.6; the best evaluated point is kept. Uses training data only."""
+    """Gauss-Newton (`gauss_newton`) on (log gain, tilt) of the closure force
+    g exp(tilt (|k|/kc - 1/2)) so that the LES's mean spectrum matches the
+    training snapshots' mean on shells kfit..kmax: residual r_k =
+    log(<E_les(k)>/target(k)), <.> over the last three quarters of a
+    `steps`-step LES from the first snapshot. The gain sets the overall
+    drain, the tilt how it is shared between the middle and the top shells.
+    Jacobian steps 0.1 in log gain and 0.3 in tilt, updates clamped to 0.2
+    and 0.6; the best evaluated point is kept. Uses training data only."""
     coarse = data["coarse"]
     rhs = make_ns_rhs(coarse)
     kfit = max(1, min(kfit, kmax))
@@ -262,25 +291,11 @@ def calibrate_closure(net, data, params, dt, kmax, mask, target, steps=2000, ite
         _log(f"  closure calibration: gain {math.exp(x[0]):.4f}, tilt {x[1]:+.3f}, "
              f"rms log error {hist[-1]['rms_log_error']:.4f}, max {hist[-1]['max_log_error']:.4f}")
 
-    x = np.array([math.log(float(net.gain.detach())), net.tilt])
-    r = resid(x)
-    record(x, r)
-    for _ in range(iters):
-        J = np.empty((r.size, 2))
-        for j, d in enumerate((0.1, 0.3)):
-            xp = x.copy()
-            xp[j] += d
-            rp = resid(xp)
-            record(xp, rp)
-            J[:, j] = (rp - r) / d
-        dx = np.clip(-np.linalg.lstsq(J, r, rcond=None)[0], [-0.2, -0.6], [0.2, 0.6])
-        x = x + dx
-        r = resid(x)
-        record(x, r)
-    best = min(hist, key=lambda h: h["rms_log_error"])
+    x0 = [math.log(float(net.gain.detach())), net.tilt]
+    best, _ = gauss_newton(resid, x0, (0.1, 0.3), (0.2, 0.6), iters, on_eval=record)
     with torch.no_grad():
-        net.gain.fill_(best["gain"])
-    net.set_tilt(best["tilt"])
+        net.gain.fill_(math.exp(best[0]))
+    net.set_tilt(best[1])
     return hist
 
 

@@ -117,3 +117,22 @@ def test_progress_line(capsys):
     assert "step 10/10 (100%)" in out and "step  5/10 ( 50%)" in out
     assert "t = " in out and "eta " in out
     assert _fmt_secs(5.04) == "5.0s" and _fmt_secs(125.0) == "2m05s"
+
+
+def test_gauss_newton_calibration_step():
+    """The LES closure calibration's solver (host logic): on a residual that
+    is linear in (log gain, tilt) it lands on the root after one clamped
+    step; the best point is returned and every evaluation is recorded."""
+    les_experiment = pytest.importorskip("hit3d.les_experiment")
+    k = np.arange(5, 22, dtype=np.float64)
+    truth = np.array([0.07, -0.16])
+    A = np.stack([-np.ones_like(k), -(k / 21.0 - 0.5)], axis=1)   # more gain/tilt -> less energy
+
+    def resid(x):
+        return A @ (np.asarray(x) - truth)
+    best, pts = les_experiment.gauss_newton(resid, [0.0, 0.0], (0.1, 0.3), (0.2, 0.6), 2)
+    assert len(pts) == 7
+    assert np.allclose(best, truth, atol=1e-12)
+    # a step larger than the clamp is cut to it
+    best2, pts2 = les_experiment.gauss_newton(resid, [0.0, 0.0], (0.1, 0.3), (0.02, 0.6), 1)
+    assert abs(pts2[-1][0][0] - 0.02) < 1e-12
