@@ -163,7 +163,7 @@ struct Fft2 {
   static constexpr int kSlots = R * (Q + 1);
   static_assert(R * Q == N && (M == 1 || M == 2), "N must be a power of two");
 
-  __device__ static __forceinline__ int out_index(int t, int s) {
+  __host__ __device__ static constexpr int out_index(int t, int s) {
     return t + Q * (s / Q) + R * (s % Q);
   }
   // exit slot s, seen as the entry slot r of a following transform on the
@@ -206,7 +206,7 @@ struct Fft3 {
   static constexpr int kSlots = (R * E1 > QT * (R + 1)) ? R * E1 : QT * (R + 1);
   static_assert(R * Q2 == QT && Q2 >= 1 && Q2 <= R && R <= 64, "Fft3 needs R >= Q/R");
 
-  __device__ static __forceinline__ int out_index(int t, int s) { return t + QT * s; }
+  __host__ __device__ static constexpr int out_index(int t, int s) { return t + QT * s; }
   static constexpr int out_slot_as_input(int s) { return s; }
 
   template <class T>

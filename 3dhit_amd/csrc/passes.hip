@@ -221,6 +221,37 @@ __device__ __forceinline__ cpx<T> herm_pair(const cpx<T>* U, const cpx<T>* W, in
   return mirror ? cpx<T>{ux + wy, wx - uy} : cpx<T>{ux - wy, uy + wx};
 }
 
+// The 2/3 truncation of the pruned layout (kz = N/3 + 1 stored modes) leaves
+// the modes kz <= k <= N - kz of the Hermitian-extended pencil zero. Bit r of
+// kEntry: entry slot r (k = q + Q r) is in that band for every thread q, so it
+// is a compile-time zero; bit s of kExit: exit slot s (k = out_index(q, s)) is
+// in the band for every q, so the forward transforms' value there is never
+// stored or read.
+template <class F, int N>
+struct ZeroBand {
+  static constexpr int kz = N / 3 + 1;
+  static constexpr bool in_band(int k) { return k >= kz && k <= N - kz; }
+  static constexpr unsigned entry_mask() {
+    unsigned m = 0;
+    for (int r = 0; r < F::R; ++r) {
+      bool all = true;
+      for (int q = 0; q < F::Q; ++q) all = all && in_band(q + F::Q * r);
+      if (all) m |= 1u << r;
+    }
+    return m;
+  }
+  static constexpr unsigned exit_mask() {
+    unsigned m = 0;
+    for (int s = 0; s < F::R; ++s) {
+      bool all = true;
+      for (int q = 0; q < F::Q; ++q) all = all && in_band(F::out_index(q, s));
+      if (all) m |= 1u << s;
+    }
+    return m;
+  }
+  static constexpr unsigned kEntry = entry_mask(), kExit = exit_mask();
+};
+
 // Separate A = FFT(a + i b) into FFT(a)[k], FFT(b)[k] using A[N-k].
 template <class T>
 __device__ __forceinline__ void split_pair(cpx<T> ak, cpx<T> bk, cpx<T>& X, cpx<T>& Y) {
@@ -236,7 +267,7 @@ __device__ __forceinline__ void split_pair(cpx<T> ak, cpx<T> bk, cpx<T>& X, cpx<
 // kz modes are followed by kzs - kz padding slots, written as zeros so that
 // every line of the output is written whole.
 // ---------------------------------------------------------------------------
-template <int N, class T>
+template <int N, class T, bool PRUNED>
 __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_fused(cpx<T>* buf, int64_t fs, int64_t npencils,
                                                      int kz, int kzs, T scale) {
   using B = PencilBlock<N>;
@@ -255,11 +286,15 @@ __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_fused(cpx<T>* buf
   F::load_tw(tw, q);
   cpx<T> z0[R], z1[R], z2[R];
   cpx<T>* zz[3] = {z0, z1, z2};
+  using Z = ZeroBand<F, N>;
+  constexpr unsigned kZeroIn = PRUNED ? Z::kEntry : 0u, kUnused = PRUNED ? Z::kExit : 0u;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
 #pragma unroll
     for (int r = 0; r < R; ++r)
-      zz[c][r] = herm_pair<N, T>(buf + c * fs + base, buf + (3 + c) * fs + base, q + Q * r, kzl);
+      zz[c][r] = ((kZeroIn >> r) & 1u) ? czero<T>()
+                                       : herm_pair<N, T>(buf + c * fs + base, buf + (3 + c) * fs + base,
+                                                         q + Q * r, kzl);
     F::template run<+1, WaveSync>(zz[c], q, L, tw);
   }
   // physical space: z_c = u_c + i w_c at position out_index(q, s)
@@ -277,6 +312,7 @@ __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_fused(cpx<T>* buf
   if (active) {
 #pragma unroll
     for (int s = 0; s < R; ++s) {
+      if ((kUnused >> s) & 1u) continue;
       const int k = F::out_index(q, s);
       if (k < kz) buf[2 * fs + base + k] = Bv[s];
     }
@@ -284,7 +320,8 @@ __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_fused(cpx<T>* buf
   }
   WaveSync::sync();
 #pragma unroll
-  for (int s = 0; s < R; ++s) lp[F::out_index(q, s)] = A[s];
+  for (int s = 0; s < R; ++s)
+    if (!((kUnused >> s) & 1u)) lp[F::out_index(q, s)] = A[s];
   WaveSync::sync();
   if (active) {
     for (int k = q; k < kzs; k += Q) {
@@ -559,10 +596,17 @@ hipError_t launch_y_inverse_curl(int N, const cpx<T>* in, int64_t in_fs, cpx<T>*
 template <class T>
 hipError_t launch_z_fused(int N, cpx<T>* buf, int64_t fs, int64_t npencils, int kz, int kzs,
                           T scale, hipStream_t s) {
+  // the pruned layout stores exactly kz = N/3 + 1 modes; anything else (the
+  // dense layout, kz = N/2 + 1) takes the kernel without compile-time zeros
+  const bool pruned = kz == N / 3 + 1 && kz < N / 2 + 1;
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(npencils, PencilBlock<NN>::PW);
-    hipLaunchKernelGGL((k_z_fused<NN, T>), dim3(nb), dim3(PencilBlock<NN>::THREADS), 0, s, buf, fs, npencils, kz,
-                       kzs, scale);
+    if (pruned)
+      hipLaunchKernelGGL((k_z_fused<NN, T, true>), dim3(nb), dim3(PencilBlock<NN>::THREADS), 0, s, buf, fs,
+                         npencils, kz, kzs, scale);
+    else
+      hipLaunchKernelGGL((k_z_fused<NN, T, false>), dim3(nb), dim3(PencilBlock<NN>::THREADS), 0, s, buf, fs,
+                         npencils, kz, kzs, scale);
   });
   return hipGetLastError();
 }
