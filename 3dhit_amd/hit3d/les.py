@@ -10,9 +10,11 @@ aliasing below the cut because 2 kcut < N_les - kcut). The closure input is
 the filtered velocity-gradient tensor d_j ubar_i in physical space
 (`velocity_gradient`, channel 3 j + i).
 
-LES. d ubar/dt = rhs(ubar) + P M [-i k_j tau_hat_ij(net(grad ubar))], with
-rhs the solver's Navier-Stokes RHS (forcing included) and the closure term
-projected and dealiased like the nonlinear term. Every piece is a
+LES. d ubar/dt = rhs(ubar) + g S(k) P M [-i k_j tau_hat_ij(net(grad ubar))]
+(+ A(ubar) ubar with the HoldResolvedEnergy forcing), with rhs the solver's
+Navier-Stokes RHS, the closure term projected and dealiased like the
+nonlinear term, g the closure gain and S(k) = exp(tilt (|k|/kc - 1/2)) its
+spectral shape; everything restricted to the modes |k| <= kcut. Every piece is a
 differentiable torch operation backed by the library: the RHS's backward
 is the hand-written adjoint (hit3d_rhs_vjp), the linear spectral operators
 use their exact adjoints, and the net's backward runs the BF16-MFMA
