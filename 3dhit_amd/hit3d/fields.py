@@ -14,7 +14,7 @@ import torch
 
 from . import _lib
 from .diagnostics import shell_spectrum
-from .grid import check_spectral, stream, to_spectral
+from .grid import check_spectral, mark_dealiased, stream, to_spectral
 
 __all__ = ["spectral_state", "random_field", "taylor_green", "default_spectrum",
            "config_spectrum", "uniform_noise", "CONFIG_SPECTRUM_C"]
@@ -64,7 +64,7 @@ def random_field(seed, grid, spectrum=default_spectrum):
     c_dev = torch.from_numpy(c.astype(np.float64)).to(grid.device)
     _lib.call("hit3d_shell_scale", grid.desc, u_hat.data_ptr(), 3, c_dev.data_ptr(), len(c),
               stream())
-    return u_hat
+    return mark_dealiased(u_hat)
 
 
 def taylor_green(grid):
@@ -80,4 +80,4 @@ def taylor_green(grid):
     u_hat = to_spectral(u.contiguous(), grid)
     _lib.call("hit3d_dealias_project", grid.desc, u_hat.data_ptr(), u_hat.data_ptr(), 3, 1, 0,
               stream())
-    return u_hat
+    return mark_dealiased(u_hat)

@@ -192,6 +192,21 @@ def check_spectral(u_hat, grid, ncomp=None, name="u_hat"):
     return u_hat.contiguous()
 
 
+# Python attribute that records "no content outside the 2/3 box" on a state
+# tensor, together with the tensor's version counter at that moment (any
+# in-place write bumps the counter and voids the mark).
+DEALIASED_MARK = "_hit3d_dealiased_version"
+
+
+def mark_dealiased(u_hat):
+    """Records on the tensor that it has no content outside the 2/3 box. The
+    library's own producers of such states (the pruned step, the RHS on a
+    dealiased state, random_field, dealias) mark their outputs, so that a
+    chain rollout -> rollout never reads the device to decide the layout."""
+    setattr(u_hat, DEALIASED_MARK, u_hat._version)
+    return u_hat
+
+
 def _new_like(u):
     return torch.empty_like(u, memory_format=torch.contiguous_format)
 
@@ -202,7 +217,7 @@ def dealias(u_hat, grid):
     out = _new_like(u)
     _lib.call("hit3d_dealias_project", grid.desc, u.data_ptr(), out.data_ptr(), u.shape[0], 1, 0,
               stream())
-    return out
+    return mark_dealiased(out)
 
 
 def project(u_hat, grid):

@@ -14,24 +14,29 @@ import time
 
 from . import _lib
 from .callbacks import Callback, Diagnostic, State
-from .grid import check_spectral, stream
-from .navierstokes import is_dealiased
+from .grid import check_spectral, mark_dealiased, stream
+from .navierstokes import known_dealiased
 
 __all__ = ["rk4_step", "rollout", "evolve", "State", "Callback", "Diagnostic"]
 
 
 def _steps(rhs, u_hat, params, dt, nsteps, t0, diag=None):
+    """`nsteps` steps of hit3d_rk4_steps on a copy of u_hat. The layout
+    (pruned when u_hat has no content outside the 2/3 box) is decided from
+    the tensor's mark when it carries one, so a chain of rollouts on the
+    library's own outputs never waits for the device."""
     g = rhs.grid
+    inside = known_dealiased(u_hat, g)
     u = check_spectral(u_hat, g, 3).clone()
     if nsteps == 0:
-        return u
-    pruned = 1 if is_dealiased(u, g) else 0
+        return mark_dealiased(u) if inside else u
+    pruned = 1 if inside else 0
     ws = g.workspace(_lib.OP_RK4, 3, pruned)
     f = params.forcing.desc()
     _lib.call("hit3d_rk4_steps", g.desc, u.data_ptr(), float(params.nu), float(dt), int(nsteps),
               float(t0), ctypes.byref(f), pruned,
               None if diag is None else diag.data_ptr(), ws.data_ptr(), ws.numel(), stream())
-    return u
+    return mark_dealiased(u) if pruned else u
 
 
 def rk4_step(rhs, u_hat, params, dt, t):

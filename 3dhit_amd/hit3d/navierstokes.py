@@ -17,7 +17,7 @@ import torch
 
 from . import _lib
 from .forcing import NoForcing
-from .grid import check_spectral, stream
+from .grid import DEALIASED_MARK, check_spectral, mark_dealiased, stream
 
 __all__ = ["NSParams", "make_ns_rhs", "linear_operator", "is_dealiased"]
 
@@ -40,13 +40,26 @@ class NSParams:
 def is_dealiased(u_hat, grid):
     """True when u_hat has no content outside the per-axis 2/3 box (the
     solver invariant); such states take the pruned transforms, which give
-    the same result as the dense ones with about half the memory traffic."""
+    the same result as the dense ones with about half the memory traffic.
+    One FP64 reduction on the device and one host read of its result."""
     u = check_spectral(u_hat, grid, 3)
     out = torch.empty(1, dtype=torch.float64, device=u.device)
     ws = grid.workspace(_lib.OP_REDUCE)
     _lib.call("hit3d_alias_energy", grid.desc, u.data_ptr(), out.data_ptr(), ws.data_ptr(),
               ws.numel(), stream())
     return float(out.item()) == 0.0
+
+
+def known_dealiased(u_hat, grid):
+    """is_dealiased without the host read when the tensor carries a valid
+    mark (same object, not written in place since it was marked); otherwise
+    the device check, whose positive answer is then recorded."""
+    if getattr(u_hat, DEALIASED_MARK, None) == u_hat._version:
+        return True
+    if is_dealiased(u_hat, grid):
+        mark_dealiased(u_hat)
+        return True
+    return False
 
 
 class NSRhs:
@@ -56,19 +69,20 @@ class NSRhs:
         self.grid = grid
 
     def __call__(self, u_hat, params, t, dealiased=None):
-        """`dealiased` (bool) skips the per-call check (one host sync) when
-        the caller knows whether u_hat is inside the 2/3 box."""
+        """`dealiased` (bool) states whether u_hat is inside the 2/3 box; by
+        default a marked tensor (see `mark_dealiased`) is taken as such and
+        any other one is checked on the device (one host read)."""
         g = self.grid
         u = check_spectral(u_hat, g, 3)
         if dealiased is None:
-            dealiased = is_dealiased(u, g)
+            dealiased = known_dealiased(u_hat, g)
         pruned = 1 if dealiased else 0
         du = torch.empty_like(u)
         ws = g.workspace(_lib.OP_RHS, 3, pruned)
         f = params.forcing.desc()
         _lib.call("hit3d_rhs", g.desc, u.data_ptr(), du.data_ptr(), float(params.nu),
                   ctypes.byref(f), float(t), pruned, ws.data_ptr(), ws.numel(), stream())
-        return du
+        return mark_dealiased(du) if pruned else du
 
     def __repr__(self):
         return f"ns_rhs({self.grid!r})"

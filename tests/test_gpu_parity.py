@@ -151,8 +151,8 @@ def test_rhs_anisotropic(shape):
 # The fused pruned passes at the large transform sizes (256-, 512- and
 # 1024-point kernels, padded kz rows) along each axis in turn, on grids small
 # enough for the oracle.
-LARGE_AXIS = [(256, 8, 8), (8, 256, 8), (8, 8, 256), (512, 8, 4), (1024, 8, 4), (4, 1024, 8),
-              (8, 4, 1024)]
+LARGE_AXIS = [(256, 8, 8), (8, 256, 8), (8, 8, 256), (512, 8, 4), (8, 512, 4), (4, 8, 512),
+              (1024, 8, 4), (4, 1024, 8), (8, 4, 1024)]
 
 
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 5e-6)])
@@ -166,7 +166,7 @@ def test_rhs_large_axis_pruned(shape, dtype, tol):
     assert rel(h3.make_ns_rhs(g)(dev(uh, g), params[1], 0.0), ref) < tol
 
 
-@pytest.mark.parametrize("shape", [(1024, 8, 4), (8, 8, 256)])
+@pytest.mark.parametrize("shape", [(1024, 8, 4), (8, 8, 256), (4, 8, 512), (8, 512, 4)])
 def test_rollout_large_axis_pruned(shape):
     """RK4 stage kernels and the per-stage FP64 coefficient at large N."""
     og, g = _grids(shape, np.float64)
@@ -335,3 +335,120 @@ def test_bad_inputs_raise():
 def test_taylor_green_matches_oracle(dtype, tol):
     og, g = _grids((16, 32, 16), dtype)
     assert rel(h3.taylor_green(g), O.taylor_green(og)) < tol
+
+
+# --- boxes with L != 2 pi (k0 != 1) --------------------------------------------------------------
+# The wavenumbers scale with 2 pi / L (grid.py:105-107) and the spectrum bins
+# by k0 = 2 pi / Lx (diagnostics.py:76-82): RHS, rollout and spectra on a cubic
+# box of side 3 and on an anisotropic box.
+
+BOXES = [((32, 32, 32), (3.0, 3.0, 3.0)), ((32, 16, 8), (3.0, 1.7, 5.5)), ((16, 32, 64), (9.0, 9.0, 9.0))]
+
+
+@pytest.mark.parametrize("box", range(len(BOXES)))
+@pytest.mark.parametrize("fi", range(len(FORCINGS)))
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 3e-6)])
+def test_rhs_non_unit_box(box, fi, dtype, tol):
+    shape, (lx, ly, lz) = BOXES[box]
+    og = O.make_grid(*shape, Lx=lx, Ly=ly, Lz=lz, dtype=dtype)
+    g = h3.make_grid(*shape, Lx=lx, Ly=ly, Lz=lz, dtype=DT[dtype])
+    uh = O.project(random_spectral(30 + fi, og), og)
+    of, gf = FORCINGS[fi]
+    ref = O.make_ns_rhs(og)(uh, O.NSParams(nu=0.02, forcing=of()), 0.0)
+    for dealiased in (True, False):   # pruned and dense layouts on the same input
+        du = h3.make_ns_rhs(g)(dev(uh, g), h3.NSParams(nu=0.02, forcing=gf()), 0.0, dealiased=dealiased)
+        assert rel(du, ref) < tol
+
+
+@pytest.mark.parametrize("box", range(len(BOXES)))
+def test_rollout_and_spectra_non_unit_box(box):
+    shape, (lx, ly, lz) = BOXES[box]
+    og = O.make_grid(*shape, Lx=lx, Ly=ly, Lz=lz)
+    g = h3.make_grid(*shape, Lx=lx, Ly=ly, Lz=lz, dtype=torch.float64)
+    uh = O.project(random_spectral(40 + box, og), og)
+    op = O.NSParams(nu=0.01, forcing=O.LundgrenForcing(nu=0.01, energy=0.5, tau=0.5))
+    hp = h3.NSParams(nu=0.01, forcing=h3.LundgrenForcing(nu=0.01, energy=0.5, tau=0.5))
+    ref = O.rollout(O.make_ns_rhs(og), uh, op, 2e-3, 10)
+    got = h3.rollout(h3.make_ns_rhs(g), dev(uh, g), hp, 2e-3, 10)
+    assert rel(got, ref) < 1e-11
+    ks, Es = h3.energy_spectrum(got, g)
+    oks, oEs = O.energy_spectrum(ref, og)
+    assert np.array_equal(ks, oks)           # (1..smax) * 2 pi / Lx
+    assert np.max(np.abs(Es - oEs)) < 1e-12 * np.max(oEs)
+    for a, b in zip(h3.component_spectra(got, g)[1:], O.component_spectra(ref, og)[1:]):
+        assert np.max(np.abs(a - b)) < 1e-12 * np.max(b)
+    assert abs(h3.energy(got, g) / O.energy(ref, og) - 1) < 1e-12
+    assert abs(h3.dissipation(got, g, 0.01) / O.dissipation(ref, og, 0.01) - 1) < 1e-12
+
+
+# --- BandForcing band edges (forcing.py:135-136: kmin^2 <= k2 <= kmax^2) -------------------------
+
+
+@pytest.mark.parametrize("dealiased", [True, False])
+def test_band_forcing_edges_closed_form(dealiased):
+    """Four divergence-free single modes on a 16^3 box, at k2 = kmin^2 - 1,
+    kmin^2, kmax^2 and kmax^2 + 1 for kmin = 2, kmax = 3: the forcing part of
+    the RHS, rhs(band) - rhs(none), must equal c u on the two edge modes and
+    vanish on the two outside ones, with c = eps / (2 E_band) and 2 E_band
+    = sum over the two forced modes of 2 |a|^2 / Ntot^2 (all at kz > 0,
+    Hermitian weight 2)."""
+    og, g = _grids((16, 16, 16), np.float64)
+    u = np.zeros((3, 16, 16, 9), dtype=np.complex128)
+    modes = {  # (ix, iy, iz): (vector perpendicular to k, amplitude, forced?)
+        (1, 1, 1): ((1.0, -1.0, 0.0), 0.7 + 0.2j, False),   # k2 = 3 = kmin^2 - 1
+        (0, 0, 2): ((1.0, 0.0, 0.0), 0.4 - 0.9j, True),     # k2 = 4 = kmin^2
+        (0, 0, 3): ((0.0, 1.0, 0.0), -1.1 + 0.3j, True),    # k2 = 9 = kmax^2
+        (1, 0, 3): ((0.0, 1.0, 0.0), 0.5 + 0.5j, False),    # k2 = 10 = kmax^2 + 1
+    }
+    for (ix, iy, iz), (vec, amp, _) in modes.items():
+        for c in range(3):
+            u[c, ix, iy, iz] = vec[c] * amp * 1000.0
+    eps = 0.3
+    two_e_band = sum(2 * np.sum(np.abs(u[:, ix, iy, iz]) ** 2)
+                     for (ix, iy, iz), (_, _, forced) in modes.items() if forced) / float(16**3) ** 2
+    coef = eps / two_e_band
+    rhs = h3.make_ns_rhs(g)
+    d = dev(u, g)
+    band = rhs(d, h3.NSParams(nu=0.0, forcing=h3.BandForcing(eps=eps, kmin=2.0, kmax=3.0)), 0.0,
+               dealiased=dealiased)
+    none = rhs(d, h3.NSParams(nu=0.0), 0.0, dealiased=dealiased)
+    f = (band - none).cpu().numpy()
+    expect = np.zeros_like(u)
+    for (ix, iy, iz), (_, _, forced) in modes.items():
+        if forced:
+            expect[:, ix, iy, iz] = coef * u[:, ix, iy, iz]
+    assert np.max(np.abs(f - expect)) < 1e-12 * np.max(np.abs(expect))
+    assert h3.BandForcing(eps=eps, kmin=2.0, kmax=3.0).injection(d, g) == eps
+
+
+# --- layout choice without host reads -----------------------------------------------------------
+
+
+def test_rollout_chain_skips_the_alias_check_and_in_place_edits_void_it(monkeypatch):
+    """States the library produced inside the 2/3 box carry a mark, so a chain
+    of rollouts never runs the alias check (a device reduction plus a host
+    read). Writing into the tensor voids the mark: the next call checks,
+    finds content outside the box and takes the dense layout, which keeps the
+    reference semantics (those modes feed the nonlinear term)."""
+    from hit3d import _lib
+    og, g = _grids((32, 32, 32), np.float64)
+    calls = []
+    real = _lib.call
+
+    def spy(name, *args):
+        calls.append(name)
+        return real(name, *args)
+    monkeypatch.setattr(_lib, "call", spy)     # every module calls through _lib.call
+    rhs = h3.make_ns_rhs(g)
+    p = h3.NSParams(nu=0.02)
+    u0 = h3.random_field(3, g, h3.config_spectrum)
+    u1 = h3.rollout(rhs, u0, p, 1e-3, 2)
+    u2 = h3.rollout(rhs, u1, p, 1e-3, 2)
+    rhs(u2, p, 0.0)
+    assert "hit3d_alias_energy" not in calls
+    u2[0, 16, 0, 0] = 0.25 + 0.0j           # Nyquist x mode: outside the 2/3 box
+    calls.clear()
+    u3 = h3.rollout(rhs, u2, p, 1e-3, 1)
+    assert calls.count("hit3d_alias_energy") == 1
+    ref = O.rollout(O.make_ns_rhs(og), u2.cpu().numpy(), O.NSParams(nu=0.02), 1e-3, 1)
+    assert rel(u3, ref) < 1e-12

@@ -230,6 +230,52 @@ def test_injection_exact_per_forcing(grid32):
     assert np.abs(cf.term(np.zeros_like(u_hat), g, 0.0)).max() == 0.0
 
 
+@pytest.mark.parametrize("kmin,kmax", [(2.0, 3.0), (1.0, 2.0), (np.sqrt(5.0), 3.0)])
+def test_band_forcing_edges_closed_form(kmin, kmax):
+    """forcing.py:135-136 forces kmin^2 <= k2 <= kmax^2, both edges included.
+    One mode per integer k2 = 1..12 (all at kz > 0, Hermitian weight 2, on a
+    16^3 unit box where k2 is exact): the term is c a on exactly the modes
+    with kmin^2 <= k2 <= kmax^2 (computed here from the integer mode numbers),
+    0 elsewhere, with c = eps / (2 E_band), 2 E_band = sum of 2 |a|^2 / Ntot^2
+    over those modes. A strict inequality at either edge, or a mask on |k|
+    instead of k2, moves a mode across and fails."""
+    g = O.make_grid(16)
+    u = np.zeros((3, 16, 16, 9), dtype=np.complex128)
+    rng = np.random.default_rng(7)
+    forced, amps = [], {}
+    for k2 in range(1, 13):
+        n = next(((ix, iy, iz) for iz in range(1, 5) for iy in range(0, 5) for ix in range(0, 5)
+                  if ix * ix + iy * iy + iz * iz == k2), None)
+        if n is None:  # k2 = 7 is not a sum of three squares
+            continue
+        a = rng.standard_normal(3) + 1j * rng.standard_normal(3)
+        u[:, n[0], n[1], n[2]] = a
+        amps[n] = a
+        if kmin * kmin <= k2 <= kmax * kmax:
+            forced.append(n)
+    assert forced
+    two_e = sum(2 * np.sum(np.abs(amps[n]) ** 2) for n in forced) / float(16**3) ** 2
+    f = O.BandForcing(eps=0.3, kmin=kmin, kmax=kmax)
+    term = f.term(u, g, 0.0)
+    expect = np.zeros_like(u)
+    for n in forced:
+        expect[:, n[0], n[1], n[2]] = (0.3 / two_e) * amps[n]
+    assert np.max(np.abs(term - expect)) <= 1e-12 * np.max(np.abs(expect))
+    assert float(f.injection(u, g)) == 0.3
+
+
+def test_band_forcing_guard_is_dtype_eps():
+    """forcing.py:141-143: no forcing when 2 E_band <= finfo(dtype).eps."""
+    g = O.make_grid(16)
+    u = np.zeros((3, 16, 16, 9), dtype=np.complex128)
+    f = O.BandForcing(eps=0.3, kmin=1.0, kmax=1.0)
+    # one mode at k2 = 1, kz = 1 (weight 2): 2 E_band = 2 |a|^2 / Ntot^2
+    u[0, 0, 0, 1] = np.sqrt(0.4 * np.finfo(np.float64).eps) * 16**3
+    assert np.abs(f.term(u, g, 0.0)).max() == 0.0 and float(f.injection(u, g)) == 0.0
+    u[0, 0, 0, 1] = np.sqrt(0.6 * np.finfo(np.float64).eps) * 16**3
+    assert np.abs(f.term(u, g, 0.0)).max() > 0.0 and float(f.injection(u, g)) == 0.3
+
+
 def test_lundgren_holds_energy(grid32):
     """dE/dt = 2AE - eps = 0 for the balance rule: E stays at its initial
     value up to RK4 truncation; with tau the energy relaxes toward E*."""
@@ -390,3 +436,23 @@ def test_taylor_green_closed_form():
     assert np.allclose(u[2], 0, atol=1e-14)
     uh = O.taylor_green(g)
     assert np.max(np.abs(g.kx * uh[0] + g.ky * uh[1] + g.kz * uh[2])) < 1e-10
+
+
+def test_forced_256_record_matches_the_oracle():
+    """tests/golden/forced_256_lundgren.npz (written by make_forced_256.py from
+    this oracle) is current: its step-0 spectrum, energy and sampled modes
+    are the oracle's random_field(0) at 256^3, E = 0.5 from the C k^4 exp(-2
+    (k/4)^2) normalisation, eps = 2 nu Omega, and E stays at 0.5 to round-off
+    under the constant-energy forcing over the 10 recorded steps."""
+    import os
+    rec = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                               "forced_256_lundgren.npz"))
+    g = O.make_grid(256)
+    u0 = O.random_field(int(rec["seed"]), g, O.config_spectrum)
+    assert np.array_equal(u0.ravel()[rec["index"]], rec["sample0"])
+    _, E = O.energy_spectrum(u0, g)
+    assert np.allclose(E, rec["spectra"][0][1:], rtol=1e-13, atol=0)
+    assert abs(rec["E"][0] - 0.5) < 1e-14
+    assert np.allclose(rec["eps"], 2 * float(rec["nu"]) * rec["Omega"], rtol=1e-15)
+    assert np.max(np.abs(rec["E"] - 0.5)) < 1e-12
+    assert np.allclose(rec["spectra"].sum(axis=1), rec["E"], rtol=1e-12)
