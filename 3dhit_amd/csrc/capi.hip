@@ -4,6 +4,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -704,6 +705,56 @@ int check_ptr(const void* p, const char* name) {
     if (rc_) return rc_;   \
   } while (0)
 
+// [E, eps, Omega, E_band] of one state from the 1024 fixed-block FP64 partial
+// sums: `partials` (3 * kReduceBlocks doubles, every entry written by the
+// partials kernel) and `scalars` (16 doubles) are device scratch.
+int energy_dissipation_impl(const hit3d_grid& gd, const void* u_hat, double nu, double kmin,
+                            double kmax, double* out, double* partials, double* scalars,
+                            hipStream_t s) {
+  const Geo g = make_geo(gd, false);
+  hit3d_forcing band{};
+  band.kind = HIT3D_FORCING_BAND;
+  band.kmin = kmin;
+  band.kmax = kmax;
+  if (gd.precision == 4)
+    H3_TRY(reduce_state<float>(gd, g, static_cast<const cpx<float>*>(u_hat), &band, partials,
+                               scalars, s));
+  else
+    H3_TRY(reduce_state<double>(gd, g, static_cast<const cpx<double>*>(u_hat), &band, partials,
+                                scalars, s));
+  hipLaunchKernelGGL(k_energy_out, dim3(1), dim3(1), 0, s, scalars, nu, out);
+  H3_HIP(hipGetLastError());
+  return HIT3D_OK;
+}
+
+// ---- device-resident scratch of hit3d_energy_dissipation_noalloc ------------
+// The partial sums and totals live in this file-scope array instead of a
+// caller workspace. Each GPU holds its own copy for the life of the process;
+// host code reaches it only through the address hipGetSymbolAddress returns
+// on that device (the array's name is not a device address in host code).
+// Every call on one GPU shares that GPU's copy: calls that overlap on one
+// GPU (two streams, or two host threads) race on it.
+constexpr size_t kScratchDoubles = 3 * size_t(h3::kReduceBlocks) + 32;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_dev;  // per device ordinal, null until first use
+
+int scratch_on_current_device(double** out) {
+  int dev = 0;
+  H3_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (size_t(dev) >= g_scratch_dev.size()) g_scratch_dev.resize(size_t(dev) + 1, nullptr);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    H3_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(HIT3D_ERR_HIP, "hipGetSymbolAddress returned a null scratch address");
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return HIT3D_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -927,21 +978,26 @@ int hit3d_energy_dissipation(const hit3d_grid* grid, const void* u_hat, double n
   H3_TRY(check_work(work, work_bytes, reduce_bytes(*grid, nullptr, nullptr, nullptr)));
   double *partials, *scalars;
   reduce_bytes(*grid, &partials, &scalars, work);
-  const Geo g = make_geo(*grid, false);
-  hit3d_forcing band{};
-  band.kind = HIT3D_FORCING_BAND;
-  band.kmin = kmin;
-  band.kmax = kmax;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  if (grid->precision == 4)
-    H3_TRY(reduce_state<float>(*grid, g, static_cast<const cpx<float>*>(u_hat), &band, partials,
-                               scalars, s));
-  else
-    H3_TRY(reduce_state<double>(*grid, g, static_cast<const cpx<double>*>(u_hat), &band, partials,
-                                scalars, s));
-  hipLaunchKernelGGL(k_energy_out, dim3(1), dim3(1), 0, s, scalars, nu, out);
-  H3_HIP(hipGetLastError());
-  return HIT3D_OK;
+  return energy_dissipation_impl(*grid, u_hat, nu, kmin, kmax, out, partials, scalars,
+                                 static_cast<hipStream_t>(stream));
+}
+
+int hit3d_energy_dissipation_noalloc(const hit3d_grid* grid, const void* u_hat, double nu,
+                                     double kmin, double kmax, double* out, void* stream) {
+  H3_TRY(check_grid(grid));
+  H3_TRY(check_ptr(u_hat, "u_hat"));
+  H3_TRY(check_ptr(out, "out"));
+  const size_t need = reduce_bytes(*grid, nullptr, nullptr, nullptr);
+  if (need > sizeof(double) * kScratchDoubles)
+    return fail(HIT3D_ERR_WORKSPACE, "hit3d_energy_dissipation_noalloc needs " + std::to_string(need) +
+                                         " bytes of scratch, its device array holds " +
+                                         std::to_string(sizeof(double) * kScratchDoubles));
+  double* base = nullptr;
+  H3_TRY(scratch_on_current_device(&base));
+  double *partials, *scalars;
+  reduce_bytes(*grid, &partials, &scalars, base);  // same carving as the workspace form
+  return energy_dissipation_impl(*grid, u_hat, nu, kmin, kmax, out, partials, scalars,
+                                 static_cast<hipStream_t>(stream));
 }
 
 int hit3d_alias_energy(const hit3d_grid* grid, const void* u_hat, double* out, void* work,

@@ -138,6 +138,19 @@ int hit3d_energy_dissipation(const hit3d_grid* grid, const void* u_hat, double n
                              double kmax, double* out, void* work, size_t work_bytes,
                              void* stream);
 
+/* hit3d_energy_dissipation without a workspace argument. No counterpart in
+   the reference: for callers that cannot allocate device memory (code under
+   HIP graph capture, serving loops, harnesses that forbid hipMalloc). Its
+   scratch (the 1024 FP64 block partials and the totals, 24.8 KB) is a
+   file-scope device array of the library, one copy per GPU, whose address is
+   looked up once per device on the first call there; the call allocates
+   nothing. Same kernels and summation order as hit3d_energy_dissipation, so
+   the results are bit-identical. Every call on one GPU shares that GPU's
+   scratch: calls that overlap on one GPU (concurrent streams or host
+   threads) race and must be serialised by the caller. */
+int hit3d_energy_dissipation_noalloc(const hit3d_grid* grid, const void* u_hat, double nu,
+                                     double kmin, double kmax, double* out, void* stream);
+
 /* out (device, 1 double) = sum of |u_hat|^2 over the modes outside the 2/3
    box: zero means the state can take the pruned (`dealiased` = 1) path. */
 int hit3d_alias_energy(const hit3d_grid* grid, const void* u_hat, double* out, void* work,
