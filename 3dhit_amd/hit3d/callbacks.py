@@ -86,7 +86,7 @@ class Diagnostic:
         if path is not None and name is None:
             raise ValueError("Diagnostic(..., path=...) also needs `name` "
                              "(the series/<name> group to persist it under)")
-        if path is not None:
+        if path is not None and not isinstance(path, MemorySink):
             _h5py()
         self.f, self.every, self.path, self.name = f, every, path, name
         self.times, self.values = [], []
@@ -108,22 +108,36 @@ class Diagnostic:
             save_series(self.path, self.name, self)
 
 
+def _series_columns(diagnostic):
+    """t plus `values` (scalar diagnostics) or one column per key (dict values)."""
+    cols = {"t": np.asarray(diagnostic.times)}
+    values = diagnostic.values
+    if values and isinstance(values[0], dict):
+        for fld in values[0]:
+            cols[fld] = np.asarray([v[fld] for v in values])
+    else:
+        cols["values"] = np.asarray(values)
+    return cols
+
+
 def save_series(path, name, diagnostic):
-    """callbacks.py:188-217: append a Diagnostic's series to the HDF5 file at
-    `path` as series/<name>/t plus `values` (scalars) or one vector per key
-    (dict values). Errors if series/<name> already exists."""
+    """callbacks.py:188-217: append a Diagnostic's series as series/<name>/t
+    plus `values` (scalars) or one vector per key (dict values), to the HDF5
+    file at `path` or to a MemorySink. Errors if series/<name> already exists."""
+    cols = _series_columns(diagnostic)
+    if isinstance(path, MemorySink):
+        if name in path.series:
+            raise ValueError(f"{series_key(name)} already exists in the memory sink (saving the "
+                             "same series twice? reruns should start from a fresh sink)")
+        path.series[name] = cols
+        return path
     h5py = _h5py()
     with h5py.File(path, "a") as f:
         if series_key(name) in f:
             raise ValueError(f"{series_key(name)} already exists in {path} (saving the same "
                              "series twice? reruns should start from a fresh file)")
-        f[series_field(name, "t")] = np.asarray(diagnostic.times)
-        values = diagnostic.values
-        if values and isinstance(values[0], dict):
-            for fld in values[0]:
-                f[series_field(name, fld)] = np.asarray([v[fld] for v in values])
-        else:
-            f[series_field(name, "values")] = np.asarray(values)
+        for fld, col in cols.items():
+            f[series_field(name, fld)] = col
     return path
 
 
@@ -156,11 +170,13 @@ class Hdf5Sink:
 
 
 class MemorySink:
-    """Keeps snapshots in host memory: `snapshots[step] = {"t": t, field:
-    array}` and the grid metadata in `meta` (for in-process consumers)."""
+    """Keeps a run in host memory: `snapshots[step] = {"t": t, field:
+    array}`, the grid metadata in `meta` and the series that save_series
+    (or Diagnostic(path=sink, name=...)) stores in `series[name]`. The
+    report readers (load_run, read_series, ...) accept it in place of a file."""
 
     def __init__(self):
-        self.snapshots, self.meta = {}, None
+        self.snapshots, self.meta, self.series = {}, None, {}
 
     def write(self, step, t, arrays, meta):
         if self.meta is None:

@@ -1,83 +1,163 @@
-"""Readers of `FieldWriter` snapshot files (report.py:22-101 of hit3d):
-`load_run`, `read_series`, `step_keys`, `read_grid`, `read_snapshot`.
-They need h5py (imported when called; there is no other file format)."""
+"""Readers of recorded runs: the counterpart of `FieldWriter`'s sinks.
+
+A run is read through a small view object over wherever the sink put it:
+
+* an HDF5 file in the reference's layout (a path or an open h5py.File;
+  needs h5py, imported on use), or
+* a `MemorySink` (snapshots and series kept in host memory by
+  `FieldWriter(sink=...)` and `save_series(sink, ...)`).
+
+Every reader takes either, so the public functions of report.py:22-101 of
+hit3d keep their names, results and messages (`load_run`, `read_series`,
+`step_keys`, `read_grid`, `read_snapshot`) while no part of them depends on
+the storage format.
+"""
 
 from __future__ import annotations
 
+import contextlib
 import warnings
 from types import SimpleNamespace
 
 import numpy as np
 import torch
 
-from .callbacks import _h5py
+from .callbacks import MemorySink, _h5py
 from .grid import make_grid
 from .schema import grid_key, is_step_key, series_key, step_key, step_num
 
 __all__ = ["load_run", "read_series", "step_keys", "read_grid", "read_snapshot"]
 
+_GRID_FIELDS = ("Nx", "Ny", "Nz", "Lx", "Ly", "Lz")
+
+
+class _FileView:
+    """A recorded run inside an open HDF5 file (reference key layout)."""
+
+    def __init__(self, f):
+        self.f = f
+
+    def steps(self):
+        return sorted(step_num(k) for k in self.f.keys() if is_step_key(k))
+
+    def grid_meta(self):
+        if "grid" not in self.f:
+            return None
+        meta = {k: self.f[grid_key(k)][()] for k in _GRID_FIELDS}
+        if grid_key("nu") in self.f:
+            meta["nu"] = self.f[grid_key("nu")][()]
+        return meta
+
+    def field_info(self, step, field="u_hat"):
+        d = self.f[f"{step_key(step)}/{field}"]
+        return tuple(d.shape), np.dtype(d.dtype)
+
+    def field(self, step, field="u_hat"):
+        return np.asarray(self.f[f"{step_key(step)}/{field}"])
+
+    def series(self, name):
+        grp = series_key(name)
+        if grp not in self.f:
+            return None
+        return {k: np.asarray(self.f[grp][k]) for k in self.f[grp].keys()}
+
+
+class _SinkView:
+    """A recorded run held by a MemorySink."""
+
+    def __init__(self, sink):
+        self.sink = sink
+
+    def steps(self):
+        return sorted(self.sink.snapshots)
+
+    def grid_meta(self):
+        return None if self.sink.meta is None else dict(self.sink.meta)
+
+    def field_info(self, step, field="u_hat"):
+        a = self.sink.snapshots[step][field]
+        return tuple(a.shape), np.dtype(a.dtype)
+
+    def field(self, step, field="u_hat"):
+        return np.asarray(self.sink.snapshots[step][field])
+
+    def series(self, name):
+        s = self.sink.series.get(name)
+        return None if s is None else {k: np.asarray(v) for k, v in s.items()}
+
+
+def _is_h5_file(obj):
+    return type(obj).__module__.startswith("h5py") and hasattr(obj, "keys")
+
+
+@contextlib.contextmanager
+def _view(source):
+    """View over `source`: a MemorySink, an open h5py.File or a file path."""
+    if isinstance(source, MemorySink):
+        yield _SinkView(source)
+    elif _is_h5_file(source):
+        yield _FileView(source)
+    else:
+        with _h5py().File(source, "r") as f:
+            yield _FileView(f)
+
+
+def _describe(source):
+    return "the memory sink" if isinstance(source, MemorySink) else str(getattr(source, "filename", source))
+
+
+def _grid_of(view, dtype):
+    meta = view.grid_meta()
+    if meta is not None:
+        g = make_grid(int(meta["Nx"]), int(meta["Ny"]), int(meta["Nz"]), Lx=float(meta["Lx"]),
+                      Ly=float(meta["Ly"]), Lz=float(meta["Lz"]), dtype=dtype)
+        return g, (float(meta["nu"]) if "nu" in meta else None)
+    steps = view.steps()
+    if not steps:
+        raise ValueError("no snapshots found in file")
+    _, nx, ny, nzh = view.field_info(steps[0])[0]
+    nz = 2 * (nzh - 1)                       # the rfft layout of an even Nz
+    warnings.warn(f"no grid metadata in file; assuming a 2π cube ({nx}x{ny}x{nz}) and unknown nu",
+                  stacklevel=3)
+    return make_grid(nx, ny, nz, dtype=dtype), None
+
 
 def step_keys(f):
-    """Sorted step_######## group names of an open file."""
-    return sorted(k for k in f.keys() if is_step_key(k))
+    """Sorted step_######## group names of a run (open file or MemorySink)."""
+    with _view(f) as v:
+        return [step_key(n) for n in v.steps()]
 
 
 def read_grid(f, dtype):
-    """The Grid (precision `dtype`) and nu from the file's grid/* group; a
-    file without it is taken as a 2 pi cube with unknown nu, sized from the
-    rfft layout of its first snapshot (even Nz assumed), with a warning."""
-    if "grid" in f:
-        g = make_grid(int(f[grid_key("Nx")][()]), int(f[grid_key("Ny")][()]),
-                      int(f[grid_key("Nz")][()]), Lx=float(f[grid_key("Lx")][()]),
-                      Ly=float(f[grid_key("Ly")][()]), Lz=float(f[grid_key("Lz")][()]),
-                      dtype=dtype)
-        nu = float(f[grid_key("nu")][()]) if grid_key("nu") in f else None
-        return g, nu
-    keys = step_keys(f)
-    if not keys:
-        raise ValueError("no snapshots found in file")
-    shape = f[f"{keys[0]}/u_hat"].shape
-    nz = 2 * (shape[3] - 1)
-    warnings.warn(f"no grid metadata in file; assuming a 2π cube ({shape[1]}x{shape[2]}x{nz}) "
-                  "and unknown nu", stacklevel=2)
-    return make_grid(shape[1], shape[2], nz, dtype=dtype), None
+    """(Grid of precision `dtype`, nu) from the run's grid metadata. A run
+    recorded without it is read as a 2 pi cube with unknown nu whose sizes
+    come from the first snapshot's rfft layout (even Nz), with a warning."""
+    with _view(f) as v:
+        return _grid_of(v, dtype)
 
 
-def _state_dtype(f):
-    c = f[f"{step_keys(f)[0]}/u_hat"].dtype
-    return torch.float64 if c == np.complex128 else torch.float32
+def read_snapshot(f, step, field="u_hat"):
+    """One recorded field (default the spectral state) of snapshot `step`, as NumPy."""
+    with _view(f) as v:
+        return v.field(int(step), field)
 
 
-def read_snapshot(f, step):
-    """Spectral state of snapshot `step` (NumPy)."""
-    return np.asarray(f[f"{step_key(step)}/u_hat"])
+def load_run(source):
+    """Namespace(grid, nu, dtype, steps) describing a recorded run without
+    reading any field array; ValueError when it holds no snapshot."""
+    with _view(source) as v:
+        steps = v.steps()
+        if not steps:
+            raise ValueError(f"no snapshots found in {_describe(source)}")
+        _, cdtype = v.field_info(steps[0])
+        dtype = torch.float64 if cdtype == np.complex128 else torch.float32
+        grid, nu = _grid_of(v, dtype)
+        return SimpleNamespace(grid=grid, nu=nu, dtype=dtype, steps=steps)
 
 
-def load_run(path):
-    """Namespace(grid, nu, dtype, steps) of the file at `path` (no field
-    arrays are read); errors if it holds no snapshot."""
-    h5py = _h5py()
-    with h5py.File(path, "r") as f:
-        keys = step_keys(f)
-        if not keys:
-            raise ValueError(f"no snapshots found in {path}")
-        dtype = _state_dtype(f)
-        grid, nu = read_grid(f, dtype)
-        return SimpleNamespace(grid=grid, nu=nu, dtype=dtype, steps=[step_num(k) for k in keys])
-
-
-def read_series(path_or_file, name):
-    """series/<name> as a dict of NumPy arrays (t plus `values` or one
-    vector per key), or None if the file has no such group."""
-    h5py = _h5py()
-    if isinstance(path_or_file, h5py.File):
-        return _read_series(path_or_file, name)
-    with h5py.File(path_or_file, "r") as f:
-        return _read_series(f, name)
-
-
-def _read_series(f, name):
-    grp = series_key(name)
-    if grp not in f:
-        return None
-    return {k: np.asarray(f[grp][k]) for k in f[grp].keys()}
+def read_series(source, name):
+    """The series recorded under `name` (save_series) as a dict of NumPy
+    arrays, `t` plus `values` or one vector per recorded key; None when the
+    run has no series of that name."""
+    with _view(source) as v:
+        return v.series(name)

@@ -108,3 +108,24 @@ def test_hdf5_files(tmp_path):
     s = h3.read_series(path, "E")
     assert np.allclose(s["t"], E.times) and np.allclose(s["values"], E.values)
     assert h3.read_series(path, "energy_budget") is None
+
+
+def test_recorded_run_read_back_through_the_report_readers():
+    """evolve with FieldWriter and Diagnostic both recording into one
+    MemorySink; load_run / read_snapshot / read_series read the run back
+    (the same readers serve HDF5 files)."""
+    from hit3d.report import read_snapshot
+    g = h3.make_grid(N, dtype=torch.float64)
+    sink = h3.MemorySink()
+    E = h3.Diagnostic(lambda s: h3.energy(s.u_hat, s.grid), every=10, path=sink, name="E")
+    B = h3.Diagnostic(h3.energy_budget_cb, every=25, path=sink, name="energy_budget")
+    u = run_beltrami(g, (h3.FieldWriter(sink=sink, every=50), E, B))
+    run = h3.load_run(sink)
+    assert run.steps == [0, 50, 100] and run.nu == NU and run.dtype == torch.float64
+    assert (run.grid.Nx, run.grid.Ny, run.grid.Nz) == (N, N, N)
+    assert np.array_equal(read_snapshot(sink, 100), u.cpu().numpy())
+    s = h3.read_series(sink, "E")
+    assert np.allclose(s["t"], E.times) and np.allclose(s["values"], E.values)
+    b = h3.read_series(sink, "energy_budget")
+    assert set(b) == {"t", "E", "eps", "P"} and len(b["t"]) == 5
+    assert np.allclose(b["eps"], 2 * NU * np.asarray(b["E"]), rtol=1e-10)   # Beltrami: Omega = E

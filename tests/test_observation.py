@@ -136,3 +136,63 @@ def test_gauss_newton_calibration_step():
     # a step larger than the clamp is cut to it
     best2, pts2 = les_experiment.gauss_newton(resid, [0.0, 0.0], (0.1, 0.3), (0.02, 0.6), 1)
     assert abs(pts2[-1][0][0] - 0.02) < 1e-12
+
+
+# --- report readers over a memory-held run (reference tests/test_report_schema_labels.py:45-81) ---
+
+
+def _recorded_run(with_meta=True, dtype=np.complex64):
+    sink = MemorySink()
+    meta = {"Nx": 16, "Ny": 8, "Nz": 32, "Lx": 3.0, "Ly": 2.0, "Lz": 1.0, "nu": 0.01}
+    rng = np.random.default_rng(0)
+    for step in (50, 0, 100):                      # written out of order
+        u = (rng.standard_normal((3, 16, 8, 17)) + 1j * rng.standard_normal((3, 16, 8, 17))).astype(dtype)
+        sink.write(step, step * 1e-3, {"u_hat": u}, meta if with_meta else {})
+    if not with_meta:
+        sink.meta = None
+    return sink
+
+
+def test_load_run_from_memory_sink():
+    from hit3d.report import read_grid, read_snapshot, step_keys
+    sink = _recorded_run()
+    run = hit3d.load_run(sink)
+    assert run.steps == [0, 50, 100] and run.nu == 0.01 and run.dtype == torch.float32
+    g = run.grid
+    assert (g.Nx, g.Ny, g.Nz, g.Lx, g.Ly, g.Lz) == (16, 8, 32, 3.0, 2.0, 1.0)
+    assert step_keys(sink) == ["step_00000000", "step_00000050", "step_00000100"]
+    assert np.array_equal(read_snapshot(sink, 50), sink.snapshots[50]["u_hat"])
+    g64, nu = read_grid(sink, torch.float64)
+    assert g64.dtype == torch.float64 and nu == 0.01
+    assert hit3d.load_run(_recorded_run(dtype=np.complex128)).dtype == torch.float64
+    assert hit3d.read_series(sink, "energy_budget") is None
+
+
+def test_load_run_without_grid_metadata_warns_and_assumes_2pi_cube():
+    sink = _recorded_run(with_meta=False)
+    with pytest.warns(UserWarning, match="no grid metadata"):
+        run = hit3d.load_run(sink)
+    g = run.grid
+    assert (g.Nx, g.Ny, g.Nz) == (16, 8, 32) and run.nu is None
+    assert abs(g.Lx - 2 * math.pi) < 1e-15
+
+
+def test_load_run_of_an_empty_run_errors():
+    with pytest.raises(ValueError, match="no snapshots found"):
+        hit3d.load_run(MemorySink())
+
+
+def test_series_round_trip_through_a_memory_sink():
+    sink = MemorySink()
+    d = Diagnostic(lambda s: 0.0, every=1, path=sink, name="E")
+    d.times, d.values = [0.0, 0.1, 0.2], [1.0, 0.9, 0.8]
+    d.finalize()
+    b = Diagnostic(lambda s: 0.0, every=1, path=sink, name="energy_budget")
+    b.times, b.values = [0.0, 0.5], [{"E": 1.0, "eps": 0.1, "P": 0.0}, {"E": 0.9, "eps": 0.2, "P": 0.0}]
+    b.finalize()
+    s = hit3d.read_series(sink, "E")
+    assert set(s) == {"t", "values"} and np.allclose(s["values"], [1.0, 0.9, 0.8])
+    s = hit3d.read_series(sink, "energy_budget")
+    assert set(s) == {"t", "E", "eps", "P"} and np.allclose(s["eps"], [0.1, 0.2])
+    with pytest.raises(ValueError, match="already exists"):
+        hit3d.save_series(sink, "E", d)
