@@ -171,13 +171,15 @@ size_t reduce_bytes(const hit3d_grid& gd, double** partials, double** scalars, v
   return c.off;
 }
 
-size_t spectrum_bytes(const hit3d_grid& gd, double** partials, void* work) {
+size_t spectrum_bytes(const hit3d_grid& gd, double** partials, double** mid, void* work) {
   Carve c;
   c.base = static_cast<char*>(work);
   const Geo g = make_geo(gd, false);
-  double* p = static_cast<double*>(
-      c.take(size_t(h3::spectrum_blocks(g)) * shell_bins(gd) * sizeof(double)));
+  const int nbins = int(shell_bins(gd));
+  double* p = static_cast<double*>(c.take(size_t(h3::spectrum_blocks(g)) * nbins * sizeof(double)));
+  double* m = static_cast<double*>(c.take(h3::spectrum_mid_doubles(nbins) * sizeof(double)));
   if (partials) *partials = p;
+  if (mid) *mid = m;
   return c.off;
 }
 
@@ -203,16 +205,6 @@ __global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ par
     __syncthreads();
   }
   if (threadIdx.x == 0) h3::finalize_write(fa, sh[0][0], sh[1][0], sh[2][0], scalars);
-}
-
-__global__ __launch_bounds__(256) void k_spectrum_final(const double* __restrict__ partials,
-                                                        int nblocks, int nbins, double scale,
-                                                        double* __restrict__ out) {
-  const int b = blockIdx.x * 256 + threadIdx.x;
-  if (b >= nbins) return;
-  h3::KSum s;
-  for (int i = 0; i < nblocks; ++i) s.add(partials[int64_t(i) * nbins + b]);
-  out[b] = s.value() * scale;
 }
 
 // coef = [A, D, 2E] from k_hold_partials' sums: D = dE/dt of du, 2E = sum w|u|^2 / Ntot^2,
@@ -788,7 +780,7 @@ int hit3d_workspace_size(const hit3d_grid* grid, int32_t op, int64_t ncomp, int3
       return HIT3D_OK;
     }
     case HIT3D_OP_SPECTRUM:
-      *bytes = spectrum_bytes(*grid, nullptr, nullptr);
+      *bytes = spectrum_bytes(*grid, nullptr, nullptr, nullptr);
       return HIT3D_OK;
     case HIT3D_OP_REDUCE:
       *bytes = reduce_bytes(*grid, nullptr, nullptr, nullptr);
@@ -944,28 +936,28 @@ int hit3d_rk4_steps(const hit3d_grid* grid, void* u_hat, double nu, double dt, i
 }
 
 int hit3d_shell_spectrum(const hit3d_grid* grid, const void* u_hat, int32_t component,
-                         double* out, void* work, size_t work_bytes, void* stream) {
+                         int32_t dealiased, double* out, void* work, size_t work_bytes,
+                         void* stream) {
   H3_TRY(check_grid(grid));
   H3_TRY(check_ptr(u_hat, "u_hat"));
   H3_TRY(check_ptr(out, "out"));
   if (component < -1 || component > 2) return fail(HIT3D_ERR_INVALID, "component must be -1..2");
-  H3_TRY(check_work(work, work_bytes, spectrum_bytes(*grid, nullptr, nullptr)));
-  double* partials;
-  spectrum_bytes(*grid, &partials, work);
+  H3_TRY(check_work(work, work_bytes, spectrum_bytes(*grid, nullptr, nullptr, nullptr)));
+  double *partials, *mid;
+  spectrum_bytes(*grid, &partials, &mid, work);
   const Geo g = make_geo(*grid, false);
   const int64_t fs = int64_t(g.nx) * g.ny * g.nzh;
   const int nbins = int(shell_bins(*grid));
+  const int box = dealiased != 0;
   hipStream_t s = static_cast<hipStream_t>(stream);
   int nb = 0;
   if (grid->precision == 4)
     H3_HIP(h3::launch_spectrum_partials<float>(static_cast<const cpx<float>*>(u_hat), fs,
-                                               component, g, nbins, partials, &nb, s));
+                                               component, g, box, nbins, partials, &nb, s));
   else
     H3_HIP(h3::launch_spectrum_partials<double>(static_cast<const cpx<double>*>(u_hat), fs,
-                                                component, g, nbins, partials, &nb, s));
-  hipLaunchKernelGGL(k_spectrum_final, dim3((nbins + 255) / 256), dim3(256), 0, s, partials, nb,
-                     nbins, 0.5 * ntot2_inv(*grid), out);
-  H3_HIP(hipGetLastError());
+                                                component, g, box, nbins, partials, &nb, s));
+  H3_HIP(h3::launch_spectrum_merge(partials, nb, nbins, mid, 0.5 * ntot2_inv(*grid), out, s));
   return HIT3D_OK;
 }
 
@@ -1024,6 +1016,16 @@ int hit3d_alias_energy(const hit3d_grid* grid, const void* u_hat, double* out, v
                      scalars);
   H3_HIP(hipGetLastError());
   H3_HIP(hipMemcpyAsync(out, scalars + 1, sizeof(double), hipMemcpyDeviceToDevice, s));
+  return HIT3D_OK;
+}
+
+int hit3d_stream_copy(const void* src, void* dst, size_t bytes, int32_t nontemporal,
+                      void* stream) {
+  H3_TRY(check_ptr(src, "src"));
+  H3_TRY(check_ptr(dst, "dst"));
+  if (bytes % 16 != 0) return fail(HIT3D_ERR_INVALID, "bytes must be a multiple of 16");
+  H3_HIP(h3::launch_stream_copy(src, dst, int64_t(bytes / 16), nontemporal,
+                                static_cast<hipStream_t>(stream)));
   return HIT3D_OK;
 }
 

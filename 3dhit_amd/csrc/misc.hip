@@ -1,5 +1,7 @@
 // Non-FFT kernels. Full-layout kernels take a dense Geo (kx = nx, ky = ny,
 // kz = nz/2+1, jp = ny*kz), under which the compact layout is the API layout.
+#include <cmath>
+
 #include "misc.hpp"
 
 namespace h3 {
@@ -119,75 +121,223 @@ __global__ __launch_bounds__(kBlock) void k_alias_partials(const cpx<T>* __restr
 }
 
 // ---- shell spectrum ----------------------------------------------------------
-// Each wave walks whole kz lines. Along a line the shell index is
-// non-decreasing in kz, so a segmented suffix sum across the 64 lanes gives
-// each shell's run total in its head lane, which adds it to the wave's own
-// LDS histogram: one writer per bin at a time, in a fixed order, so the
-// result is bit-reproducible. Waves then merge in order into per-block rows.
+// Each wave walks whole kz lines of the full-layout state. All loads of a
+// line (up to kSpecChunks chunks of 64 modes, every component) are issued
+// before any of them is used, so a wave keeps a whole line in flight. Along
+// a line the shell index is non-decreasing in kz, so a segmented suffix sum
+// across the 64 lanes gives each shell's run total in its head lane, which
+// adds it to the wave's own LDS histogram (Neumaier sum and compensation):
+// one writer per bin at a time, in a fixed order, so the result is
+// bit-reproducible. The waves of a block merge in order into one row of
+// partials; two fixed-order merge kernels reduce the rows.
+//
+// `box` = 1 reads only the modes inside the per-axis 2/3 box (the caller
+// promises that every other mode is zero: the solver's dealiased states),
+// about 0.3 of the full array.
 
 constexpr int kSpecWaves = kBlock / 64;
+constexpr int kSpecChunks = 9;        // ceil((1024/2 + 1) / 64)
+constexpr int kSpecGroups = 32;       // row groups of the first merge level
 
-int spectrum_blocks(const Geo& g) {
-  const int64_t lines = int64_t(g.kx) * g.ky;
+struct SpecLines {
+  int lx, ly, lz;   // lines along x and y, modes per line
+  int px, py;       // positive-frequency counts of the retained x / y rows
+};
+
+__host__ __device__ inline SpecLines spec_lines(const Geo& g, int box) {
+  SpecLines L;
+  if (box) {
+    L.lx = 2 * g.cx + 1;
+    L.ly = 2 * g.cy + 1;
+    L.lz = g.cz + 1;
+  } else {
+    L.lx = g.nx;
+    L.ly = g.ny;
+    L.lz = g.nzh;
+  }
+  L.px = (L.lx + 1) / 2;
+  L.py = (L.ly + 1) / 2;
+  return L;
+}
+
+int spectrum_blocks(const Geo& g, int box) {
+  const SpecLines L = spec_lines(g, box);
+  const int64_t lines = int64_t(L.lx) * L.ly;
   const int64_t b = (lines + kSpecWaves - 1) / kSpecWaves;
   return int(b < kReduceBlocks ? b : kReduceBlocks);
 }
 
+int spectrum_blocks(const Geo& g) { return spectrum_blocks(g, 0); }
+
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_spectrum_partials(const cpx<T>* __restrict__ st,
                                                               int64_t fs, int component, Geo g,
-                                                              int nbins,
+                                                              int box, int nbins, int nrow,
                                                               double* __restrict__ partials) {
-  extern __shared__ double hist[];  // [kSpecWaves][nbins][2] (sum, compensation)
+  // [kSpecWaves][nbins][2] (sum, compensation): nbins covers every shell the
+  // read modes reach; the partial rows are nrow >= nbins long (zero beyond)
+  extern __shared__ double hist[];
   const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
   for (int i = threadIdx.x; i < kSpecWaves * nbins * 2; i += kBlock) hist[i] = 0.0;
   __syncthreads();
   double* h = hist + wave * nbins * 2;
-  const int64_t lines = int64_t(g.kx) * g.ky;
-  const int f0 = component < 0 ? 0 : component, f1 = component < 0 ? 3 : component + 1;
+  const SpecLines L = spec_lines(g, box);
+  const int64_t lines = int64_t(L.lx) * L.ly;
+  const int f0 = component < 0 ? 0 : component, nf = component < 0 ? 3 : 1;
+  const int nch = (L.lz + 63) / 64;
   for (int64_t l = int64_t(blockIdx.x) * kSpecWaves + wave; l < lines;
        l += int64_t(gridDim.x) * kSpecWaves) {
-    const int ix = int(l / g.ky), iy = int(l - int64_t(ix) * g.ky);
-    const T kxv = wavenumber<T>(g.k0x, freq_of(ix, g.kx, g.px));
-    const T kyv = wavenumber<T>(g.k0y, freq_of(iy, g.ky, g.py));
-    const int64_t base = int64_t(ix) * g.jp + int64_t(iy) * g.kzs;
-    for (int z0 = 0; z0 < g.kz; z0 += 64) {
-      const int iz = z0 + lane;
-      double v = 0.0;
-      int key = 0x7fffffff;
-      if (iz < g.kz) {
-        for (int f = f0; f < f1; ++f) {
-          const cpx<T> u = st[f * fs + base + iz];
-          v += double(u.x) * u.x + double(u.y) * u.y;
-        }
-        v *= hermitian_weight(iz, g.nz);
-        key = shell_of<T>(k2_of(kxv, kyv, wavenumber<T>(g.k0z, iz)), g.k0x);
-      }
+    const int ix = int(l / L.ly), iy = int(l - int64_t(ix) * L.ly);
+    const int x = box ? row_of_compact(ix, g.nx, L.lx, L.px) : ix;
+    const int y = box ? row_of_compact(iy, g.ny, L.ly, L.py) : iy;
+    const T kxv = wavenumber<T>(g.k0x, x < (g.nx + 1) / 2 ? x : x - g.nx);
+    const T kyv = wavenumber<T>(g.k0y, y < (g.ny + 1) / 2 ? y : y - g.ny);
+    const cpx<T>* line = st + (int64_t(x) * g.ny + y) * g.nzh + int64_t(f0) * fs;
+    // every load of the line first
+    cpx<T> u[kSpecChunks][3];
 #pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const double ov = __shfl_down(v, off);
-        const int ok = __shfl_down(key, off);
-        if (lane + off < 64 && ok == key) v += ov;
-      }
-      const int prev = __shfl_up(key, 1);
-      const bool head = lane == 0 || prev != key;
-      if (head && key < nbins) {
-        // Neumaier update of bin `key`
-        const double s = h[2 * key], t = s + v;
-        h[2 * key + 1] += fabs(s) >= fabs(v) ? (s - t) + v : (v - t) + s;
-        h[2 * key] = t;
+    for (int c = 0; c < kSpecChunks; ++c) {
+      const int iz = 64 * c + lane;
+      const bool ok = c < nch && iz < L.lz;
+#pragma unroll
+      for (int f = 0; f < 3; ++f)
+        u[c][f] = (ok && f < nf) ? line[int64_t(f) * fs + iz] : czero<T>();
+    }
+#pragma unroll
+    for (int c = 0; c < kSpecChunks; ++c) {
+      if (c < nch) {  // uniform across the wave
+        const int iz = 64 * c + lane;
+        double v = 0.0;
+        int key = 0x7fffffff;
+        if (iz < L.lz) {
+#pragma unroll
+          for (int f = 0; f < 3; ++f) v += double(u[c][f].x) * u[c][f].x + double(u[c][f].y) * u[c][f].y;
+          v *= hermitian_weight(iz, g.nz);
+          key = shell_of<T>(k2_of(kxv, kyv, wavenumber<T>(g.k0z, iz)), g.k0x);
+        }
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          const double ov = __shfl_down(v, off);
+          const int ok = __shfl_down(key, off);
+          if (lane + off < 64 && ok == key) v += ov;
+        }
+        const int prev = __shfl_up(key, 1);
+        const bool head = lane == 0 || prev != key;
+        if (head && key < nbins) {
+          // Neumaier update of bin `key`
+          const double s = h[2 * key], t = s + v;
+          h[2 * key + 1] += fabs(s) >= fabs(v) ? (s - t) + v : (v - t) + s;
+          h[2 * key] = t;
+        }
       }
     }
   }
   __syncthreads();
-  for (int b = threadIdx.x; b < nbins; b += kBlock) {
+  for (int b = threadIdx.x; b < nrow; b += kBlock) {
     KSum acc;
-    for (int w = 0; w < kSpecWaves; ++w) {
-      acc.add(hist[(w * nbins + b) * 2]);
-      acc.add(hist[(w * nbins + b) * 2 + 1]);
-    }
-    partials[int64_t(blockIdx.x) * nbins + b] = acc.value();
+    if (b < nbins)
+      for (int w = 0; w < kSpecWaves; ++w) {
+        acc.add(hist[(w * nbins + b) * 2]);
+        acc.add(hist[(w * nbins + b) * 2 + 1]);
+      }
+    partials[int64_t(blockIdx.x) * nrow + b] = acc.value();
   }
+}
+
+// First merge level: block (bx, gy) sums rows gy, gy + kSpecGroups, ... of
+// bins [64 bx, 64 bx + 64): each wave takes every kSpecWaves-th of those
+// rows (Neumaier), the waves combine in order, and the group's (sum,
+// compensation) pair goes to mid[gy][bin].
+__global__ __launch_bounds__(kBlock) void k_spectrum_merge_rows(const double* __restrict__ partials,
+                                                                int nrows, int nbins,
+                                                                double* __restrict__ mid) {
+  __shared__ double sh[kSpecWaves][64][2];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
+  const int b = blockIdx.x * 64 + lane;
+  const int gy = blockIdx.y;
+  KSum acc;
+  if (b < nbins)
+    for (int r = gy + kSpecGroups * wave; r < nrows; r += kSpecGroups * kSpecWaves)
+      acc.add(partials[int64_t(r) * nbins + b]);
+  sh[wave][lane][0] = acc.s;
+  sh[wave][lane][1] = acc.c;
+  __syncthreads();
+  if (wave == 0 && b < nbins) {
+    KSum t;
+    for (int w = 0; w < kSpecWaves; ++w) {
+      t.add(sh[w][lane][0]);
+      t.add(sh[w][lane][1]);
+    }
+    mid[(int64_t(gy) * nbins + b) * 2] = t.s;
+    mid[(int64_t(gy) * nbins + b) * 2 + 1] = t.c;
+  }
+}
+
+// Second level: one thread per bin adds the kSpecGroups pairs in order.
+__global__ __launch_bounds__(kBlock) void k_spectrum_merge_groups(const double* __restrict__ mid,
+                                                                  int nbins, double scale,
+                                                                  double* __restrict__ out) {
+  const int b = blockIdx.x * kBlock + threadIdx.x;
+  if (b >= nbins) return;
+  KSum t;
+  for (int gy = 0; gy < kSpecGroups; ++gy) {
+    t.add(mid[(int64_t(gy) * nbins + b) * 2]);
+    t.add(mid[(int64_t(gy) * nbins + b) * 2 + 1]);
+  }
+  out[b] = t.value() * scale;
+}
+
+size_t spectrum_mid_doubles(int nbins) { return size_t(kSpecGroups) * nbins * 2; }
+
+hipError_t launch_spectrum_merge(const double* partials, int nrows, int nbins, double* mid,
+                                 double scale, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_spectrum_merge_rows, dim3((nbins + 63) / 64, kSpecGroups), dim3(kBlock), 0, s,
+                     partials, nrows, nbins, mid);
+  hipLaunchKernelGGL(k_spectrum_merge_groups, dim3((nbins + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                     mid, nbins, scale, out);
+  return hipGetLastError();
+}
+
+// ---- streaming copy (bandwidth reference) --------------------------------------
+// dst = src for n16 16-byte words, U words per thread with all loads before
+// the stores; NT marks both as non-temporal (streaming). The measured rate
+// is the attainable HBM read + write bandwidth the pass kernels compare to.
+
+typedef unsigned int StreamWord __attribute__((ext_vector_type(4)));
+
+template <int U, bool NT>
+__global__ __launch_bounds__(kBlock) void k_stream_copy(const StreamWord* __restrict__ src,
+                                                        StreamWord* __restrict__ dst, int64_t n16) {
+  const int64_t base = int64_t(blockIdx.x) * U * kBlock + threadIdx.x;
+  StreamWord v[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int64_t i = base + int64_t(u) * kBlock;
+    if (i < n16) v[u] = NT ? __builtin_nontemporal_load(src + i) : src[i];
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int64_t i = base + int64_t(u) * kBlock;
+    if (i < n16) {
+      if constexpr (NT) __builtin_nontemporal_store(v[u], dst + i);
+      else dst[i] = v[u];
+    }
+  }
+}
+
+hipError_t launch_stream_copy(const void* src, void* dst, int64_t n16, int nontemporal,
+                              hipStream_t s) {
+  if (n16 <= 0) return hipSuccess;
+  const auto* in = static_cast<const StreamWord*>(src);
+  auto* out = static_cast<StreamWord*>(dst);
+  if (nontemporal) {
+    const int64_t nb = (n16 + 4 * kBlock - 1) / (4 * kBlock);
+    hipLaunchKernelGGL((k_stream_copy<4, true>), dim3(unsigned(nb)), dim3(kBlock), 0, s, in, out, n16);
+  } else {
+    const int64_t nb = (n16 + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL((k_stream_copy<1, false>), dim3(unsigned(nb)), dim3(kBlock), 0, s, in, out, n16);
+  }
+  return hipGetLastError();
 }
 
 // ---- Philox4x64-10 uniform noise -------------------------------------------
@@ -508,12 +658,22 @@ hipError_t launch_alias_partials(const cpx<T>* full, int64_t fs, const Geo& g, d
 
 template <class T>
 hipError_t launch_spectrum_partials(const cpx<T>* st, int64_t fs, int component, const Geo& g,
-                                    int nbins, double* partials, int* nblocks, hipStream_t s) {
-  const int nb = spectrum_blocks(g);
+                                    int box, int nbins, double* partials, int* nblocks,
+                                    hipStream_t s) {
+  if ((g.nz / 2 + 1 + 63) / 64 > kSpecChunks) return hipErrorInvalidValue;
+  const int nb = spectrum_blocks(g, box);
   *nblocks = nb;
-  const size_t shmem = size_t(kSpecWaves) * nbins * 2 * sizeof(double);
+  // the box read reaches shells up to that of its corner (cx, cy, cz); two
+  // bins of margin cover the rounding of the shell rule in the state precision
+  int used = nbins;
+  if (box) {
+    const double kx = g.cx * g.k0x, ky = g.cy * g.k0y, kz = g.cz * g.k0z;
+    const int corner = int(std::floor(std::sqrt(kx * kx + ky * ky + kz * kz) / g.k0x + 0.5));
+    used = corner + 3 < nbins ? corner + 3 : nbins;
+  }
+  const size_t shmem = size_t(kSpecWaves) * used * 2 * sizeof(double);
   hipLaunchKernelGGL((k_spectrum_partials<T>), dim3(nb), dim3(kBlock), shmem, s, st, fs, component,
-                     g, nbins, partials);
+                     g, box, used, nbins, partials);
   return hipGetLastError();
 }
 
@@ -648,7 +808,7 @@ H3_INSTANTIATE_ADJ(double)
   template hipError_t launch_alias_partials<T>(const cpx<T>*, int64_t, const Geo&, double*,       \
                                                hipStream_t);                                       \
   template hipError_t launch_spectrum_partials<T>(const cpx<T>*, int64_t, int, const Geo&, int,   \
-                                                  double*, int*, hipStream_t);                     \
+                                                  int, double*, int*, hipStream_t);                \
   template hipError_t launch_philox_uniform<T>(T*, int64_t, uint64_t, hipStream_t);               \
   template hipError_t launch_dealias_project<T>(const cpx<T>*, cpx<T>*, int64_t, int, const Geo&, \
                                                 int, int, hipStream_t);                            \

@@ -23,11 +23,21 @@ hipError_t launch_quad_partials(const cpx<T>* st, int64_t fs, const Geo& g, T km
 template <class T>
 hipError_t launch_alias_partials(const cpx<T>* full, int64_t fs, const Geo& g, double* partials,
                                  hipStream_t s);
-// partials[nblocks][nbins]; returns nblocks through *nblocks
+// partials[nblocks][nbins] of a full-layout state (box = 1: only the modes
+// inside the 2/3 box are read); returns nblocks through *nblocks
 template <class T>
 hipError_t launch_spectrum_partials(const cpx<T>* st, int64_t fs, int component, const Geo& g,
-                                    int nbins, double* partials, int* nblocks, hipStream_t s);
-int spectrum_blocks(const Geo& g);
+                                    int box, int nbins, double* partials, int* nblocks,
+                                    hipStream_t s);
+int spectrum_blocks(const Geo& g);  // the largest count either read takes
+// out[b] = scale * (fixed-order sum over the nrows partial rows of bin b);
+// mid holds spectrum_mid_doubles(nbins) doubles
+size_t spectrum_mid_doubles(int nbins);
+hipError_t launch_spectrum_merge(const double* partials, int nrows, int nbins, double* mid,
+                                 double scale, double* out, hipStream_t s);
+// dst = src, n16 16-byte words (both 16-byte aligned); nontemporal: streaming hints
+hipError_t launch_stream_copy(const void* src, void* dst, int64_t n16, int nontemporal,
+                              hipStream_t s);
 template <class T>
 hipError_t launch_philox_uniform(T* out, int64_t n, uint64_t seed, hipStream_t s);
 template <class T>

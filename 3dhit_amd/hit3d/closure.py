@@ -145,14 +145,17 @@ class ClosureNet:
         self._shape = None
 
     def spectral_shape(self):
-        """exp(tilt (|k|/kc - 1/2)) on the net's grid (real, [Nx][Ny][Nz/2+1])."""
+        """exp(tilt (|k|/kc - 1/2)) on the net's grid (real, [Nx][Ny][Nz/2+1]),
+        |k| and the cut kc both physical wavenumbers."""
         if self._shape is None:
             self._shape = torch.empty(self.grid.k2.shape, dtype=self.grid.dtype, device=self.grid.device)
             self._fill_shape()
         return self._shape
 
     def _fill_shape(self):
-        kc = float(min(self.grid.Nx, self.grid.Ny, self.grid.Nz) // 3)
+        # kc: the 2/3 cut as a wavenumber, (2 pi / L_i) (N_i // 3), smallest over the axes
+        g = self.grid
+        kc = min(2 * math.pi / L * (N // 3) for N, L in ((g.Nx, g.Lx), (g.Ny, g.Ly), (g.Nz, g.Lz)))
         with torch.no_grad():
             self._shape.copy_(torch.exp(self.tilt * (torch.sqrt(self.grid.k2) / kc - 0.5)))
 

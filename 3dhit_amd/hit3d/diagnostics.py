@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .grid import check_spectral, stream
+from .grid import DEALIASED_MARK, check_spectral, stream
 
 __all__ = [
     "energy", "enstrophy", "dissipation", "energy_spectrum", "component_spectra",
@@ -55,12 +55,15 @@ def shell_bins(grid):
 
 
 def shell_spectrum(u_hat, grid, component=-1):
-    """All bins 0..smax (bin 0 = the k = 0 mode) as a device float64 tensor."""
+    """All bins 0..smax (bin 0 = the k = 0 mode) as a device float64 tensor.
+    A state the library marked as dealiased (see grid.mark_dealiased) is
+    read only inside the 2/3 box; any other one in full (no device check)."""
     u = check_spectral(u_hat, grid, 3)
     nb = shell_bins(grid)
     out = torch.empty(nb, dtype=torch.float64, device=u.device)
     ws = grid.workspace(_lib.OP_SPECTRUM)
-    _lib.call("hit3d_shell_spectrum", grid.desc, u.data_ptr(), int(component), out.data_ptr(),
+    box = 1 if getattr(u_hat, DEALIASED_MARK, None) == u_hat._version else 0
+    _lib.call("hit3d_shell_spectrum", grid.desc, u.data_ptr(), int(component), box, out.data_ptr(),
               ws.data_ptr(), ws.numel(), stream())
     return out
 

@@ -54,6 +54,7 @@ def random_field(seed, grid, spectrum=default_spectrum):
     u_hat = to_spectral(uniform_noise(seed, grid), grid)
     _lib.call("hit3d_dealias_project", grid.desc, u_hat.data_ptr(), u_hat.data_ptr(), 3, 1, 1,
               stream())
+    mark_dealiased(u_hat)                                    # the shell rescale keeps it so
     Es = shell_spectrum(u_hat, grid).cpu().numpy()           # bins 0..smax, FP64
     npd = np.float32 if grid.dtype == torch.float32 else np.float64
     k0 = 2 * np.pi / grid.Lx

@@ -5,9 +5,8 @@
    on the LES grid: the filtered state ubar and the exact subgrid stress
    tau_hat (kept in device memory).
 2. A-priori training of the closure net (MSE on the deviatoric tau,
-   channels normalised)
-   on the first `train_frac` of the snapshots; the loss on the rest is
-   reported.
+   channels normalised) on all but the last `test_frac` of the training
+   record's snapshots; the loss on those is reported.
 3. A-posteriori training, gradients through the LES steps (the solver's
    hand-written RHS adjoint plus the net's backward kernels) reaching the
    net and the scalar gain of the closure force: first `windows`
@@ -21,15 +20,20 @@
    that the mean spectrum of a `calib_steps` LES matches the training
    snapshots' mean on shells 5..kcut. The LES evolves exactly the
    filter's modes (|k| <= kcut, a sphere inside the solver's 2/3 cube).
-4. Evaluation: an LES from the first snapshot, with and without the
-   closure; the time-averaged shell spectrum E(k), k = 1..kcut, over the
-   DNS record's time window against the filtered DNS spectrum over the
-   same times. The closure is accepted when max_k |E_les/E_dns - 1| <=
-   tol; otherwise the result falls back to the plain DNS. Reported beside
-   it: the same statistic over `long_factor` times the record, and the DNS
-   record's own half-versus-half spectrum difference (its sampling noise).
-   GPU time of the DNS and of the LES over the same simulated time is
-   measured with HIP events.
+4. Evaluation on held-out data: the DNS continues past the training
+   record for `holdout_time` more time units, keeping only the filtered
+   spectrum of every snapshot (and the filtered state where the record
+   starts). An LES from that first held-out state, with and without the
+   closure, is compared with it: time-averaged shell spectrum E(k),
+   k = 1..kcut, over the held-out window. The closure is accepted when
+   max_k |E_les/E_dns - 1| <= tol on this held-out statistic; otherwise the
+   result falls back to the plain DNS. Reported beside it: the in-sample
+   statistic (an LES from the first training snapshot against the training
+   record, the data the gain and tilt were fitted to), the same LES
+   continued to `long_factor` times the held-out window against the
+   held-out mean, and the DNS's own record-to-record difference (training
+   mean against held-out mean: its sampling noise). GPU time of the DNS and
+   of the LES over the same simulated time is measured with HIP events.
 
 The LES forcing (HoldResolvedEnergy) is f = A(u) u with A set at every
 stage so that the resolved energy relaxes to the filtered DNS's mean
@@ -55,8 +59,8 @@ from .fields import config_spectrum, random_field
 from .forcing import LundgrenForcing
 from .grid import make_grid, to_physical
 from .integrators import rollout
-from .les import (HoldResolvedEnergy, LesStepGraph, check_les_state, les_rollout, les_step,
-                  sphere_mask, subgrid_stress, velocity_gradient)
+from .les import (HoldResolvedEnergy, LesStepGraph, check_les_state, filter_state, les_rollout,
+                  les_step, sphere_mask, subgrid_stress, velocity_gradient)
 from .navierstokes import NSParams, make_ns_rhs
 
 
@@ -83,7 +87,7 @@ def _e_omega(u, grid):
     return float(s[0]), float(s[2])
 
 
-def generate(fine_n, les_n, kcut, nu, dt_dns, stride, spinup_time, nsnap, seed):
+def generate(fine_n, les_n, kcut, nu, dt_dns, stride, spinup_time, nsnap, seed, holdout_time=0.0):
     fine = make_grid(fine_n, dtype=torch.float32)
     coarse = make_grid(les_n, dtype=torch.float32)
     rhs = make_ns_rhs(fine)
@@ -119,10 +123,28 @@ def generate(fine_n, les_n, kcut, nu, dt_dns, stride, spinup_time, nsnap, seed):
         if (s + 1) % max(1, nsnap // 20) == 0:
             _log(f"  snapshot {s + 1}/{nsnap}: E = {E:.4f}, A = {coef[s]:.4f}, "
                  f"DNS {1e3 * dns_steps_s / ((s + 1) * stride):.2f} ms/step, stress {1e3 * stress_s / (s + 1):.1f} ms")
+    # held-out record: the same DNS continued, the filtered spectrum of every
+    # snapshot (no stresses), and the filtered state it starts from
+    nhold = int(round(holdout_time / (stride * dt_dns)))
+    kmax = int(kcut)
+    hold_spectra = np.empty((nhold, kmax))
+    hold_start, hold_e = None, np.empty(max(nhold, 1))
+    t1 = t0 + nsnap * stride * dt_dns
+    for s in range(nhold):
+        u, sec = _timed(lambda: rollout(rhs, u, params, dt_dns, stride, t1 + s * stride * dt_dns))
+        dns_steps_s += sec
+        ub = filter_state(u, fine, coarse, kcut)
+        if s == 0:
+            hold_start = ub.clone()
+        hold_spectra[s] = _spectrum(ub, coarse, kmax)
+        hold_e[s] = _e_omega(ub, coarse)[0]
+        if (s + 1) % max(1, nhold // 10) == 0:
+            _log(f"  held-out snapshot {s + 1}/{nhold}, DNS {1e3 * dns_steps_s / ((nsnap + s + 1) * stride):.2f} ms/step")
     del u
     return {"fine": fine, "coarse": coarse, "ubar": ubar, "tau": tau, "coef": coef, "e_res": e_res,
-            "dns_ms_per_step": 1e3 * dns_steps_s / (nsnap * stride),
-            "stress_ms": 1e3 * stress_s / nsnap, "t_start": t0}
+            "dns_ms_per_step": 1e3 * dns_steps_s / ((nsnap + nhold) * stride),
+            "stress_ms": 1e3 * stress_s / nsnap, "t_start": t0,
+            "hold_start": hold_start, "hold_spectra": hold_spectra, "hold_energy": hold_e[:nhold]}
 
 
 def _deviatoric(tau):
@@ -299,33 +321,32 @@ def calibrate_closure(net, data, params, dt, kmax, mask, target, steps=2000, ite
     return hist
 
 
-def evaluate(net, data, params, dt, steps, kmax, mask, every=10, long_steps=0):
-    """LES of max(steps, long_steps) steps from the first snapshot. The
-    acceptance statistic pairs LES step n with snapshot n over the first
-    `steps` steps (the DNS record's time window); the long run's mean
-    spectrum against the whole record's mean is reported beside it."""
-    coarse = data["coarse"]
-    rhs = make_ns_rhs(coarse)
-    u0 = data["ubar"][0].clone()
-    check_les_state(u0, coarse)
+def evaluate(net, grid, u0, ref_spectra, params, dt, steps, kmax, mask, every=10, long_steps=0):
+    """LES of max(steps, long_steps) steps from u0 against a filtered DNS
+    record whose row n is the spectrum (shells 1..kmax) at LES step n (row 0
+    = u0). The statistic pairs LES step n with row n over the first `steps`
+    steps, every `every`-th; the long run's mean spectrum is compared with
+    the record's mean. Also the cost of one replayed step."""
+    rhs = make_ns_rhs(grid)
+    check_les_state(u0, grid)
     spectra = []
     nrun = max(steps, long_steps)
 
     def obs(n, u):
-        spectra.append(_spectrum(u, coarse, kmax))
+        spectra.append(_spectrum(u, grid, kmax))
     torch.cuda.synchronize()
-    _, sec = _timed(lambda: les_rollout(u0, rhs, params, net, dt, nrun, observe=obs, every=every,
+    _, sec = _timed(lambda: les_rollout(u0.clone(), rhs, params, net, dt, nrun, observe=obs, every=every,
                                         mask=mask))
-    dns = data["spectra"]
-    npair = min(steps // every, (dns.shape[0] - 1) // every)
-    ref = dns[every:every * npair + 1:every].mean(0)
+    npair = min(steps // every, (ref_spectra.shape[0] - 1) // every)
+    ref = ref_spectra[every:every * npair + 1:every].mean(0)
     E = np.mean(spectra[:npair], axis=0)
     err = np.abs(E / ref - 1.0)
     out = {"E_les": E.tolist(), "E_dns_filtered": ref.tolist(), "rel_err": err.tolist(),
-           "max_rel_err": float(err.max()), "seconds_observed": sec}
+           "max_rel_err": float(err.max()), "max_rel_err_k_le_20": float(err[:20].max()),
+           "worst_k": int(np.argmax(err)) + 1, "steps": steps, "seconds_observed": sec}
     if long_steps > steps:
         E_long = np.mean(spectra, axis=0)
-        err_long = np.abs(E_long / dns.mean(0) - 1.0)
+        err_long = np.abs(E_long / ref_spectra.mean(0) - 1.0)
         out.update({"long_steps": nrun, "E_les_long": E_long.tolist(), "rel_err_long": err_long.tolist(),
                     "max_rel_err_long": float(err_long.max())})
     # step cost: replays of the captured step, without observations (each
@@ -341,15 +362,48 @@ def evaluate(net, data, params, dt, steps, kmax, mask, every=10, long_steps=0):
     return out
 
 
-def run(fine_n=512, les_n=64, kcut=21.0, nu=1 / 2000, dt_dns=1e-3, stride=10, spinup_time=10.0,
-        nsnap=4000, train_frac=0.8, epochs=6, lr=1e-3, windows=30, window=50, lr_post=3e-4,
-        attractor_windows=100, spin=150, calib_steps=2000, eval_steps=4000, long_factor=3, tol=0.05,
-        channels=32, relax=1.0, seed=0):
+def save_closure(path, net, target, e_target, config):
+    """The trained closure (weights, scales, gain, tilt, clipping) and the
+    training record's mean spectrum and resolved energy, as one .npz."""
+    st = net.state()
+    arrays = {"target_spectrum": np.asarray(target, dtype=np.float64), "e_target": np.float64(e_target),
+              "in_scale": st["in_scale"].cpu().numpy(), "out_scale": st["out_scale"].cpu().numpy(),
+              "gain": st["gain"].cpu().numpy(), "tilt": np.float64(st["tilt"]),
+              "clip_backscatter": np.bool_(st["clip_backscatter"]),
+              "config": np.bytes_(json.dumps(config))}
+    for i, L in enumerate(st["layers"]):
+        arrays[f"w{i}"] = L["w"].cpu().numpy()
+        arrays[f"b{i}"] = L["b"].cpu().numpy()
+    np.savez_compressed(path, **arrays)
+
+
+def load_closure(path, grid):
+    """(ClosureNet on `grid` with the saved state, target spectrum, resolved
+    energy target, config dict) from save_closure's file."""
+    z = np.load(path)
+    config = json.loads(bytes(z["config"]).decode())
+    net = ClosureNet(grid, channels=int(z["w0"].shape[0]), seed=0)
+    dev = grid.device
+    st = {"layers": [{"w": torch.from_numpy(z[f"w{i}"]).to(dev), "b": torch.from_numpy(z[f"b{i}"]).to(dev)}
+                     for i in range(3)],
+          "in_scale": torch.from_numpy(z["in_scale"]), "out_scale": torch.from_numpy(z["out_scale"]),
+          "gain": torch.from_numpy(z["gain"]).to(dev), "clip_backscatter": bool(z["clip_backscatter"]),
+          "tilt": float(z["tilt"])}
+    net.load(st)
+    return net, np.asarray(z["target_spectrum"]), float(z["e_target"]), config
+
+
+def run(fine_n=512, les_n=64, kcut=21.0, nu=1 / 2000, dt_dns=2e-3, stride=5, spinup_time=10.0,
+        nsnap=2000, holdout_time=40.0, test_frac=0.1, epochs=6, lr=1e-3, windows=30, window=50,
+        lr_post=3e-4, attractor_windows=100, spin=150, calib_steps=2000, long_factor=3, tol=0.05,
+        channels=32, relax=1.0, seed=0, save=None):
+    """The experiment end to end; returns the result dict (and writes the
+    trained closure to `save` as .npz when given)."""
     wall0 = time.time()
-    data = generate(fine_n, les_n, kcut, nu, dt_dns, stride, spinup_time, nsnap, seed)
+    data = generate(fine_n, les_n, kcut, nu, dt_dns, stride, spinup_time, nsnap, seed, holdout_time)
     coarse = data["coarse"]
     V = coarse.Nx * coarse.Ny * coarse.Nz
-    ntrain = int(train_frac * nsnap)
+    ntrain = nsnap - int(round(test_frac * nsnap))
     train, test = np.arange(ntrain), np.arange(ntrain, nsnap)
     net = ClosureNet(coarse, channels=channels, seed=seed)
     sub = np.linspace(0, ntrain - 1, min(ntrain, 50)).astype(int)
@@ -357,52 +411,69 @@ def run(fine_n=512, les_n=64, kcut=21.0, nu=1 / 2000, dt_dns=1e-3, stride=10, sp
     ts = torch.stack([_deviatoric(to_physical(data["tau"][s], coarse).reshape(6, V)).std(dim=1)
                       for s in sub]).mean(0)
     net.set_scales(1.0 / gs, ts)
-    _log(f"closure net: {net.num_parameters()} parameters, a-priori training on {ntrain} snapshots")
+    _log(f"closure net: {net.num_parameters()} parameters, a-priori training on {ntrain} snapshots "
+         f"({len(test)} held out for its loss)")
     apriori = train_apriori(net, data, train, test, epochs, lr, seed)
     dt = stride * dt_dns
     kmax = int(kcut)
     data["spectra"] = np.stack([_spectrum(data["ubar"][s], coarse, kmax) for s in range(nsnap)])
     half = nsnap // 2
     floor = np.abs(data["spectra"][:half].mean(0) / data["spectra"][half:].mean(0) - 1.0)
-    e_target = float(np.mean(data["e_res"][:ntrain]))
+    e_target = float(np.mean(data["e_res"][:nsnap]))
     params = NSParams(nu=nu, forcing=HoldResolvedEnergy(e_target, relax))
-    target = torch.tensor(data["spectra"][:ntrain].mean(0), dtype=torch.float64, device=coarse.device)
+    target_np = data["spectra"].mean(0)
+    target = torch.tensor(target_np, dtype=torch.float64, device=coarse.device)
     _log(f"a-posteriori training: {windows} trajectory + {attractor_windows} attractor windows of "
-         f"{window} LES steps (dt {dt}, resolved energy held at {e_target:.4f}); DNS record's own "
+         f"{window} LES steps (dt {dt}, resolved energy held at {e_target:.4f}); training record's own "
          f"half-vs-half spectrum difference: max {floor.max():.3f}")
     mask = sphere_mask(coarse, kcut)
-    apost = train_aposteriori(net, data, params, dt, train, windows, window, lr_post, seed, kmax, mask,
+    train_all = np.arange(nsnap)
+    apost = train_aposteriori(net, data, params, dt, train_all, windows, window, lr_post, seed, kmax, mask,
                               attractor_windows=attractor_windows, spin=spin, target=target)
     calib = calibrate_closure(net, data, params, dt, kmax, mask, target,
                               steps=min(calib_steps, nsnap - 1)) if calib_steps > 0 else []
-    steps = min(eval_steps, nsnap - 1)   # LES step n pairs with snapshot n
-    _log(f"evaluation: {steps} LES steps paired with the DNS record ({long_factor * steps} in all) "
-         f"with and without the closure")
-    with_cl = evaluate(net, data, params, dt, steps, kmax, mask, long_steps=long_factor * steps)
-    without = evaluate(None, data, params, dt, steps, kmax, mask)
+    hold = data["hold_spectra"]
+    nhold = hold.shape[0]
+    # held-out: LES step n pairs with held-out snapshot n (row 0 = the start state)
+    ref_hold = np.concatenate([_spectrum(data["hold_start"], coarse, kmax)[None], hold[1:]], axis=0)
+    steps = nhold - 1
+    _log(f"evaluation on held-out data: {steps} LES steps from the first held-out snapshot "
+         f"({long_factor * steps} in all), with and without the closure; in-sample check over the "
+         f"training record")
+    with_cl = evaluate(net, coarse, data["hold_start"], ref_hold, params, dt, steps, kmax, mask,
+                       long_steps=long_factor * steps)
+    without = evaluate(None, coarse, data["hold_start"], ref_hold, params, dt, steps, kmax, mask)
+    in_sample = evaluate(net, coarse, data["ubar"][0], data["spectra"], params, dt, nsnap - 1, kmax, mask)
+    record_diff = np.abs(target_np / hold.mean(0) - 1.0)
     dns_s = data["dns_ms_per_step"] * 1e-3 * steps * stride
+    config = {"fine_n": fine_n, "les_n": les_n, "kcut": kcut, "nu": nu, "dt_dns": dt_dns, "dt_les": dt,
+              "stride": stride, "spinup_time": spinup_time, "nsnap": nsnap, "holdout_time": holdout_time,
+              "holdout_snapshots": nhold, "apriori_train_snapshots": ntrain, "epochs": epochs,
+              "windows": windows, "attractor_windows": attractor_windows, "spin": spin, "window": window,
+              "calib_steps": calib_steps, "eval_steps": steps, "channels": channels,
+              "parameters": net.num_parameters(), "forcing_les": params.forcing.label, "tolerance": tol}
+    accepted = with_cl["max_rel_err"] <= tol
     res = {
-        "config": {"fine_n": fine_n, "les_n": les_n, "kcut": kcut, "nu": nu, "dt_dns": dt_dns,
-                   "dt_les": dt, "stride": stride, "spinup_time": spinup_time, "nsnap": nsnap,
-                   "train_snapshots": ntrain, "epochs": epochs, "windows": windows,
-                   "attractor_windows": attractor_windows, "spin": spin, "window": window,
-                   "calib_steps": calib_steps,
-                   "eval_steps": steps, "channels": channels, "parameters": net.num_parameters(),
-                   "forcing_les": params.forcing.label, "tolerance": tol},
-        "dns_half_vs_half_rel_diff": floor.tolist(), "dns_half_vs_half_max": float(floor.max()),
+        "config": config,
+        "training_half_vs_half_rel_diff": floor.tolist(), "training_half_vs_half_max": float(floor.max()),
+        "training_vs_holdout_rel_diff": record_diff.tolist(),
+        "training_vs_holdout_max": float(record_diff.max()),
         "dns_ms_per_step": data["dns_ms_per_step"], "stress_ms_per_snapshot": data["stress_ms"],
         "apriori": apriori, "aposteriori_loss_first_last": [apost[0], float(np.mean(apost[-10:]))],
         "closure_gain": float(net.gain.detach()), "closure_tilt": net.tilt, "closure_calibration": calib,
-        "les": with_cl, "les_no_closure": without,
-        "gpu_seconds": {"dns": dns_s, "les": with_cl["seconds"],
-                        "speedup": dns_s / with_cl["seconds"]},
-        "accepted": with_cl["max_rel_err"] <= tol,
-        "fallback": "none" if with_cl["max_rel_err"] <= tol else "plain DNS (closure error above tolerance)",
+        "les": with_cl, "les_no_closure": without, "les_in_sample": in_sample,
+        "gpu_seconds": {"dns": dns_s, "les": with_cl["seconds"], "speedup": dns_s / with_cl["seconds"]},
+        "accepted": accepted, "acceptance_statistic": "held-out",
+        "fallback": "none" if accepted else "plain DNS (closure error above tolerance)",
         "wall_s": time.time() - wall0,
     }
-    _log(f"E(k) max relative error, k = 1..{kmax}: closure {with_cl['max_rel_err']:.3f} "
-         f"({with_cl.get('max_rel_err_long', float('nan')):.3f} over {with_cl.get('long_steps', steps)} steps), "
-         f"no closure {without['max_rel_err']:.3f}; GPU seconds DNS {dns_s:.1f} vs LES "
+    if save:
+        save_closure(save, net, target_np, e_target, config)
+    _log(f"E(k) max relative error, k = 1..{kmax}, held out: closure {with_cl['max_rel_err']:.3f} "
+         f"(k = {with_cl['worst_k']}; k <= 20: {with_cl['max_rel_err_k_le_20']:.3f}; "
+         f"{with_cl.get('max_rel_err_long', float('nan')):.3f} over {with_cl.get('long_steps', steps)} steps), "
+         f"no closure {without['max_rel_err']:.3f}; in sample {in_sample['max_rel_err']:.3f}; DNS training vs "
+         f"held-out record {record_diff.max():.3f}; GPU seconds DNS {dns_s:.1f} vs LES "
          f"{with_cl['seconds']:.2f} ({res['gpu_seconds']['speedup']:.0f}x)")
     return res
 
@@ -411,6 +482,8 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--quick", action="store_true", help="small smoke configuration (128^3 -> 32^3)")
     ap.add_argument("--out", default=None, help="write the result JSON here")
+    ap.add_argument("--save", default=None, help="write the trained closure (.npz) here")
+    ap.add_argument("--holdout-time", dest="holdout_time", type=float, default=None)
     ap.add_argument("--nsnap", type=int, default=None)
     ap.add_argument("--epochs", type=int, default=None)
     ap.add_argument("--windows", type=int, default=None)
@@ -419,9 +492,10 @@ def main(argv=None):
     kw = {}
     if a.quick:
         kw = dict(fine_n=128, les_n=32, kcut=10.0, nu=1 / 500, dt_dns=2e-3, stride=4, spinup_time=4.0,
-                  nsnap=200, epochs=2, windows=6, attractor_windows=4, spin=20, window=10,
-                  calib_steps=100, eval_steps=200)
-    for k in ("nsnap", "epochs", "windows", "attractor_windows"):
+                  nsnap=200, holdout_time=1.6, epochs=2, windows=6, attractor_windows=4, spin=20,
+                  window=10, calib_steps=100)
+    kw["save"] = a.save
+    for k in ("nsnap", "epochs", "windows", "attractor_windows", "holdout_time"):
         if getattr(a, k) is not None:
             kw[k] = getattr(a, k)
     res = run(**kw)

@@ -51,26 +51,156 @@ def pass_bytes(N, elem=8):
     return {k: v * elem for k, v in per.items()}, step * elem
 
 
-def hbm_copy_gbs(torch, reps=4):
-    """Device-to-device copy of 2 GiB with the 256 MB Infinity Cache flushed
-    (a 1 GiB write) before each copy: the bandwidth this box's HBM delivers to
-    a plain streaming kernel, reported beside the 8 TB/s datasheet peak."""
+def hbm_ceiling(torch, lib, stream, reps=5):
+    """Read + write bandwidth of plain streaming copies of 2 GiB, with the
+    256 MB Infinity Cache flushed (a 1 GiB write) before each: torch's
+    copy_, the library's hand-written copy kernel (hit3d_stream_copy, one
+    16-byte word per thread) and the same with non-temporal loads and stores
+    (four words per thread). Median of `reps`, GB/s. The largest is the
+    attainable ceiling the pass kernels are compared with, beside the 8 TB/s
+    datasheet peak."""
     n = 1 << 29
     src = torch.ones(n, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
     flush = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
-    vals = []
-    for i in range(reps + 1):
-        flush.fill_(float(i))
+    ways = {
+        "torch_copy": lambda: dst.copy_(src),
+        "stream_copy": lambda: lib.call("hit3d_stream_copy", src.data_ptr(), dst.data_ptr(), 4 * n, 0,
+                                        stream),
+        "stream_copy_nontemporal": lambda: lib.call("hit3d_stream_copy", src.data_ptr(),
+                                                    dst.data_ptr(), 4 * n, 1, stream),
+    }
+    out = {}
+    for name, fn in ways.items():
+        vals = []
+        for i in range(reps + 1):
+            flush.fill_(float(i))
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            if i:
+                vals.append(2 * 4 * n / (e0.elapsed_time(e1) * 1e-3) / 1e9)
+        out[name] = round(float(np.median(vals)), 1)
+    del src, dst, flush
+    out["best"] = max(out.values())
+    return out
+
+
+def host_info():
+    """CPU model, logical cores this process may use, RAM (GiB)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    ram = None
+    try:
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemTotal"):
+                    ram = round(int(line.split()[1]) / 2**20, 1)
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "cores_available": len(os.sched_getaffinity(0)), "ram_gib": ram}
+
+
+def timed_loops(torch, run, steps, repeats):
+    """`repeats` loops of `steps` steps, each bracketed by device
+    synchronisation and timed with HIP events on the launch stream; returns
+    the per-step milliseconds of every loop and the wall time of all."""
+    ms = []
+    w0 = time.perf_counter()
+    for _ in range(repeats):
+        torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
-        dst.copy_(src)
+        run(steps)
         e1.record()
         torch.cuda.synchronize()
-        if i:
-            vals.append(2 * 4 * n / (e0.elapsed_time(e1) * 1e-3) / 1e9)
-    del src, dst, flush
-    return round(float(np.median(vals)), 1)
+        ms.append(e0.elapsed_time(e1) / steps)
+    return ms, time.perf_counter() - w0
+
+
+def pass_profile(h3, lib, g, u, nu, dt, fdesc, ws, stream, nsteps):
+    """Per-pass HIP-event timing (hit3d_profile_rk4) of `nsteps` steps on a
+    copy of u, with each pass's algorithmic bytes and GB/s."""
+    ms = (ctypes.c_double * 7)()
+    cnt = (ctypes.c_int64 * 7)()
+    up = u.clone()
+    lib.call("hit3d_profile_rk4", g.desc, up.data_ptr(), nu, dt, nsteps, ctypes.byref(fdesc), 1, ms,
+             cnt, ws.data_ptr(), ws.numel(), stream)
+    del up
+    per_bytes, _ = pass_bytes(g.Nx)
+    passes = {}
+    for i, name in enumerate(lib.PASS_NAMES):
+        if cnt[i] == 0:
+            continue
+        avg = ms[i] / cnt[i]
+        d = {"avg_ms": round(avg, 5), "launches": int(cnt[i]),
+             "share": round(ms[i] / sum(ms[j] for j in range(7)), 4)}
+        if name in per_bytes:
+            d["bytes"] = per_bytes[name]
+            d["gbs"] = round(per_bytes[name] / (avg * 1e-3) / 1e9, 1)
+        passes[name] = d
+    return passes
+
+
+def spectrum_timing(torch, h3, u, g, reps=10):
+    """Milliseconds per shell spectrum of the (marked, dealiased) state, read
+    inside the 2/3 box, and of an unmarked copy, read in full."""
+    out = {}
+    for name, x in (("box_read_ms", u), ("full_read_ms", u.clone())):
+        h3.diagnostics.shell_spectrum(x, g)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            h3.diagnostics.shell_spectrum(x, g)
+        e1.record()
+        torch.cuda.synchronize()
+        out[name] = round(e0.elapsed_time(e1) / reps, 4)
+    return out
+
+
+def large_grid_leg(torch, h3, lib, N=1024, steps=5, repeats=3, warmup=2):
+    """The 1024^3 forced configuration (nu = 1/4000, dt = 5e-4, constant
+    energy Lundgren forcing, seeded field): median of `repeats` timed loops
+    of `steps` RK4 steps, per-pass profile, spectrum cost."""
+    nu, dt = 1 / 4000, 5e-4
+    g = h3.make_grid(N, dtype=torch.float32)
+    u = h3.random_field(0, g, h3.config_spectrum)
+    g._cache.pop(("ws", lib.OP_TO_SPECTRAL, 3, 1), None)   # the transform workspace is done
+    torch.cuda.empty_cache()
+    fdesc = h3.LundgrenForcing(nu=nu).desc()
+    ws = g.workspace(lib.OP_RK4, 3, 1)
+    stream = h3.grid.stream()
+
+    def run(n):
+        lib.call("hit3d_rk4_steps", g.desc, u.data_ptr(), nu, dt, n, 0.0, ctypes.byref(fdesc), 1, None,
+                 ws.data_ptr(), ws.numel(), stream)
+    run(warmup)
+    ms, wall = timed_loops(torch, run, steps, repeats)
+    ms_step = float(np.median(ms))
+    sps = 1000.0 / ms_step
+    _, step_bytes = pass_bytes(N)
+    passes = pass_profile(h3, lib, g, u, nu, dt, fdesc, ws, stream, 2)
+    spec = spectrum_timing(torch, h3, u, g, reps=3)
+    E = h3.energy(u, g)
+    del u, ws, g
+    torch.cuda.empty_cache()
+    return {"workload": f"{N}^3 Lundgren-forced DNS, nu=1/4000, dt=5e-4, float32 state, FP64 sums",
+            "steps_per_s": round(sps, 3), "ms_per_step": round(ms_step, 3),
+            "loop_ms_per_step": [round(x, 3) for x in ms], "steps_per_loop": steps,
+            "grid_point_steps_per_s": round(sps * N**3, 1), "step_bytes": step_bytes,
+            "hbm_frac_step": round(step_bytes * sps / (HBM_PEAK_GBS * 1e9), 4),
+            "passes": passes, "spectrum": spec, "energy_end": E, "wall_s": round(wall, 2)}
 
 
 def adjoint_timing(torch, h3, N=128, steps=200, checkpoint=20):
@@ -128,6 +258,28 @@ def profiled_traffic(N, kernel_key):
                 return int(float(cols[-3]) * 1e6), os.path.relpath(path, ROOT)  # "total MB" column
             except ValueError:
                 return None, None
+    return None, None
+
+
+def profiled_kernel_ms(N, kernel_key):
+    """Average launch duration (ms) of the dominant kernel in the committed
+    rocprofv3 kernel-trace summary of this workload, for comparison with the
+    live HIP-event figure; (None, None) when there is no summary."""
+    path = os.path.join(ROOT, "profiles", f"rk4_{N}_kernel_trace.md")
+    pat = {"z_c2r_cross_r2c": f"k_z_fused<{N},", "y_inverse_curl": f"k_y_inverse_curl<{N},",
+           "x_inverse_curl": f"k_x_inverse_curl<{N},", "y_forward": f"k_strided<{N}, -1,",
+           "x_forward_stage": f"k_x_forward_stage<{N},"}.get(kernel_key)
+    if pat is None or not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if pat in line and "|" in line:
+            cols = [c.strip() for c in line.strip().strip("|").split("|")]
+            for c in cols[1:]:
+                try:
+                    v = float(c)
+                except ValueError:
+                    continue
+                return v / 1000.0, os.path.relpath(path, ROOT)   # first numeric column: average us
     return None, None
 
 
@@ -198,7 +350,8 @@ def cpu_baseline(u0, N, nu, dt, budget_s=15.0):
     el = time.perf_counter() - t0
     return {"value": n / el, "unit": "RK4 steps/s", "cores": O._WORKERS, "kind": "port",
             "sample": f"{N}^3 float32 Lundgren step of oracle/hit3d_oracle.py (scipy.fft "
-                      f"workers={O._WORKERS}), 1 warm-up + {n} timed steps, {el:.1f} s"}
+                      f"workers={O._WORKERS}), 1 warm-up + {n} timed steps, {el:.1f} s",
+            **host_info()}
 
 
 def main():
@@ -212,6 +365,10 @@ def main():
                     help="skip the 128^3 adjoint (gradient) timing")
     ap.add_argument("--no-les", action="store_true",
                     help="skip the LES-versus-DNS step timing (512^3 DNS, 64^3 LES)")
+    ap.add_argument("--repeats", type=int, default=5,
+                    help="timed loops of --steps steps; value is the median")
+    ap.add_argument("--no-1024", dest="no_1024", action="store_true",
+                    help="skip the 1024^3 leg")
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run 3 device copies of 1 GiB (reference kernel for the "
@@ -234,7 +391,8 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    N, nu, dt = args.n, 1.0 / 1000, 1e-3
+    N = args.n
+    nu, dt = (1.0 / 4000, 5e-4) if N == 1024 else (1.0 / 1000, 1e-3)
     g = h3.make_grid(N, dtype=torch.float32)
     u0 = h3.random_field(0, g, h3.config_spectrum)
     assert h3.is_dealiased(u0, g)
@@ -250,53 +408,24 @@ def main():
                   1, None, ws.data_ptr(), ws.numel(), stream)
 
     steps(args.warmup)
-    torch.cuda.synchronize()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    w0 = time.perf_counter()
-    ev0.record()
-    steps(args.steps)
-    ev1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - w0
-    ms_total = ev0.elapsed_time(ev1)
-    ms_step = ms_total / args.steps
+    ms_loops, wall = timed_loops(torch, steps, args.steps, args.repeats)
+    ms_step = float(np.median(ms_loops))
     sps = 1000.0 / ms_step
     E1 = h3.energy(u, g)
 
     # per-pass HIP-event profile of the same workload (separate run)
-    ms = (ctypes.c_double * 7)()
-    cnt = (ctypes.c_int64 * 7)()
-    up = u.clone()
-    _lib.call("hit3d_profile_rk4", g.desc, up.data_ptr(), nu, dt, args.profile_steps,
-              ctypes.byref(fdesc), 1, ms, cnt, ws.data_ptr(), ws.numel(), stream)
+    passes = pass_profile(h3, _lib, g, u, nu, dt, fdesc, ws, stream, args.profile_steps)
     per_bytes, step_bytes = pass_bytes(N)
-    passes = {}
-    for i, name in enumerate(_lib.PASS_NAMES):
-        if cnt[i] == 0:
-            continue
-        avg = ms[i] / cnt[i]
-        d = {"avg_ms": round(avg, 5), "launches": int(cnt[i]),
-             "share": round(ms[i] / sum(ms[j] for j in range(7)), 4)}
-        if name in per_bytes:
-            d["bytes"] = per_bytes[name]
-            d["gbs"] = round(per_bytes[name] / (avg * 1e-3) / 1e9, 1)
-        passes[name] = d
     dom = max((k for k in passes if k in per_bytes), key=lambda k: passes[k]["avg_ms"] * passes[k]["launches"])
     achieved = passes[dom]["gbs"]
     traffic, traffic_src = profiled_traffic(N, dom)
+    rocprof_ms, rocprof_src = profiled_kernel_ms(N, dom)
 
-    # spectrum cost (not part of the step)
-    spec = h3.diagnostics.shell_spectrum(u, g)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(10):
-        spec = h3.diagnostics.shell_spectrum(u, g)
-    e1.record()
-    torch.cuda.synchronize()
-    spec_ms = e0.elapsed_time(e1) / 10
-
-    copy_gbs = hbm_copy_gbs(torch)
+    spec = spectrum_timing(torch, h3, u, g)
+    ceiling = hbm_ceiling(torch, _lib, stream)
+    del ws
+    torch.cuda.empty_cache()
+    big = None if (args.no_1024 or N == 1024) else large_grid_leg(torch, h3, _lib)
     adj = None if args.no_adjoint else adjoint_timing(torch, h3)
     les_t = None if args.no_les else les_speed(torch, h3)
 
@@ -312,6 +441,9 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_step, 5),
+        "timing": f"median of {args.repeats} timed loops of {args.steps} steps (HIP events, "
+                  "device synchronised around each loop)",
+        "loop_ms_per_step": [round(x, 5) for x in ms_loops],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -319,7 +451,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"{N}^3 Lundgren-forced DNS (constant energy 0.5, FP64 sums per stage), "
-                        f"nu=1/1000, dt=1e-3, seeded k^4 exp(-2(k/4)^2) field, pruned 2/3 pipeline",
+                        f"nu=1/{round(1 / nu)}, dt={dt:g}, seeded k^4 exp(-2(k/4)^2) field, "
+                        "pruned 2/3 pipeline",
             "N": N, "nu": nu, "dt": dt, "reductions": "f64",
         },
         "grid_point_steps_per_s": round(sps * N**3, 1),
@@ -328,12 +461,16 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled)",
-                     "traffic_source": traffic_src, "algorithmic_bytes": per_bytes[dom]},
-        "hbm_copy_gbs": copy_gbs,
+                     "traffic_source": traffic_src, "algorithmic_bytes": per_bytes[dom],
+                     "avg_ms_live": passes[dom]["avg_ms"], "avg_ms_rocprof": rocprof_ms,
+                     "rocprof_source": rocprof_src,
+                     "frac_of_stream_copy": round(achieved / ceiling["best"], 4)},
+        "hbm_ceiling_gbs": ceiling,
+        "n1024": big,
         "adjoint": adj,
         "les": les_t,
         "passes": passes,
-        "spectrum_ms": round(spec_ms, 4),
+        "spectrum": spec,
         "energy": {"start": E0, "end": E1},
         "wall_s_timed": round(wall, 4),
         "cpu_baseline": cpu,

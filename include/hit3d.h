@@ -127,9 +127,13 @@ int hit3d_rk4_steps(const hit3d_grid* grid, void* u_hat, double nu, double dt, i
 /* FP64 shell spectrum (diagnostics.py:85-114): out[s] (device, nbins doubles)
    = sum over modes of shell s of w |u_c|^2 / (2 Ntot^2), summed over
    components (component = -1) or for one component (0..2). Bin 0 holds the
-   k = 0 mode. Deterministic (fixed order), Neumaier-compensated. */
+   k = 0 mode. Deterministic (fixed order), Neumaier-compensated.
+   `dealiased` = 1 promises that u_hat has no content outside the 2/3 box
+   (as for hit3d_rhs), so only the modes inside it are read (about 0.3 of
+   the array); 0 reads every mode. */
 int hit3d_shell_spectrum(const hit3d_grid* grid, const void* u_hat, int32_t component,
-                         double* out, void* work, size_t work_bytes, void* stream);
+                         int32_t dealiased, double* out, void* work, size_t work_bytes,
+                         void* stream);
 
 /* FP64 reductions (diagnostics.py:56-70, forcing.py:48-52): out (device, 4
    doubles) = [E, eps = 2 nu Omega, Omega, E_band], E_band over
@@ -181,6 +185,12 @@ int hit3d_rhs_vjp(const hit3d_grid* grid, const void* u_hat, const void* v, void
 int hit3d_rk4_step_vjp(const hit3d_grid* grid, const void* u_hat, const void* ubar_next,
                        void* ubar, double nu, double dt, double t, const hit3d_forcing* forcing,
                        void* work, size_t work_bytes, void* stream);
+
+/* Measurement kernel (no reference counterpart): dst = src over `bytes`
+   (a multiple of 16; both pointers 16-byte aligned) by a plain streaming
+   copy, with non-temporal loads and stores when `nontemporal` is non-zero.
+   Its rate is the attainable HBM bandwidth the pass kernels are held to. */
+int hit3d_stream_copy(const void* src, void* dst, size_t bytes, int32_t nontemporal, void* stream);
 
 /* Measurement form of hit3d_rk4_steps (no reference counterpart): runs the
    same nsteps with a HIP event after every launch on `stream`, waits for the

@@ -144,16 +144,23 @@ def test_subgrid_stress_matches_numpy():
     assert np.abs(h3.to_physical(ubar, coarse).cpu().numpy() - ub[:, ::2, ::2, ::2]).max() < 1e-12
 
 
-def test_les_experiment_driver_end_to_end():
+def test_les_experiment_driver_end_to_end(tmp_path):
     """The whole pipeline at toy size: DNS 64^3 -> LES 16^3 (kcut 5), data,
     a-priori and a-posteriori training, evaluation and timing."""
     from hit3d import les_experiment
     res = les_experiment.run(fine_n=64, les_n=16, kcut=5.0, nu=1 / 200, dt_dns=4e-3, stride=2,
-                             spinup_time=0.2, nsnap=24, epochs=1, windows=2, window=3,
-                             attractor_windows=2, spin=3, eval_steps=20, channels=16)
-    assert res["config"]["eval_steps"] == 20 and len(res["les"]["E_les"]) == 5
-    assert res["les"]["long_steps"] == 60 and np.isfinite(res["les"]["max_rel_err_long"])
-    assert np.isfinite(res["dns_half_vs_half_max"])
+                             spinup_time=0.2, nsnap=24, holdout_time=0.4, epochs=1, windows=2, window=3,
+                             attractor_windows=2, spin=3, calib_steps=20, channels=16,
+                             save=str(tmp_path / "closure.npz"))
+    assert res["config"]["eval_steps"] == 49 and len(res["les"]["E_les"]) == 5
+    assert res["config"]["holdout_snapshots"] == 50 and res["acceptance_statistic"] == "held-out"
+    assert res["les"]["long_steps"] == 147 and np.isfinite(res["les"]["max_rel_err_long"])
+    assert res["les_in_sample"]["steps"] == 23 and np.isfinite(res["les_in_sample"]["max_rel_err"])
+    assert np.isfinite(res["training_half_vs_half_max"]) and np.isfinite(res["training_vs_holdout_max"])
+    net, target, e_target, config = les_experiment.load_closure(str(tmp_path / "closure.npz"),
+                                                                h3.make_grid(16, dtype=torch.float32))
+    assert len(target) == 5 and e_target > 0 and config["kcut"] == 5.0
+    assert abs(float(net.gain.detach()) - res["closure_gain"]) < 1e-6 and net.tilt == res["closure_tilt"]
     assert res["config"]["forcing_les"].startswith("HoldResolvedEnergy")
     assert len(res["closure_calibration"]) == 7          # start + 2 x (2 differences + step)
     best = min(res["closure_calibration"], key=lambda c: c["rms_log_error"])

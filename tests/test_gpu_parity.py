@@ -452,3 +452,20 @@ def test_rollout_chain_skips_the_alias_check_and_in_place_edits_void_it(monkeypa
     assert calls.count("hit3d_alias_energy") == 1
     ref = O.rollout(O.make_ns_rhs(og), u2.cpu().numpy(), O.NSParams(nu=0.02), 1e-3, 1)
     assert rel(u3, ref) < 1e-12
+
+
+@pytest.mark.parametrize("shape,dtype", [((64, 64, 64), torch.float64), ((256, 32, 512), torch.float32),
+                                         ((16, 128, 8), torch.float64)])
+def test_spectrum_box_read_equals_full_read(shape, dtype):
+    """A marked (dealiased) state's spectrum reads only the 2/3 box; it
+    agrees with the full read of the same data and sums to the energy."""
+    g = h3.make_grid(*shape, dtype=dtype)
+    u = h3.random_field(4, g, h3.config_spectrum)        # marked by random_field
+    box = h3.diagnostics.shell_spectrum(u, g).cpu().numpy()
+    plain = h3.diagnostics.shell_spectrum(u.clone(), g).cpu().numpy()   # a clone carries no mark
+    for c in range(3):
+        assert np.allclose(h3.diagnostics.shell_spectrum(u, g, c).cpu().numpy(),
+                           h3.diagnostics.shell_spectrum(u.clone(), g, c).cpu().numpy(), rtol=1e-13, atol=0)
+    assert np.allclose(box, plain, rtol=1e-13, atol=0)
+    assert abs(box.sum() - h3.energy(u, g)) < 1e-13
+    assert np.array_equal(box, h3.diagnostics.shell_spectrum(u, g).cpu().numpy())   # reproducible
