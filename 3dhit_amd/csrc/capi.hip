@@ -559,7 +559,7 @@ VjpBuffers carve_vjp(const hit3d_grid& gd, void* work, size_t* total) {
 
 template <class T>
 int rhs_vjp_impl(const hit3d_grid& gd, const cpx<T>* u, const cpx<T>* v, cpx<T>* out, double nu,
-                 const hit3d_forcing* f, void* work, hipStream_t s) {
+                 const hit3d_forcing* f, void* work, hipStream_t s, double* pgrad = nullptr) {
   size_t total;
   VjpBuffers b = carve_vjp(gd, work, &total);
   const Geo g = make_geo(gd, false);
@@ -607,6 +607,9 @@ int rhs_vjp_impl(const hit3d_grid& gd, const cpx<T>* u, const cpx<T>* v, cpx<T>*
     H3_HIP(hipGetLastError());
   }
   H3_HIP(h3::launch_adj_combine<T>(S, v, vp, u, out, fs, g, kmin2, kmax2, c, s));
+  if (pgrad)  // b.scalars[3] = 2E_band of u (reduce_state above, band forcing)
+    H3_HIP(h3::launch_param_grad<T>(v, u, fs, g, kmin2, kmax2, c.fkind, b.scalars + 3, c.tiny,
+                                    b.partials, pgrad, s));
   return HIT3D_OK;
 }
 
@@ -641,7 +644,8 @@ StepVjpBuffers carve_step_vjp(const hit3d_grid& gd, void* work, size_t* total) {
 
 template <class T>
 int rk4_step_vjp_impl(const hit3d_grid& gd, const cpx<T>* u, const cpx<T>* ub_in, cpx<T>* out,
-                      double nu, double dt, const hit3d_forcing* f, void* work, hipStream_t s) {
+                      double nu, double dt, const hit3d_forcing* f, void* work, hipStream_t s,
+                      double* pgrad) {
   size_t total;
   StepVjpBuffers b = carve_step_vjp(gd, work, &total);
   const int64_t n = int64_t(3) * gd.nx * gd.ny * (gd.nz / 2 + 1);
@@ -660,16 +664,16 @@ int rk4_step_vjp_impl(const hit3d_grid& gd, const cpx<T>* u, const cpx<T>* ub_in
   H3_HIP(h3::launch_axpby<T>(y4, T(1), u, fdt, k, n, s));
   // reverse sweep: kbar_i from the combination weights, then stage adjoints 4..1
   H3_HIP(h3::launch_axpby<T>(kb, sdt, ub, T(0), ub, n, s));
-  if ((rc = rhs_vjp_impl<T>(gd, y4, kb, yb, nu, f, b.vjp_ws, s))) return rc;
+  if ((rc = rhs_vjp_impl<T>(gd, y4, kb, yb, nu, f, b.vjp_ws, s, pgrad))) return rc;
   H3_HIP(h3::launch_axpby<T>(out, T(1), ub, T(1), yb, n, s));
   H3_HIP(h3::launch_axpby<T>(kb, tdt, ub, fdt, yb, n, s));
-  if ((rc = rhs_vjp_impl<T>(gd, y3, kb, yb, nu, f, b.vjp_ws, s))) return rc;
+  if ((rc = rhs_vjp_impl<T>(gd, y3, kb, yb, nu, f, b.vjp_ws, s, pgrad))) return rc;
   H3_HIP(h3::launch_axpby<T>(out, T(1), out, T(1), yb, n, s));
   H3_HIP(h3::launch_axpby<T>(kb, tdt, ub, hdt, yb, n, s));
-  if ((rc = rhs_vjp_impl<T>(gd, y2, kb, yb, nu, f, b.vjp_ws, s))) return rc;
+  if ((rc = rhs_vjp_impl<T>(gd, y2, kb, yb, nu, f, b.vjp_ws, s, pgrad))) return rc;
   H3_HIP(h3::launch_axpby<T>(out, T(1), out, T(1), yb, n, s));
   H3_HIP(h3::launch_axpby<T>(kb, sdt, ub, hdt, yb, n, s));
-  if ((rc = rhs_vjp_impl<T>(gd, u, kb, yb, nu, f, b.vjp_ws, s))) return rc;
+  if ((rc = rhs_vjp_impl<T>(gd, u, kb, yb, nu, f, b.vjp_ws, s, pgrad))) return rc;
   H3_HIP(h3::launch_axpby<T>(out, T(1), out, T(1), yb, n, s));
   return HIT3D_OK;
 }
@@ -1099,8 +1103,8 @@ int hit3d_profile_rk4(const hit3d_grid* grid, void* u_hat, double nu, double dt,
 }
 
 int hit3d_rhs_vjp(const hit3d_grid* grid, const void* u_hat, const void* v, void* ubar, double nu,
-                  const hit3d_forcing* forcing, double t, void* work, size_t work_bytes,
-                  void* stream) {
+                  const hit3d_forcing* forcing, double t, double* param_grad, void* work,
+                  size_t work_bytes, void* stream) {
   (void)t;
   H3_TRY(check_grid(grid));
   H3_TRY(check_forcing(forcing));
@@ -1115,15 +1119,15 @@ int hit3d_rhs_vjp(const hit3d_grid* grid, const void* u_hat, const void* v, void
   if (grid->precision == 4)
     return rhs_vjp_impl<float>(*grid, static_cast<const cpx<float>*>(u_hat),
                                static_cast<const cpx<float>*>(v), static_cast<cpx<float>*>(ubar),
-                               nu, forcing, work, s);
+                               nu, forcing, work, s, param_grad);
   return rhs_vjp_impl<double>(*grid, static_cast<const cpx<double>*>(u_hat),
                               static_cast<const cpx<double>*>(v), static_cast<cpx<double>*>(ubar),
-                              nu, forcing, work, s);
+                              nu, forcing, work, s, param_grad);
 }
 
 int hit3d_rk4_step_vjp(const hit3d_grid* grid, const void* u_hat, const void* ubar_next,
                        void* ubar, double nu, double dt, double t, const hit3d_forcing* forcing,
-                       void* work, size_t work_bytes, void* stream) {
+                       double* param_grad, void* work, size_t work_bytes, void* stream) {
   (void)t;
   H3_TRY(check_grid(grid));
   H3_TRY(check_forcing(forcing));
@@ -1140,10 +1144,12 @@ int hit3d_rk4_step_vjp(const hit3d_grid* grid, const void* u_hat, const void* ub
   if (grid->precision == 4)
     return rk4_step_vjp_impl<float>(*grid, static_cast<const cpx<float>*>(u_hat),
                                     static_cast<const cpx<float>*>(ubar_next),
-                                    static_cast<cpx<float>*>(ubar), nu, dt, forcing, work, s);
+                                    static_cast<cpx<float>*>(ubar), nu, dt, forcing, work, s,
+                                    param_grad);
   return rk4_step_vjp_impl<double>(*grid, static_cast<const cpx<double>*>(u_hat),
                                    static_cast<const cpx<double>*>(ubar_next),
-                                   static_cast<cpx<double>*>(ubar), nu, dt, forcing, work, s);
+                                   static_cast<cpx<double>*>(ubar), nu, dt, forcing, work, s,
+                                   param_grad);
 }
 
 }  // extern "C"
@@ -1339,6 +1345,26 @@ int hit3d_velocity_products(const hit3d_grid* grid, const void* u, void* uu, voi
     H3_HIP(h3::launch_products<float>(static_cast<const float*>(u), V, static_cast<float*>(uu), s));
   else
     H3_HIP(h3::launch_products<double>(static_cast<const double*>(u), V, static_cast<double*>(uu), s));
+  return HIT3D_OK;
+}
+
+int hit3d_les_combine(const hit3d_grid* grid, const void* du_hat, const void* tau_hat,
+                      const void* shape, const float* gain, double kcut, void* out, void* stream) {
+  H3_TRY(check_grid(grid));
+  H3_TRY(check_ptr(du_hat, "du_hat"));
+  H3_TRY(check_ptr(out, "out"));
+  if (tau_hat) {
+    H3_TRY(check_ptr(tau_hat, "tau_hat"));
+    if (!gain) return fail(HIT3D_ERR_INVALID, "gain is NULL");
+  }
+  if (grid->precision != 4) return fail(HIT3D_ERR_INVALID, "the closure runs on float32 grids");
+  if (!(kcut >= 0.0)) return fail(HIT3D_ERR_INVALID, "kcut must be >= 0");
+  const Geo g = make_geo(*grid, false);
+  H3_HIP(h3::launch_les_combine<float>(static_cast<const cpx<float>*>(du_hat),
+                                       static_cast<const cpx<float>*>(tau_hat),
+                                       static_cast<const float*>(shape), gain, float(kcut * kcut),
+                                       static_cast<cpx<float>*>(out), g,
+                                       static_cast<hipStream_t>(stream)));
   return HIT3D_OK;
 }
 

@@ -593,6 +593,60 @@ hipError_t launch_products(const T* u, int64_t V, T* uu, hipStream_t s) {
   return hipGetLastError();
 }
 
+// out = [k2 <= kcut2] (du + gain S(k) P M F), F_i = -i sum_j k_j tau_ij: the
+// closure force assembled onto the resolved RHS in one pass over the modes
+// (the stress divergence of k_stress_div, the 2/3 mask and Leray projection
+// of the nonlinear term, the force's gain and spectral shape S, the sum and
+// the LES's spherical mask). `shape` (real [nx][ny][nz/2+1]) may be null
+// (S = 1); `gain` is a device scalar, so captured steps follow its updates;
+// a null `tau` leaves only the mask (no closure force).
+template <class T>
+__global__ void k_les_combine(const cpx<T>* du, const cpx<T>* __restrict__ tau,
+                              const T* __restrict__ shape, const float* __restrict__ gain, T kcut2,
+                              cpx<T>* out, Geo g) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  const T gn = tau ? T(*gain) : T(0);
+  for (int64_t m = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; m < n;
+       m += int64_t(gridDim.x) * blockDim.x) {
+    const int iz = int(m % g.nzh), iy = int((m / g.nzh) % g.ny), ix = int(m / (int64_t(g.nzh) * g.ny));
+    const int nx = ix < g.nx / 2 ? ix : ix - g.nx, ny = iy < g.ny / 2 ? iy : iy - g.ny;
+    const T k[3] = {wavenumber<T>(g.k0x, nx), wavenumber<T>(g.k0y, ny), wavenumber<T>(g.k0z, iz)};
+    const T k2 = k2_of(k[0], k[1], k[2]);
+    if (!(k2 <= kcut2)) {
+      for (int i = 0; i < 3; ++i) out[i * n + m] = czero<T>();
+      continue;
+    }
+    cpx<T> f[3] = {czero<T>(), czero<T>(), czero<T>()};
+    if (tau && in_box(nx, ny, iz, g)) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        cpx<T> s{T(0), T(0)};
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const cpx<T> t = tau[sidx(i, j) * n + m];
+          s = {s.x + k[j] * t.x, s.y + k[j] * t.y};
+        }
+        f[i] = {s.y, -s.x};  // -i s
+      }
+      const cpx<T> div = inv_k2_of(k2) * (k[0] * f[0] + k[1] * f[1] + k[2] * f[2]);
+      const T c = gn * (shape ? shape[m] : T(1));
+#pragma unroll
+      for (int i = 0; i < 3; ++i) f[i] = c * (f[i] - k[i] * div);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out[i * n + m] = du[i * n + m] + f[i];
+  }
+}
+
+template <class T>
+hipError_t launch_les_combine(const cpx<T>* du, const cpx<T>* tau, const T* shape, const float* gain,
+                              T kcut2, cpx<T>* out, const Geo& g, hipStream_t s) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  hipLaunchKernelGGL((k_les_combine<T>), dim3(grid_for(n)), dim3(256), 0, s, du, tau, shape, gain, kcut2,
+                     out, g);
+  return hipGetLastError();
+}
+
 template <class T>
 hipError_t launch_stress_div(const cpx<T>* in, cpx<T>* out, const Geo& g, int dir, hipStream_t s) {
   const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
@@ -604,7 +658,9 @@ hipError_t launch_stress_div(const cpx<T>* in, cpx<T>* out, const Geo& g, int di
   template hipError_t launch_truncate<T>(const cpx<T>*, int, int, int, cpx<T>*, int, int, int, int,  \
                                          double, T, hipStream_t);                                    \
   template hipError_t launch_products<T>(const T*, int64_t, T*, hipStream_t);                        \
-  template hipError_t launch_stress_div<T>(const cpx<T>*, cpx<T>*, const Geo&, int, hipStream_t);
+  template hipError_t launch_stress_div<T>(const cpx<T>*, cpx<T>*, const Geo&, int, hipStream_t);    \
+  template hipError_t launch_les_combine<T>(const cpx<T>*, const cpx<T>*, const T*, const float*, T, \
+                                            cpx<T>*, const Geo&, hipStream_t);
 H3_CLOSURE_INST(float)
 H3_CLOSURE_INST(double)
 

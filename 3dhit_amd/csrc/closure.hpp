@@ -45,5 +45,8 @@ template <class T>
 hipError_t launch_products(const T* u, int64_t V, T* uu, hipStream_t s);
 template <class T>
 hipError_t launch_stress_div(const cpx<T>* in, cpx<T>* out, const Geo& g, int dir, hipStream_t s);
+template <class T>
+hipError_t launch_les_combine(const cpx<T>* du, const cpx<T>* tau, const T* shape, const float* gain,
+                              T kcut2, cpx<T>* out, const Geo& g, hipStream_t s);
 
 }  // namespace h3

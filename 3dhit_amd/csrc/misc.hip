@@ -2,6 +2,7 @@
 // kz = nz/2+1, jp = ny*kz), under which the compact layout is the API layout.
 #include <cmath>
 
+#include "../../include/hit3d.h"
 #include "misc.hpp"
 
 namespace h3 {
@@ -530,6 +531,71 @@ __global__ __launch_bounds__(kBlock) void k_dot_partials(const cpx<T>* __restric
   block_sum3m(s, 0.0, 0.0, partials + 3 * blockIdx.x);
 }
 
+// Pairings of a cotangent kb with the parameter derivatives of the RHS at
+// the state y (navierstokes.py:89-95, forcing.py:74-163), over all entries of
+// the full-layout arrays (the gradient convention dJ = Re sum conj(g) dx):
+//   (sum Re conj(kb) (-k2 y),  sum Re conj(P M kb) y,  same over kmin2 <= k2 <= kmax2),
+// i.e. d/dnu, d/dA of a linear forcing term A y and, once divided by the
+// band's 2E, d/deps of the band forcing term (eps / 2E_band) y on the band.
+// P M (projection after the 2/3 mask) is self-adjoint, so it acts on kb.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_param_partials(const cpx<T>* __restrict__ kb,
+                                                           const cpx<T>* __restrict__ y, int64_t fs,
+                                                           Geo g, T kmin2, T kmax2,
+                                                           double* __restrict__ partials) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  double s_nu = 0.0, s_a = 0.0, s_band = 0.0;
+  for (int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x; idx < n;
+       idx += int64_t(gridDim.x) * kBlock) {
+    const ModeIdx m = mode_of(idx, g);
+    const T kx = wavenumber<T>(g.k0x, m.nx), ky = wavenumber<T>(g.k0y, m.ny),
+            kz = wavenumber<T>(g.k0z, m.nz);
+    const T k2 = k2_of(kx, ky, kz);
+    const cpx<T> a0 = kb[idx], a1 = kb[fs + idx], a2 = kb[2 * fs + idx];
+    const cpx<T> b0 = y[idx], b1 = y[fs + idx], b2 = y[2 * fs + idx];
+    const double ab = double(a0.x) * b0.x + double(a0.y) * b0.y + double(a1.x) * b1.x +
+                      double(a1.y) * b1.y + double(a2.x) * b2.x + double(a2.y) * b2.y;
+    s_nu -= double(k2) * ab;
+    if (in_box(m.nx, m.ny, m.nz, g)) {
+      const cpx<T> div = inv_k2_of(k2) * (kx * a0 + ky * a1 + kz * a2);
+      const cpx<T> p0 = a0 - kx * div, p1 = a1 - ky * div, p2 = a2 - kz * div;
+      const double pb = double(p0.x) * b0.x + double(p0.y) * b0.y + double(p1.x) * b1.x +
+                        double(p1.y) * b1.y + double(p2.x) * b2.x + double(p2.y) * b2.y;
+      s_a += pb;
+      if (kmin2 <= k2 && k2 <= kmax2) s_band += pb;
+    }
+  }
+  block_sum3m(s_nu, s_a, s_band, partials + 3 * blockIdx.x);
+}
+
+// pgrad[0..2] += (S_nu, S_A when the forcing is linear, S_band / 2E_band when
+// it is the band forcing and 2E_band > tiny): fixed-order Neumaier sums of the
+// block partials; twoE_band (device) comes from the FP64 sums of the same state.
+__global__ __launch_bounds__(kBlock) void k_param_accumulate(const double* __restrict__ partials,
+                                                             int nblocks, int kind,
+                                                             const double* __restrict__ two_e_band,
+                                                             double tiny, double* __restrict__ pgrad) {
+  __shared__ double sh[3][kBlock];
+  KSum acc[3];
+  for (int i = threadIdx.x; i < nblocks; i += kBlock)
+    for (int c = 0; c < 3; ++c) acc[c].add(partials[3 * i + c]);
+  for (int c = 0; c < 3; ++c) sh[c][threadIdx.x] = acc[c].value();
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int c = 0; c < 3; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    pgrad[0] += sh[0][0];
+    if (kind == HIT3D_FORCING_LINEAR) pgrad[1] += sh[1][0];
+    if (kind == HIT3D_FORCING_BAND) {
+      const double tb = *two_e_band;
+      if (tb > tiny) pgrad[2] += sh[2][0] / tb;
+    }
+  }
+}
+
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_hold_partials(const cpx<T>* __restrict__ u,
                                                           const cpx<T>* __restrict__ du,
@@ -747,6 +813,17 @@ hipError_t launch_dot_partials(const cpx<T>* x, const cpx<T>* y, int64_t fs, con
 }
 
 template <class T>
+hipError_t launch_param_grad(const cpx<T>* kb, const cpx<T>* y, int64_t fs, const Geo& g, T kmin2,
+                             T kmax2, int kind, const double* two_e_band, double tiny,
+                             double* partials, double* pgrad, hipStream_t s) {
+  hipLaunchKernelGGL((k_param_partials<T>), dim3(kReduceBlocks), dim3(kBlock), 0, s, kb, y, fs, g,
+                     kmin2, kmax2, partials);
+  hipLaunchKernelGGL(k_param_accumulate, dim3(1), dim3(kBlock), 0, s, partials, kReduceBlocks, kind,
+                     two_e_band, tiny, pgrad);
+  return hipGetLastError();
+}
+
+template <class T>
 hipError_t launch_hold_partials(const cpx<T>* u, const cpx<T>* du, int64_t fs, const Geo& g,
                                 double* partials, hipStream_t s) {
   hipLaunchKernelGGL((k_hold_partials<T>), dim3(kReduceBlocks), dim3(kBlock), 0, s, u, du, fs, g,
@@ -792,6 +869,9 @@ hipError_t launch_axpby(cpx<T>* out, T a, const cpx<T>* x, T b, const cpx<T>* y,
                                       hipStream_t);                                                       \
   template hipError_t launch_hold_partials<T>(const cpx<T>*, const cpx<T>*, int64_t, const Geo&,  \
                                               double*, hipStream_t);                               \
+  template hipError_t launch_param_grad<T>(const cpx<T>*, const cpx<T>*, int64_t, const Geo&, T, T, \
+                                           int, const double*, double, double*, double*,          \
+                                           hipStream_t);                                           \
   template hipError_t launch_hold_apply<T>(const cpx<T>*, const cpx<T>*, cpx<T>*, int64_t,        \
                                            const double*, hipStream_t);
 

@@ -66,6 +66,13 @@ hipError_t launch_adj_cross(const T* ph, int64_t pfs, T* out, int64_t ofs, int64
 template <class T>
 hipError_t launch_dot_partials(const cpx<T>* x, const cpx<T>* y, int64_t fs, const Geo& g,
                                T kmin2, T kmax2, int use_band, double* partials, hipStream_t s);
+// pgrad[0..2] (device) += d/dnu, d/dA (linear forcing), d/deps (band forcing)
+// of Re sum conj(kb) rhs(y), both full-layout [3] states; two_e_band (device)
+// holds y's band 2E (read for the band forcing only); partials: kReduceBlocks*3
+template <class T>
+hipError_t launch_param_grad(const cpx<T>* kb, const cpx<T>* y, int64_t fs, const Geo& g, T kmin2,
+                             T kmax2, int kind, const double* two_e_band, double tiny,
+                             double* partials, double* pgrad, hipStream_t s);
 // partials[kReduceBlocks][3]: (sum w Re(conj(u) du), sum w |u|^2, 0) over full-layout states,
 // w the Hermitian weight of the mode's kz column
 template <class T>

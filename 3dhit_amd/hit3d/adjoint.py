@@ -11,6 +11,13 @@ segment during the backward sweep.
 Gradient convention (torch's): for a real J, the gradient g of a complex
 tensor satisfies dJ = Re sum(conj(g) * du) over all entries; jax.vjp's
 cotangent for the same J is conj(g).
+
+The physical scalars are differentiable too, as NSParams' leaves are in the
+reference (navierstokes.py:1-8, tests/test_grad.py:43-45, 59-74): every VJP
+here can accumulate [dJ/dnu, dJ/dA (LinearForcing), dJ/deps (BandForcing)]
+into a device float64 vector (`param_grad`), reduced on the GPU in FP64 from
+the stage cotangents; `differentiable_rollout` returns them to torch
+autograd when nu, A or eps are tensors that require grad.
 """
 
 from __future__ import annotations
@@ -20,50 +27,70 @@ import ctypes
 import torch
 
 from . import _lib
+from .forcing import BandForcing, LinearForcing
 from .grid import check_spectral, stream
 from .integrators import rollout as _rollout
 
-__all__ = ["make_ns_rhs_vjp", "rk4_step_vjp", "rollout_vjp", "differentiable_rollout"]
+__all__ = ["make_ns_rhs_vjp", "rk4_step_vjp", "rollout_vjp", "differentiable_rollout",
+           "param_grad_buffer"]
 
 
 def make_ns_rhs_vjp(grid):
     """v -> (d rhs / d u_hat)^* v at u_hat, for `rhs = make_ns_rhs(grid)`."""
 
-    def vjp(u_hat, v, params, t):
+    def vjp(u_hat, v, params, t, param_grad=None):
+        """`param_grad` (device float64, 3 entries): the derivatives of
+        Re<v, rhs(u_hat)> in nu, A, eps are added to it."""
         u = check_spectral(u_hat, grid, 3)
         vv = check_spectral(v, grid, 3, "v")
         out = torch.empty_like(u)
         ws = grid.workspace(_lib.OP_RHS_VJP)
         f = params.forcing.desc()
         _lib.call("hit3d_rhs_vjp", grid.desc, u.data_ptr(), vv.data_ptr(), out.data_ptr(),
-                  float(params.nu), ctypes.byref(f), float(t), ws.data_ptr(), ws.numel(),
-                  stream())
+                  float(params.nu), ctypes.byref(f), float(t), _pg_ptr(param_grad, u),
+                  ws.data_ptr(), ws.numel(), stream())
         return out
 
     return vjp
 
 
-def _step_vjp_into(grid, u, ub, params, dt, t, out):
+def param_grad_buffer(device):
+    """Zeroed device accumulator [dJ/dnu, dJ/dA, dJ/deps] (float64)."""
+    return torch.zeros(3, dtype=torch.float64, device=device)
+
+
+def _pg_ptr(pg, like):
+    if pg is None:
+        return None
+    if pg.dtype != torch.float64 or pg.numel() != 3 or pg.device != like.device or not pg.is_contiguous():
+        raise ValueError("param_grad must be a contiguous float64 device tensor of 3 entries")
+    return pg.data_ptr()
+
+
+def _step_vjp_into(grid, u, ub, params, dt, t, out, param_grad=None):
     ws = grid.workspace(_lib.OP_RK4_VJP)
     f = params.forcing.desc()
     _lib.call("hit3d_rk4_step_vjp", grid.desc, u.data_ptr(), ub.data_ptr(), out.data_ptr(),
-              float(params.nu), float(dt), float(t), ctypes.byref(f), ws.data_ptr(), ws.numel(),
-              stream())
+              float(params.nu), float(dt), float(t), ctypes.byref(f), _pg_ptr(param_grad, u),
+              ws.data_ptr(), ws.numel(), stream())
 
 
-def rk4_step_vjp(rhs, u_hat, ubar, params, dt, t):
-    """Cotangent of u_hat given the cotangent `ubar` of rk4_step's result."""
+def rk4_step_vjp(rhs, u_hat, ubar, params, dt, t, param_grad=None):
+    """Cotangent of u_hat given the cotangent `ubar` of rk4_step's result
+    (parameter derivatives added to `param_grad` when given)."""
     g = rhs.grid
     u = check_spectral(u_hat, g, 3)
     ub = check_spectral(ubar, g, 3, "ubar")
     out = torch.empty_like(u)
-    _step_vjp_into(g, u, ub, params, dt, t, out)
+    _step_vjp_into(g, u, ub, params, dt, t, out, param_grad)
     return out
 
 
-def rollout_vjp(rhs, u_hat, ubar, params, dt, nsteps, t0=0.0, checkpoint=20):
+def rollout_vjp(rhs, u_hat, ubar, params, dt, nsteps, t0=0.0, checkpoint=20, param_grad=None):
     """Cotangent of rollout's input given the cotangent of its output, with
-    checkpointed recomputation (memory: nsteps/checkpoint + checkpoint states)."""
+    checkpointed recomputation (memory: nsteps/checkpoint + checkpoint
+    states); the parameter derivatives of every step are added to
+    `param_grad` when given."""
     g = rhs.grid
     if checkpoint < 1:
         raise ValueError("checkpoint must be >= 1")
@@ -79,16 +106,21 @@ def rollout_vjp(rhs, u_hat, ubar, params, dt, nsteps, t0=0.0, checkpoint=20):
         for n in range(s, e - 1):
             states.append(_rollout(rhs, states[-1], params, dt, 1, t0 + n * dt))
         for n in range(e - 1, s - 1, -1):
-            _step_vjp_into(g, states[n - s], ub, params, dt, t0 + n * dt, ub)
+            _step_vjp_into(g, states[n - s], ub, params, dt, t0 + n * dt, ub, param_grad)
     return ub
+
+
+def _leaf(x):
+    return x if isinstance(x, torch.Tensor) else None
 
 
 class _Rollout(torch.autograd.Function):
     """rollout with the hand-written adjoint as its backward (the role of
-    jax.custom_vjp around the reference's scan)."""
+    jax.custom_vjp around the reference's scan). nu and the forcing
+    coefficient enter as tensors when they are to be differentiated."""
 
     @staticmethod
-    def forward(ctx, u_hat, rhs, params, dt, nsteps, t0, checkpoint):
+    def forward(ctx, u_hat, nu, coef, rhs, params, dt, nsteps, t0, checkpoint):
         ctx.save_for_backward(u_hat)
         ctx.cfg = (rhs, params, dt, nsteps, t0, checkpoint)
         return _rollout(rhs, u_hat.detach(), params, dt, nsteps, t0)
@@ -97,11 +129,30 @@ class _Rollout(torch.autograd.Function):
     def backward(ctx, grad_out):
         (u0,) = ctx.saved_tensors
         rhs, params, dt, nsteps, t0, checkpoint = ctx.cfg
-        g = rollout_vjp(rhs, u0, grad_out.contiguous(), params, dt, nsteps, t0, checkpoint)
-        return g, None, None, None, None, None, None
+        need_nu, need_coef = ctx.needs_input_grad[1], ctx.needs_input_grad[2]
+        pg = param_grad_buffer(u0.device) if (need_nu or need_coef) else None
+        g = rollout_vjp(rhs, u0, grad_out.contiguous(), params, dt, nsteps, t0, checkpoint, pg)
+        g_nu = g_coef = None
+        if pg is not None:
+            host = pg.cpu()
+            if need_nu:
+                g_nu = host[0].to(params.nu.dtype).reshape(params.nu.shape)
+            if need_coef:
+                slot, leaf = (1, params.forcing.A) if isinstance(params.forcing, LinearForcing) else \
+                             (2, params.forcing.eps)
+                g_coef = host[slot].to(leaf.dtype).reshape(leaf.shape)
+        return g, g_nu, g_coef, None, None, None, None, None, None
 
 
 def differentiable_rollout(rhs, u_hat, params, dt, nsteps, t0=0.0, checkpoint=20):
-    """rollout(...) whose result back-propagates to u_hat through the
-    hand-written adjoint (use with torch.autograd)."""
-    return _Rollout.apply(u_hat, rhs, params, dt, int(nsteps), float(t0), int(checkpoint))
+    """rollout(...) whose result back-propagates through the hand-written
+    adjoint (use with torch.autograd) to u_hat and, when they are tensors
+    that require grad, to params.nu and to the forcing's coefficient
+    (LinearForcing.A or BandForcing.eps) — the leaves the reference's
+    jax.grad reaches (tests/test_grad.py:43-45, 59-74). Scalar parameters are
+    host (CPU) tensors; their gradients come back as such."""
+    f = params.forcing
+    coef = _leaf(getattr(f, "A", None)) if isinstance(f, LinearForcing) else \
+        _leaf(getattr(f, "eps", None)) if isinstance(f, BandForcing) else None
+    return _Rollout.apply(u_hat, _leaf(params.nu), coef, rhs, params, dt, int(nsteps), float(t0),
+                          int(checkpoint))

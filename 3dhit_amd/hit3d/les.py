@@ -235,8 +235,32 @@ def closure_force(u, grid, net):
 def sphere_mask(grid, kcut):
     """0/1 mask of the modes with |k| <= kcut: the LES resolves exactly the
     sharp spherical filter's modes (the solver's 2/3 box is a cube, whose
-    corners reach sqrt(3) kcut)."""
-    return (grid.k2 <= float(kcut) ** 2).to(grid.dtype)
+    corners reach sqrt(3) kcut). The tensor carries `kcut`, which lets the
+    fused library kernel apply it."""
+    m = (grid.k2 <= float(kcut) ** 2).to(grid.dtype)
+    m.kcut = float(kcut)
+    return m
+
+
+def _les_rhs_fused(u, rhs, p, hold, net, t, kcut):
+    """les_rhs without autograd: the NS RHS, then the closure force, the
+    sum and the spherical mask in one library pass (hit3d_les_combine),
+    then the energy hold."""
+    grid = rhs.grid
+    du = rhs(u, p, t, dealiased=True)
+    out = torch.empty_like(du)
+    if net is not None:
+        V = grid.Nx * grid.Ny * grid.Nz
+        tau = net(velocity_gradient(u, grid).reshape(9, V)).reshape(6, grid.Nx, grid.Ny, grid.Nz)
+        tau_hat = to_spectral(tau, grid)
+        _lib.call("hit3d_les_combine", grid.desc, du.data_ptr(), tau_hat.data_ptr(),
+                  net.spectral_shape().data_ptr(), net.gain.data_ptr(), kcut, out.data_ptr(), stream())
+    else:
+        _lib.call("hit3d_les_combine", grid.desc, du.data_ptr(), None, None, None, kcut, out.data_ptr(),
+                  stream())
+    if hold is not None:
+        out = _hold_raw(u, out, grid, hold.energy, hold.tau)[0]
+    return out
 
 
 @dataclass(frozen=True)
@@ -313,6 +337,9 @@ def les_rhs(u, rhs, params, net, t=0.0, mask=None):
     last, from the masked total."""
     hold = params.forcing if isinstance(params.forcing, HoldResolvedEnergy) else None
     p = NSParams(nu=params.nu, forcing=NoForcing()) if hold is not None else params
+    if not torch.is_grad_enabled() and rhs.grid.dtype == torch.float32 and (
+            mask is None or getattr(mask, "kcut", None) is not None):
+        return _les_rhs_fused(u, rhs, p, hold, net, t, math.inf if mask is None else mask.kcut)
     du = _NSRhs.apply(u, rhs, p, t)
     if net is not None:
         du = du + closure_force(u, rhs.grid, net)

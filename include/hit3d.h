@@ -173,18 +173,25 @@ int hit3d_shell_scale(const hit3d_grid* grid, void* u_hat, int64_t ncomp, const 
    written out by hand. Gradient convention: for a real J, dJ = Re sum over
    all entries of conj(ubar) * du_hat (jax's cotangent is conj(ubar)).
    Includes the derivative of the state-dependent forcing coefficients
-   (BAND, LUNDGREN). Dense transforms; ubar must not alias u_hat or v. */
+   (BAND, LUNDGREN). Dense transforms; ubar must not alias u_hat or v.
+   If `param_grad` (device, 3 doubles) is not NULL, the derivatives of
+   Re sum conj(v) rhs(u_hat) with respect to the physical scalars are ADDED
+   to it: [d/dnu, d/dA (LINEAR forcing; else 0), d/deps (BAND; else 0)] —
+   the gradients jax.grad takes through NSParams' leaves
+   (navierstokes.py:40-59, tests/test_grad.py:43-45, 59-74). */
 int hit3d_rhs_vjp(const hit3d_grid* grid, const void* u_hat, const void* v, void* ubar, double nu,
-                  const hit3d_forcing* forcing, double t, void* work, size_t work_bytes,
-                  void* stream);
+                  const hit3d_forcing* forcing, double t, double* param_grad, void* work,
+                  size_t work_bytes, void* stream);
 
 /* Adjoint of one classic RK4 step (integrators.py:32-38): from the step's
    start state u_hat and the cotangent ubar_next of its result, writes the
    cotangent ubar of u_hat. Recomputes the stage inputs, then runs the four
-   stage adjoints in reverse. ubar may alias ubar_next. */
+   stage adjoints in reverse. ubar may alias ubar_next. `param_grad` (device,
+   3 doubles, or NULL) accumulates the step's parameter derivatives as in
+   hit3d_rhs_vjp, summed over the four stages. */
 int hit3d_rk4_step_vjp(const hit3d_grid* grid, const void* u_hat, const void* ubar_next,
                        void* ubar, double nu, double dt, double t, const hit3d_forcing* forcing,
-                       void* work, size_t work_bytes, void* stream);
+                       double* param_grad, void* work, size_t work_bytes, void* stream);
 
 /* Measurement kernel (no reference counterpart): dst = src over `bytes`
    (a multiple of 16; both pointers 16-byte aligned) by a plain streaming
@@ -272,6 +279,16 @@ int hit3d_velocity_products(const hit3d_grid* grid, const void* u, void* uu, voi
    -> out[6], in the convention dL = Re sum conj(g) dx. No aliasing. */
 int hit3d_stress_divergence(const hit3d_grid* grid, const void* in, void* out, int32_t adjoint,
                             void* stream);
+
+/* The LES right-hand side with its closure force, in one pass (float32
+   grids; no reference counterpart): out = [k2 <= kcut^2] (du + gain S P M F)
+   with F_i = -sum_j i k_j tau_ij from the spectral stress tau_hat[6] (order of
+   hit3d_velocity_products), P M the 2/3 mask and Leray projection, S the
+   real spectral shape `shape` [nx][ny][nz/2+1] (NULL: 1) and `gain` a device
+   float; tau_hat NULL: no closure force (out = masked du). du_hat, out: [3]
+   full layout; out may alias du_hat. */
+int hit3d_les_combine(const hit3d_grid* grid, const void* du_hat, const void* tau_hat,
+                      const void* shape, const float* gain, double kcut, void* out, void* stream);
 
 /* Forcing that holds the resolved energy of an LES (no reference
    counterpart; the LES analogue of forcing.py's constant-energy forcing, with

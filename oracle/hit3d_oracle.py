@@ -334,6 +334,7 @@ def make_ns_rhs(grid):
         du = project(dealias(du, grid), grid)
         return (du - params.nu * grid.k2 * u_hat).astype(u_hat.dtype, copy=False)
 
+    rhs.grid = grid
     return rhs
 
 
@@ -437,9 +438,31 @@ def make_ns_rhs_vjp(grid):
     return vjp
 
 
-def rk4_step_vjp(rhs, rhs_vjp, u_hat, ubar, params, dt, t):
+def rhs_param_vjp(u_hat, v, params, grid):
+    """Derivatives of Re<v, rhs(u_hat)> with respect to the physical scalars
+    NSParams carries (navierstokes.py:40-59, forcing.py:74-163; the leaves
+    jax.grad reaches in tests/test_grad.py:43-45, 59-74): [d/dnu, d/dA of
+    LinearForcing (else 0), d/deps of BandForcing (else 0)]. rhs depends on
+    nu through -nu k2 u, on A through P M (A u), on eps through
+    P M (eps / 2E_band(u) band u) (0 below the finfo(dtype).eps guard)."""
+    out = np.zeros(3)
+    out[0] = _inner(v, -grid.k2 * u_hat)
+    f = params.forcing
+    if isinstance(f, LinearForcing):
+        out[1] = _inner(v, project(dealias(u_hat, grid), grid))
+    elif isinstance(f, BandForcing):
+        band = f._mask(grid)
+        sB = _weighted_2E(u_hat, grid, band)
+        if sB > np.finfo(grid.dtype).eps:
+            out[2] = _inner(v, project(dealias(band * u_hat, grid), grid)) / sB
+    return out
+
+
+def rk4_step_vjp(rhs, rhs_vjp, u_hat, ubar, params, dt, t, pgrad=None):
     """VJP of rk4_step (integrators.py:32-38): recomputes the stage inputs,
-    then runs the four stage adjoints backwards."""
+    then runs the four stage adjoints backwards. `pgrad` (array of 3)
+    accumulates the parameter derivatives, rhs_param_vjp of every stage
+    input with its stage cotangent."""
     k1 = rhs(u_hat, params, t)
     y2 = u_hat + 0.5 * dt * k1
     k2 = rhs(y2, params, t + 0.5 * dt)
@@ -449,20 +472,26 @@ def rk4_step_vjp(rhs, rhs_vjp, u_hat, ubar, params, dt, t):
     out = ubar.copy()
     kb1 = kb4 = (dt / 6.0) * ubar
     kb2 = kb3 = (dt / 3.0) * ubar
-    yb = rhs_vjp(y4, kb4, params, t + dt)
+    grid = rhs.grid
+
+    def stage(y, kb, tt):
+        if pgrad is not None:
+            pgrad[:] = pgrad + rhs_param_vjp(y, kb, params, grid)
+        return rhs_vjp(y, kb, params, tt)
+    yb = stage(y4, kb4, t + dt)
     out, kb3 = out + yb, kb3 + dt * yb
-    yb = rhs_vjp(y3, kb3, params, t + 0.5 * dt)
+    yb = stage(y3, kb3, t + 0.5 * dt)
     out, kb2 = out + yb, kb2 + 0.5 * dt * yb
-    yb = rhs_vjp(y2, kb2, params, t + 0.5 * dt)
+    yb = stage(y2, kb2, t + 0.5 * dt)
     out, kb1 = out + yb, kb1 + 0.5 * dt * yb
-    return out + rhs_vjp(u_hat, kb1, params, t)
+    return out + stage(u_hat, kb1, t)
 
 
-def rollout_vjp(rhs, rhs_vjp, u_hat, ubar, params, dt, nsteps, t0=0.0, checkpoint=20):
+def rollout_vjp(rhs, rhs_vjp, u_hat, ubar, params, dt, nsteps, t0=0.0, checkpoint=20, pgrad=None):
     """VJP of rollout (integrators.py:41-53) with checkpointed recomputation:
     the forward keeps every `checkpoint`-th state; the backward recomputes
     each segment's states from its checkpoint and runs the step adjoints in
-    reverse."""
+    reverse (parameter derivatives accumulated into `pgrad`, if given)."""
     ckpt = {0: u_hat}
     u = u_hat
     for n in range(nsteps):
@@ -475,7 +504,7 @@ def rollout_vjp(rhs, rhs_vjp, u_hat, ubar, params, dt, nsteps, t0=0.0, checkpoin
         for n in range(start, stop - 1):
             states.append(rk4_step(rhs, states[-1], params, dt, t0 + n * dt))
         for n in range(stop - 1, start - 1, -1):
-            ubar = rk4_step_vjp(rhs, rhs_vjp, states[n - start], ubar, params, dt, t0 + n * dt)
+            ubar = rk4_step_vjp(rhs, rhs_vjp, states[n - start], ubar, params, dt, t0 + n * dt, pgrad)
     return ubar
 
 

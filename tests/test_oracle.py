@@ -424,6 +424,46 @@ def test_rk4_step_and_rollout_vjp_match_central_differences():
         assert abs(an - fd) < 1e-7 * abs(fd)
 
 
+PARAM_CASES = [
+    ("nu", 0, lambda x: O.NSParams(nu=x, forcing=O.LinearForcing(A=0.15)), 0.02),
+    ("A", 1, lambda x: O.NSParams(nu=0.02, forcing=O.LinearForcing(A=x)), 0.15),
+    ("nu", 0, lambda x: O.NSParams(nu=x, forcing=O.BandForcing(eps=0.3, kmin=1.0, kmax=3.0)), 0.02),
+    ("eps", 2, lambda x: O.NSParams(nu=0.02, forcing=O.BandForcing(eps=x, kmin=1.0, kmax=3.0)), 0.3),
+    ("nu", 0, lambda x: O.NSParams(nu=x), 0.02),
+]
+
+
+@pytest.mark.parametrize("case", range(len(PARAM_CASES)))
+def test_parameter_gradients_match_central_differences(case):
+    """d/dnu, d/dA, d/deps of J = Re<ub, rollout(u, 5 steps)> (and of one RHS)
+    from the hand adjoint against central differences in the parameter (the
+    gradients the reference takes with jax.grad through NSParams,
+    tests/test_grad.py:43-45, 59-74). The other slots stay 0."""
+    name, slot, make, x0 = PARAM_CASES[case]
+    g = O.make_grid(16)
+    u = O.random_field(11, g, O.config_spectrum)
+    ub = _rand_c(12, u.shape)
+    rhs, vjp = O.make_ns_rhs(g), O.make_ns_rhs_vjp(g)
+    h = 1e-4 * x0             # central differences: O(h^2) truncation, round-off well below
+    # one RHS, also on a state with content outside the 2/3 box and a
+    # divergent part (where the P M in the forcing derivatives matters)
+    rough = u + 1e-2 * np.abs(u).max() * _rand_c(13, u.shape)
+    for x in (u, rough):
+        J1 = lambda p: O._inner(ub, rhs(x, make(p), 0.0))  # noqa: E731
+        fd = (J1(x0 + h) - J1(x0 - h)) / (2 * h)
+        pg = O.rhs_param_vjp(x, ub, make(x0), g)
+        assert abs(pg[slot] - fd) < 1e-7 * abs(fd)
+    f = make(x0).forcing
+    live = {0} | ({1} if isinstance(f, O.LinearForcing) else set()) | ({2} if isinstance(f, O.BandForcing) else set())
+    assert set(np.flatnonzero(pg)) == live
+    Jn = lambda x: O._inner(ub, O.rollout(rhs, u, make(x), 2e-3, 5))  # noqa: E731
+    fd = (Jn(x0 + h) - Jn(x0 - h)) / (2 * h)
+    for ck in (1, 2):
+        pg = np.zeros(3)
+        O.rollout_vjp(rhs, vjp, u, ub, make(x0), 2e-3, 5, checkpoint=ck, pgrad=pg)
+        assert abs(pg[slot] - fd) < 1e-7 * abs(fd), (name, pg, fd)
+
+
 def test_taylor_green_closed_form():
     """fields.py:71-87: the field is band-limited (|n_i| = 1), so its physical
     values come back exactly and it is divergence-free."""
