@@ -1,6 +1,7 @@
 // C ABI of the hit3d HIP library: argument checks, workspace layouts and the
 // launch sequences of the RHS / RK4 step (integrators.py:32-53,
 // navierstokes.py:62-97) and the diagnostics (diagnostics.py:45-114).
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -398,7 +399,7 @@ int rhs_impl(const hit3d_grid& gd, const cpx<T>* u_hat, cpx<T>* du_hat, double n
 template <class T>
 int rk4_impl(const hit3d_grid& gd, cpx<T>* u_hat, double nu, double dt, int64_t nsteps,
              const hit3d_forcing* f, bool pruned, double* diag, void* work, hipStream_t s,
-             PassTimer* tm = nullptr) {
+             PassTimer* tm = nullptr, bool integrating_factor = false) {
   const Geo g = make_geo(gd, pruned);
   size_t total;
   StepBuffers b = carve_step(gd, g, pruned, true, work, &total);
@@ -437,7 +438,7 @@ int rk4_impl(const hit3d_grid& gd, cpx<T>* u_hat, double nu, double dt, int64_t 
       a.us = us_in;
       a.out = stage == 4 ? u : us;
       int nb = 0;
-      H3_HIP(h3::launch_x_forward_stage<T>(g.nx, stage, a, s, &nb));
+      H3_HIP(h3::launch_x_forward_stage<T>(g.nx, integrating_factor ? stage + 4 : stage, a, s, &nb));
       mark(tm, kP5);
       if (sums) {
         job.nblocks = nb;
@@ -634,7 +635,10 @@ StepVjpBuffers carve_step_vjp(const hit3d_grid& gd, void* work, size_t* total) {
   b.y4 = c.take(st);
   b.kb = c.take(st);
   b.yb = c.take(st);
-  carve_step(gd, make_geo(gd, false), false, false, nullptr, &b.rhs_bytes);
+  size_t dense = 0, pruned = 0;  // the stage recomputation runs either layout
+  carve_step(gd, make_geo(gd, false), false, false, nullptr, &dense);
+  carve_step(gd, make_geo(gd, true), true, false, nullptr, &pruned);
+  b.rhs_bytes = std::max(dense, pruned);
   carve_vjp(gd, nullptr, &b.vjp_bytes);
   b.rhs_ws = c.take(b.rhs_bytes);
   b.vjp_ws = c.take(b.vjp_bytes);
@@ -645,7 +649,7 @@ StepVjpBuffers carve_step_vjp(const hit3d_grid& gd, void* work, size_t* total) {
 template <class T>
 int rk4_step_vjp_impl(const hit3d_grid& gd, const cpx<T>* u, const cpx<T>* ub_in, cpx<T>* out,
                       double nu, double dt, const hit3d_forcing* f, void* work, hipStream_t s,
-                      double* pgrad) {
+                      double* pgrad, bool pruned) {
   size_t total;
   StepVjpBuffers b = carve_step_vjp(gd, work, &total);
   const int64_t n = int64_t(3) * gd.nx * gd.ny * (gd.nz / 2 + 1);
@@ -654,13 +658,14 @@ int rk4_step_vjp_impl(const hit3d_grid& gd, const cpx<T>* u, const cpx<T>* ub_in
   *yb = P(b.yb);
   const T hdt = T(0.5 * dt), fdt = T(dt), sdt = T(dt / 6.0), tdt = T(dt / 3.0);
   H3_HIP(hipMemcpyAsync(ub, ub_in, n * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s));
-  // stage inputs (integrators.py:34-37), dense transforms
-  int rc = rhs_impl<T>(gd, u, k, nu, f, false, b.rhs_ws, s);
+  // stage inputs (integrators.py:34-37); the pruned transforms when u has no
+  // content outside the 2/3 box (then neither have the stage inputs)
+  int rc = rhs_impl<T>(gd, u, k, nu, f, pruned, b.rhs_ws, s);
   if (rc) return rc;
   H3_HIP(h3::launch_axpby<T>(y2, T(1), u, hdt, k, n, s));
-  if ((rc = rhs_impl<T>(gd, y2, k, nu, f, false, b.rhs_ws, s))) return rc;
+  if ((rc = rhs_impl<T>(gd, y2, k, nu, f, pruned, b.rhs_ws, s))) return rc;
   H3_HIP(h3::launch_axpby<T>(y3, T(1), u, hdt, k, n, s));
-  if ((rc = rhs_impl<T>(gd, y3, k, nu, f, false, b.rhs_ws, s))) return rc;
+  if ((rc = rhs_impl<T>(gd, y3, k, nu, f, pruned, b.rhs_ws, s))) return rc;
   H3_HIP(h3::launch_axpby<T>(y4, T(1), u, fdt, k, n, s));
   // reverse sweep: kbar_i from the combination weights, then stage adjoints 4..1
   H3_HIP(h3::launch_axpby<T>(kb, sdt, ub, T(0), ub, n, s));
@@ -939,6 +944,28 @@ int hit3d_rk4_steps(const hit3d_grid* grid, void* u_hat, double nu, double dt, i
                           pruned, diag, work, s);
 }
 
+int hit3d_rk4_if_steps(const hit3d_grid* grid, void* u_hat, double nu, double dt, int64_t nsteps,
+                       double t0, const hit3d_forcing* forcing, int32_t dealiased, double* diag,
+                       void* work, size_t work_bytes, void* stream) {
+  (void)t0;
+  H3_TRY(check_grid(grid));
+  H3_TRY(check_forcing(forcing));
+  H3_TRY(check_ptr(u_hat, "u_hat"));
+  if (nsteps < 0) return fail(HIT3D_ERR_INVALID, "nsteps must be >= 0");
+  if (!std::isfinite(dt) || !std::isfinite(nu) || nu < 0.0 || dt < 0.0)
+    return fail(HIT3D_ERR_INVALID, "dt and nu must be finite and non-negative");
+  const bool pruned = dealiased != 0;
+  size_t need;
+  carve_step(*grid, make_geo(*grid, pruned), pruned, true, nullptr, &need);
+  H3_TRY(check_work(work, work_bytes, need));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (grid->precision == 4)
+    return rk4_impl<float>(*grid, static_cast<cpx<float>*>(u_hat), nu, dt, nsteps, forcing, pruned,
+                           diag, work, s, nullptr, true);
+  return rk4_impl<double>(*grid, static_cast<cpx<double>*>(u_hat), nu, dt, nsteps, forcing, pruned,
+                          diag, work, s, nullptr, true);
+}
+
 int hit3d_shell_spectrum(const hit3d_grid* grid, const void* u_hat, int32_t component,
                          int32_t dealiased, double* out, void* work, size_t work_bytes,
                          void* stream) {
@@ -1127,7 +1154,8 @@ int hit3d_rhs_vjp(const hit3d_grid* grid, const void* u_hat, const void* v, void
 
 int hit3d_rk4_step_vjp(const hit3d_grid* grid, const void* u_hat, const void* ubar_next,
                        void* ubar, double nu, double dt, double t, const hit3d_forcing* forcing,
-                       double* param_grad, void* work, size_t work_bytes, void* stream) {
+                       int32_t dealiased, double* param_grad, void* work, size_t work_bytes,
+                       void* stream) {
   (void)t;
   H3_TRY(check_grid(grid));
   H3_TRY(check_forcing(forcing));
@@ -1145,11 +1173,11 @@ int hit3d_rk4_step_vjp(const hit3d_grid* grid, const void* u_hat, const void* ub
     return rk4_step_vjp_impl<float>(*grid, static_cast<const cpx<float>*>(u_hat),
                                     static_cast<const cpx<float>*>(ubar_next),
                                     static_cast<cpx<float>*>(ubar), nu, dt, forcing, work, s,
-                                    param_grad);
+                                    param_grad, dealiased != 0);
   return rk4_step_vjp_impl<double>(*grid, static_cast<const cpx<double>*>(u_hat),
                                    static_cast<const cpx<double>*>(ubar_next),
                                    static_cast<cpx<double>*>(ubar), nu, dt, forcing, work, s,
-                                   param_grad);
+                                   param_grad, dealiased != 0);
 }
 
 }  // extern "C"

@@ -441,6 +441,12 @@ __device__ __forceinline__ void block_sum3(double a, double b, double c, double*
   }
 }
 
+// exp(-x) in the state precision (x >= 0): the integrating factor
+template <class T>
+__device__ __forceinline__ T exp_neg(T x) {
+  if constexpr (sizeof(T) == 4) return expf(-x); else return exp(-x);
+}
+
 // ---------------------------------------------------------------------------
 // P5: x-forward of (nl0, nl1, nl2), then per retained mode
 //   du = P[mask (NL + f)] - nu k2 us          (navierstokes.py:89-95)
@@ -499,11 +505,47 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
     cpx<T> d2 = keep ? v2[s] + fc * s2 : czero<T>();
     const cpx<T> div = ik2 * (kxv * d0 + kyv * d1 + kzv * d2);
     const T nk2 = a.nu * k2;
-    d0 = d0 - kxv * div - nk2 * s0;
-    d1 = d1 - kyv * div - nk2 * s1;
-    d2 = d2 - kzv * div - nk2 * s2;
+    constexpr bool IF = MODE >= kStageIf1;
+    if constexpr (IF) {  // the integrating factor carries the viscous term
+      d0 = d0 - kxv * div;
+      d1 = d1 - kyv * div;
+      d2 = d2 - kzv * div;
+    } else {
+      d0 = d0 - kxv * div - nk2 * s0;
+      d1 = d1 - kyv * div - nk2 * s1;
+      d2 = d2 - kzv * div - nk2 * s2;
+    }
     cpx<T> o0, o1, o2;
-    if constexpr (MODE == kStageRhs) {
+    if constexpr (IF) {
+      // Lawson RK4 with E_h = exp(-nu k2 dt/2), E_f = E_h^2 (u: step start):
+      //   u2 = E_h (u + dt/2 k1),  u3 = E_h u + dt/2 k2,  u4 = E_f u + dt E_h k3,
+      //   u_new = E_f u + dt/6 (E_f k1 + 2 E_h (k2 + k3)) + dt/6 k4
+      const T eh = exp_neg<T>(T(0.5) * nk2 * a.dt), ef = eh * eh;
+      if constexpr (MODE == kStageIf4) {
+        o0 = a.acc[o] + sdt * d0;
+        o1 = a.acc[o + a.st_fs] + sdt * d1;
+        o2 = a.acc[o + 2 * a.st_fs] + sdt * d2;
+      } else {
+        const cpx<T> b0 = a.u[o], b1 = a.u[o + a.st_fs], b2 = a.u[o + 2 * a.st_fs];
+        if constexpr (MODE == kStageIf1) {
+          o0 = eh * (b0 + hdt * d0); o1 = eh * (b1 + hdt * d1); o2 = eh * (b2 + hdt * d2);
+          a.acc[o] = ef * (b0 + sdt * d0);
+          a.acc[o + a.st_fs] = ef * (b1 + sdt * d1);
+          a.acc[o + 2 * a.st_fs] = ef * (b2 + sdt * d2);
+        } else {
+          const T tde = tdt * eh;
+          if constexpr (MODE == kStageIf2) {
+            o0 = eh * b0 + hdt * d0; o1 = eh * b1 + hdt * d1; o2 = eh * b2 + hdt * d2;
+          } else {
+            const T fde = a.dt * eh;
+            o0 = ef * b0 + fde * d0; o1 = ef * b1 + fde * d1; o2 = ef * b2 + fde * d2;
+          }
+          a.acc[o] = a.acc[o] + tde * d0;
+          a.acc[o + a.st_fs] = a.acc[o + a.st_fs] + tde * d1;
+          a.acc[o + 2 * a.st_fs] = a.acc[o + 2 * a.st_fs] + tde * d2;
+        }
+      }
+    } else if constexpr (MODE == kStageRhs) {
       o0 = d0; o1 = d1; o2 = d2;
     } else if constexpr (MODE == kStage4) {
       o0 = a.acc[o] + sdt * d0;
@@ -646,6 +688,10 @@ hipError_t launch_x_forward_stage(int N, int mode, const StageArgs<T>& a, hipStr
       case kStage2: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage2, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a); break;
       case kStage3: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage3, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a); break;
       case kStage4: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage4, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a); break;
+      case kStageIf1: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageIf1, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a); break;
+      case kStageIf2: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageIf2, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a); break;
+      case kStageIf3: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageIf3, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a); break;
+      case kStageIf4: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageIf4, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a); break;
       default: return hipErrorInvalidValue;
     }
   });

@@ -67,7 +67,11 @@ struct StridedArgs {
 };
 
 // Fused x-forward + spectral operator + RK4 stage (pass P5).
-enum StageMode { kStageRhs = 0, kStage1 = 1, kStage2 = 2, kStage3 = 3, kStage4 = 4 };
+// Stage combinations of classic RK4 (1-4) and of the integrating-factor
+// (Lawson) RK4 (5-8), whose stages apply exp(-nu k2 dt/2) and exp(-nu k2 dt)
+// to the state and leave the viscous term out of the stage derivative.
+enum StageMode { kStageRhs = 0, kStage1 = 1, kStage2 = 2, kStage3 = 3, kStage4 = 4,
+                 kStageIf1 = 5, kStageIf2 = 6, kStageIf3 = 7, kStageIf4 = 8 };
 enum ForcingTerm { kTermNone = 0, kTermUniform = 1, kTermBand = 2 };
 
 template <class T>

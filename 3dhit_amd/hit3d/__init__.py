@@ -22,7 +22,7 @@ from .grid import (
     Grid, ddx, ddy, ddz, dealias, laplacian, make_grid, project, spectral_shape, to_physical,
     to_spectral,
 )
-from .integrators import Callback, Diagnostic, State, evolve, rk4_step, rollout
+from .integrators import Callback, Diagnostic, State, evolve, if_rk4_step, rk4_step, rollout
 from .labels import label, run_label
 from .navierstokes import NSParams, is_dealiased, linear_operator, make_ns_rhs
 from .report import load_run, read_series
@@ -34,7 +34,7 @@ __all__ = [
     "uniform_noise", "CONFIG_SPECTRUM_C",
     "NoForcing", "LinearForcing", "BandForcing", "LundgrenForcing",
     "NSParams", "make_ns_rhs", "linear_operator", "is_dealiased",
-    "rk4_step", "rollout", "evolve", "State", "Callback", "Diagnostic",
+    "rk4_step", "if_rk4_step", "rollout", "evolve", "State", "Callback", "Diagnostic",
     "FieldWriter", "save_series", "Hdf5Sink", "MemorySink", "load_run", "read_series",
     "label", "run_label",
     "energy", "enstrophy", "dissipation", "energy_spectrum", "component_spectra",

@@ -28,7 +28,7 @@ import torch
 
 from . import _lib
 from .forcing import BandForcing, LinearForcing
-from .grid import check_spectral, stream
+from .grid import DEALIASED_MARK, check_spectral, stream
 from .integrators import rollout as _rollout
 
 __all__ = ["make_ns_rhs_vjp", "rk4_step_vjp", "rollout_vjp", "differentiable_rollout",
@@ -68,11 +68,14 @@ def _pg_ptr(pg, like):
 
 
 def _step_vjp_into(grid, u, ub, params, dt, t, out, param_grad=None):
+    """The step's stage inputs are recomputed with the pruned transforms when
+    u carries the dealiased mark (rollout's checkpoints do): no device read."""
     ws = grid.workspace(_lib.OP_RK4_VJP)
     f = params.forcing.desc()
+    inside = 1 if getattr(u, DEALIASED_MARK, None) == u._version else 0
     _lib.call("hit3d_rk4_step_vjp", grid.desc, u.data_ptr(), ub.data_ptr(), out.data_ptr(),
-              float(params.nu), float(dt), float(t), ctypes.byref(f), _pg_ptr(param_grad, u),
-              ws.data_ptr(), ws.numel(), stream())
+              float(params.nu), float(dt), float(t), ctypes.byref(f), inside,
+              _pg_ptr(param_grad, u), ws.data_ptr(), ws.numel(), stream())
 
 
 def rk4_step_vjp(rhs, u_hat, ubar, params, dt, t, param_grad=None):

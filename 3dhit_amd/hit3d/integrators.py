@@ -17,14 +17,19 @@ from .callbacks import Callback, Diagnostic, State
 from .grid import check_spectral, mark_dealiased, stream
 from .navierstokes import known_dealiased
 
-__all__ = ["rk4_step", "rollout", "evolve", "State", "Callback", "Diagnostic"]
+__all__ = ["rk4_step", "if_rk4_step", "rollout", "evolve", "State", "Callback", "Diagnostic"]
 
 
-def _steps(rhs, u_hat, params, dt, nsteps, t0, diag=None):
+_SCHEMES = {"rk4": "hit3d_rk4_steps", "if-rk4": "hit3d_rk4_if_steps"}
+
+
+def _steps(rhs, u_hat, params, dt, nsteps, t0, diag=None, scheme="rk4"):
     """`nsteps` steps of hit3d_rk4_steps on a copy of u_hat. The layout
     (pruned when u_hat has no content outside the 2/3 box) is decided from
     the tensor's mark when it carries one, so a chain of rollouts on the
     library's own outputs never waits for the device."""
+    if scheme not in _SCHEMES:
+        raise ValueError(f"scheme must be one of {sorted(_SCHEMES)}, got {scheme!r}")
     g = rhs.grid
     inside = known_dealiased(u_hat, g)
     u = check_spectral(u_hat, g, 3).clone()
@@ -33,7 +38,7 @@ def _steps(rhs, u_hat, params, dt, nsteps, t0, diag=None):
     pruned = 1 if inside else 0
     ws = g.workspace(_lib.OP_RK4, 3, pruned)
     f = params.forcing.desc()
-    _lib.call("hit3d_rk4_steps", g.desc, u.data_ptr(), float(params.nu), float(dt), int(nsteps),
+    _lib.call(_SCHEMES[scheme], g.desc, u.data_ptr(), float(params.nu), float(dt), int(nsteps),
               float(t0), ctypes.byref(f), pruned,
               None if diag is None else diag.data_ptr(), ws.data_ptr(), ws.numel(), stream())
     return mark_dealiased(u) if pruned else u
@@ -44,12 +49,23 @@ def rk4_step(rhs, u_hat, params, dt, t):
     return _steps(rhs, u_hat, params, dt, 1, t)
 
 
-def rollout(rhs, u_hat, params, dt, nsteps, t0=0.0):
+def rollout(rhs, u_hat, params, dt, nsteps, t0=0.0, *, scheme="rk4"):
     """integrators.py:41-53: `nsteps` RK4 steps from t0 (pure: the input is
-    not modified)."""
+    not modified). scheme="if-rk4" takes the integrating-factor (Lawson) form
+    instead: the viscous operator linear_operator(grid, params) = -nu k2 is
+    integrated exactly and classic RK4 advances the rest (the option the
+    reference leaves open at integrators.py:14-16)."""
     if int(nsteps) < 0:
         raise ValueError("nsteps must be >= 0")
-    return _steps(rhs, u_hat, params, dt, int(nsteps), t0)
+    return _steps(rhs, u_hat, params, dt, int(nsteps), t0, scheme=scheme)
+
+
+def if_rk4_step(rhs, u_hat, params, dt, t):
+    """One integrating-factor (Lawson) RK4 step: with E_h = exp(-nu k2 dt/2)
+    and N = rhs + nu k2 (the non-viscous part of the RHS),
+    u2 = E_h (u + dt/2 k1), u3 = E_h u + dt/2 k2, u4 = E_h^2 u + dt E_h k3,
+    u_new = E_h^2 u + dt/6 (E_h^2 k1 + 2 E_h (k2 + k3) + k4), k_i = N(u_i)."""
+    return _steps(rhs, u_hat, params, dt, 1, t, scheme="if-rk4")
 
 
 # ---------------------------------------------------------------------------

@@ -484,6 +484,8 @@ def main(argv=None):
     ap.add_argument("--out", default=None, help="write the result JSON here")
     ap.add_argument("--save", default=None, help="write the trained closure (.npz) here")
     ap.add_argument("--holdout-time", dest="holdout_time", type=float, default=None)
+    ap.add_argument("--stride", type=int, default=None, help="DNS steps per snapshot and per LES step")
+    ap.add_argument("--calib-steps", dest="calib_steps", type=int, default=None)
     ap.add_argument("--nsnap", type=int, default=None)
     ap.add_argument("--epochs", type=int, default=None)
     ap.add_argument("--windows", type=int, default=None)
@@ -495,7 +497,7 @@ def main(argv=None):
                   nsnap=200, holdout_time=1.6, epochs=2, windows=6, attractor_windows=4, spin=20,
                   window=10, calib_steps=100)
     kw["save"] = a.save
-    for k in ("nsnap", "epochs", "windows", "attractor_windows", "holdout_time"):
+    for k in ("nsnap", "epochs", "windows", "attractor_windows", "holdout_time", "stride", "calib_steps"):
         if getattr(a, k) is not None:
             kw[k] = getattr(a, k)
     res = run(**kw)

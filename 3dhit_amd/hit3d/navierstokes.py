@@ -94,5 +94,7 @@ def make_ns_rhs(grid):
 
 
 def linear_operator(grid, params):
-    """navierstokes.py:100-103: -nu k^2."""
+    """navierstokes.py:100-103: -nu k^2 (the diagonal viscous operator that
+    the integrating-factor scheme, rollout(..., scheme="if-rk4"), integrates
+    exactly; its kernels form exp(-nu k2 dt) per mode from the same k2)."""
     return -float(params.nu) * grid.k2

@@ -124,6 +124,16 @@ int hit3d_rk4_steps(const hit3d_grid* grid, void* u_hat, double nu, double dt, i
                     double t0, const hit3d_forcing* forcing, int32_t dealiased, double* diag,
                     void* work, size_t work_bytes, void* stream);
 
+/* hit3d_rk4_steps with the integrating-factor (Lawson) form of classic RK4:
+   the viscous operator -nu k2 of linear_operator (navierstokes.py:100-103,
+   the hook integrators.py:14-16 leaves for such schemes) is integrated
+   exactly through exp(-nu k2 dt/2) and exp(-nu k2 dt), and RK4 advances the
+   rest of the RHS. Same arguments, workspace and diag as hit3d_rk4_steps;
+   nu >= 0, dt >= 0. */
+int hit3d_rk4_if_steps(const hit3d_grid* grid, void* u_hat, double nu, double dt, int64_t nsteps,
+                       double t0, const hit3d_forcing* forcing, int32_t dealiased, double* diag,
+                       void* work, size_t work_bytes, void* stream);
+
 /* FP64 shell spectrum (diagnostics.py:85-114): out[s] (device, nbins doubles)
    = sum over modes of shell s of w |u_c|^2 / (2 Ntot^2), summed over
    components (component = -1) or for one component (0..2). Bin 0 holds the
@@ -186,12 +196,15 @@ int hit3d_rhs_vjp(const hit3d_grid* grid, const void* u_hat, const void* v, void
 /* Adjoint of one classic RK4 step (integrators.py:32-38): from the step's
    start state u_hat and the cotangent ubar_next of its result, writes the
    cotangent ubar of u_hat. Recomputes the stage inputs, then runs the four
-   stage adjoints in reverse. ubar may alias ubar_next. `param_grad` (device,
-   3 doubles, or NULL) accumulates the step's parameter derivatives as in
-   hit3d_rhs_vjp, summed over the four stages. */
+   stage adjoints in reverse. ubar may alias ubar_next. `dealiased` = 1
+   promises that u_hat has no content outside the 2/3 box (as for hit3d_rhs):
+   the stage inputs are then recomputed with the pruned transforms. `param_grad`
+   (device, 3 doubles, or NULL) accumulates the step's parameter derivatives
+   as in hit3d_rhs_vjp, summed over the four stages. */
 int hit3d_rk4_step_vjp(const hit3d_grid* grid, const void* u_hat, const void* ubar_next,
                        void* ubar, double nu, double dt, double t, const hit3d_forcing* forcing,
-                       double* param_grad, void* work, size_t work_bytes, void* stream);
+                       int32_t dealiased, double* param_grad, void* work, size_t work_bytes,
+                       void* stream);
 
 /* Measurement kernel (no reference counterpart): dst = src over `bytes`
    (a multiple of 16; both pointers 16-byte aligned) by a plain streaming

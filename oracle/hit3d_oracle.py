@@ -352,11 +352,36 @@ def rk4_step(rhs, u_hat, params, dt, t):
     return u_hat + (dt / 6.0) * (k1 + 2 * k2 + 2 * k3 + k4)
 
 
-def rollout(rhs, u_hat, params, dt, nsteps, t0=0.0, observe=None):
-    """integrators.py:41-53 (scan over rk4_step). `observe(n, u)` is an
-    oracle-only hook called after every step (used for spectra cadences)."""
+def linear_operator(grid, params):
+    """navierstokes.py:100-103: -nu k2."""
+    return -params.nu * grid.k2
+
+
+def if_rk4_step(rhs, u_hat, params, dt, t):
+    """Integrating-factor (Lawson) RK4 with the hook navierstokes.py:100-103
+    (integrators.py:14-16 leaves the scheme open): with L = linear_operator,
+    E_h = exp(L dt/2) and N(u) = rhs(u) - L u, classic RK4 on
+    v = exp(-L (s - t)) u gives
+      u2 = E_h (u + dt/2 k1), u3 = E_h u + dt/2 k2, u4 = E_h^2 u + dt E_h k3,
+      u_new = E_h^2 u + dt/6 (E_h^2 k1 + 2 E_h (k2 + k3) + k4),  k_i = N(u_i)."""
+    L = linear_operator(rhs.grid, params)
+    eh = np.exp(0.5 * dt * L)
+    ef = eh * eh
+    N = lambda u, tt: rhs(u, params, tt) - L * u  # noqa: E731
+    k1 = N(u_hat, t)
+    k2 = N(eh * (u_hat + 0.5 * dt * k1), t + 0.5 * dt)
+    k3 = N(eh * u_hat + 0.5 * dt * k2, t + 0.5 * dt)
+    k4 = N(ef * u_hat + dt * eh * k3, t + dt)
+    return ef * u_hat + (dt / 6.0) * (ef * k1 + 2 * eh * (k2 + k3) + k4)
+
+
+def rollout(rhs, u_hat, params, dt, nsteps, t0=0.0, observe=None, step=None):
+    """integrators.py:41-53 (scan over rk4_step, or over `step`, e.g.
+    if_rk4_step). `observe(n, u)` is an oracle-only hook called after every
+    step (used for spectra cadences)."""
+    step = rk4_step if step is None else step
     for i in range(nsteps):
-        u_hat = rk4_step(rhs, u_hat, params, dt, t0 + i * dt)
+        u_hat = step(rhs, u_hat, params, dt, t0 + i * dt)
         if observe is not None:
             observe(i + 1, u_hat)
     return u_hat

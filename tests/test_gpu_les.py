@@ -345,17 +345,17 @@ GOLDEN_CLOSURE = __import__("os").path.join(__import__("os").path.dirname(__file
                                             "les_closure_64.npz")
 
 
-def _les_from_target(net_or_none, target, e_target, steps, seed=5):
-    """64^3 LES (|k| <= 21, nu = 1/2000, dt = 0.01, resolved energy held at
-    the training record's mean) from a random-phase field carrying the
-    target spectrum; returns the spectra (shells 1..21) every 10 steps."""
+def _les_from_target(net_or_none, target, e_target, steps, dt, seed=5):
+    """64^3 LES (|k| <= 21, nu = 1/2000, step dt, resolved energy held at the
+    training record's mean) from a random-phase field carrying the target
+    spectrum; returns the spectra (shells 1..21) every 10 steps."""
     from hit3d.les_experiment import _spectrum
     g = h3.make_grid(64, dtype=torch.float32)
     spec = lambda k: np.where((k >= 0.5) & (k <= 21.5), np.interp(k, np.arange(1, 22), target), 0.0)  # noqa: E731
     u = les.filter_state(h3.random_field(seed, g, spec), g, g, 21.0)
     params = h3.NSParams(nu=1 / 2000, forcing=les.HoldResolvedEnergy(e_target, 1.0))
     out = []
-    les.les_rollout(u, h3.make_ns_rhs(g), params, net_or_none, 0.01, steps, every=10,
+    les.les_rollout(u, h3.make_ns_rhs(g), params, net_or_none, dt, steps, every=10,
                     mask=les.sphere_mask(g, 21.0), observe=lambda n, v: out.append(_spectrum(v, g, 21)))
     return np.asarray(out)
 
@@ -370,10 +370,12 @@ def test_les_with_the_committed_closure_reproduces_the_filtered_dns_spectrum():
     g = h3.make_grid(64, dtype=torch.float32)
     net, target, e_target, config = load_closure(GOLDEN_CLOSURE, g)
     assert config["les_n"] == 64 and config["kcut"] == 21.0 and len(target) == 21
-    spectra = _les_from_target(net, target, e_target, 2000)
-    E = spectra[50:].mean(0)                      # steps 500..2000 (15 time units)
+    dt = config["dt_les"]
+    n = int(round(20.0 / dt))                     # 20 time units, the first 5 discarded
+    spectra = _les_from_target(net, target, e_target, n, dt)
+    E = spectra[n // 40:].mean(0)
     err = np.abs(E / target - 1.0)
-    bare = _les_from_target(None, target, e_target, 500)[25:].mean(0)
+    bare = _les_from_target(None, target, e_target, n // 4, dt)[n // 80:].mean(0)
     err_bare = np.abs(bare / target - 1.0)
     print(f"LES with the committed closure vs the filtered DNS mean: max rel err k=1..2 "
           f"{err[:2].max():.3f}, k=3..21 {err[2:].max():.3f} (k={int(np.argmax(err[2:])) + 3}); "

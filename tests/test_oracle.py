@@ -496,3 +496,52 @@ def test_forced_256_record_matches_the_oracle():
     assert np.allclose(rec["eps"], 2 * float(rec["nu"]) * rec["Omega"], rtol=1e-15)
     assert np.max(np.abs(rec["E"] - 0.5)) < 1e-12
     assert np.allclose(rec["spectra"].sum(axis=1), rec["E"], rtol=1e-12)
+
+
+# --- integrating-factor (Lawson) RK4: the scheme navierstokes.py:100-103 leaves room for ---
+
+
+def test_if_rk4_reduces_to_rk4_without_viscosity():
+    g = O.make_grid(16)
+    u = O.random_field(3, g, O.config_spectrum)
+    rhs, p = O.make_ns_rhs(g), O.NSParams(nu=0.0, forcing=O.LinearForcing(A=0.1))
+    assert np.allclose(O.if_rk4_step(rhs, u, p, 1e-2, 0.0), O.rk4_step(rhs, u, p, 1e-2, 0.0),
+                       rtol=0, atol=1e-14 * np.abs(u).max())
+
+
+def test_if_rk4_is_exact_for_pure_viscous_decay():
+    """Beltrami (u x w = 0): the stage derivative N vanishes, so the scheme
+    reproduces exp(-nu t) at any step size, where RK4 carries its
+    truncation error (reference tests/test_navierstokes.py:22-32)."""
+    g = O.make_grid(16)
+    u0 = O.beltrami(g)
+    rhs, p = O.make_ns_rhs(g), O.NSParams(nu=0.5)
+    u = O.rollout(rhs, u0, p, 0.4, 5, step=O.if_rk4_step)
+    exact = np.exp(-0.5 * 2.0) * u0
+    assert np.max(np.abs(u - exact)) < 1e-13 * np.max(np.abs(u0))
+    u_rk4 = O.rollout(rhs, u0, p, 0.4, 5)
+    assert np.max(np.abs(u_rk4 - exact)) > 1e-6 * np.max(np.abs(u0))
+
+
+def test_if_rk4_is_fourth_order():
+    """Global error at t = 0.2 against a 64-step solution of the same
+    scheme: halving dt divides it by about 16."""
+    g = O.make_grid(16)
+    u0 = O.random_field(5, g, O.config_spectrum)
+    rhs, p = O.make_ns_rhs(g), O.NSParams(nu=0.05, forcing=O.LundgrenForcing(nu=0.05))
+    ref = O.rollout(rhs, u0, p, 0.2 / 64, 64, step=O.if_rk4_step)
+    e = [np.linalg.norm(O.rollout(rhs, u0, p, 0.2 / n, n, step=O.if_rk4_step) - ref) for n in (4, 8)]
+    assert 12 < e[0] / e[1] < 20
+
+
+def test_if_rk4_is_stable_where_rk4_is_not():
+    """nu k2 dt = 3.75 on the highest retained modes (beyond RK4's real-axis
+    limit 2.79): the integrating factor keeps the viscous decay, RK4 grows."""
+    g = O.make_grid(16)
+    u0 = O.random_field(6, g, O.config_spectrum)
+    rhs, p = O.make_ns_rhs(g), O.NSParams(nu=1.0)
+    E0 = O.energy(u0, g)
+    assert O.energy(O.rollout(rhs, u0, p, 0.05, 10, step=O.if_rk4_step), g) < E0
+    with np.errstate(all="ignore"):
+        E_rk4 = O.energy(O.rollout(rhs, u0, p, 0.05, 10), g)
+    assert not E_rk4 <= 10 * E0          # grows without bound (overflow to nan included)
