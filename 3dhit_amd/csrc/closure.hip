@@ -134,6 +134,129 @@ __global__ __launch_bounds__(64) void k_conv_fwd(const __bf16* __restrict__ X,
 }
 
 // ---------------------------------------------------------------------------
+// Forward on a tile staged in LDS: a workgroup of 8 waves computes a
+// 4 x 8 x 16 block of voxels (wave w: plane a0 + w/2, rows b0 + 4(w%2) ..
+// + 3, 16 voxels per row as in k_conv_fwd). Its periodic halo of 6 x 10 x 18
+// voxels (all CIN channels) and the layer's packed weights are loaded into
+// LDS once, so the K loop reads both MFMA operands from LDS: each input row
+// is fetched from memory by one workgroup (plus the halo overlap) instead of
+// by the waves of its 27 neighbours, and the weights once per workgroup
+// instead of once per wave. Weight rows are padded by 16 bytes so that the
+// 16 rows one A read touches fall in distinct banks. Needs D % 4 == 0,
+// H % 8 == 0, W % 16 == 0. Same products and summation order as k_conv_fwd.
+// ---------------------------------------------------------------------------
+constexpr int kTileA = 4, kTileB = 8, kTileC = 16, kTileWaves = 8;
+constexpr int kHaloA = kTileA + 2, kHaloB = kTileB + 2, kHaloC = kTileC + 2;
+
+template <int CIN, int COUT>
+__global__ __launch_bounds__(64 * kTileWaves) void k_conv_fwd_tiled(const __bf16* __restrict__ X,
+                                                                    const __bf16* __restrict__ Wp,
+                                                                    const float* __restrict__ bias,
+                                                                    int cout_real, ConvGeom g, int mode,
+                                                                    float* __restrict__ pre,
+                                                                    __bf16* __restrict__ act,
+                                                                    float* __restrict__ out) {
+  constexpr int NT = 64 * kTileWaves;
+  constexpr int RB = COUT / 16;
+  constexpr int KS = (27 * CIN + 31) / 32;
+  constexpr int CHUNKS = CIN / 8;                    // 16-byte chunks per voxel row
+  constexpr int HALO = kHaloA * kHaloB * kHaloC;
+  constexpr int WROW = 4 * KS;                       // 16-byte chunks per weight row
+  constexpr int WROWP = WROW + 1;                    // padded row
+  constexpr int NHALO = HALO * CHUNKS, NW = COUT * WROW;
+  constexpr int NLOAD = (NHALO + NW + NT - 1) / NT;
+  static_assert(CIN == 16 || CIN == 32, "CIN must be 16 or 32");
+  __shared__ bf16x8 tile[NHALO];
+  __shared__ bf16x8 wts[COUT * WROWP];
+  const int tid = threadIdx.x, wave = tid / 64, l = tid % 64, li = l % 16, lg = l / 16;
+  const int nc = g.W / kTileC, nbk = g.H / kTileB;
+  const int c0 = (blockIdx.x % nc) * kTileC;
+  const int b0 = ((blockIdx.x / nc) % nbk) * kTileB;
+  const int a0 = (blockIdx.x / (nc * nbk)) * kTileA;
+  const int wa = wave / 2, wb = 4 * (wave % 2);      // this wave's plane and first row in the tile
+  // stage halo and weights: all loads of a thread issue before its LDS stores
+  bf16x8 stage[NLOAD];
+#pragma unroll
+  for (int j = 0; j < NLOAD; ++j) {
+    const int i = tid + NT * j;
+    if (i < NHALO) {
+      const int t = i / CHUNKS, ch = i % CHUNKS;
+      const int tc = t % kHaloC, tb = (t / kHaloC) % kHaloB, ta = t / (kHaloC * kHaloB);
+      const int n = (wrap(a0 - 1 + ta, g.D) * g.H + wrap(b0 - 1 + tb, g.H)) * g.W + wrap(c0 - 1 + tc, g.W);
+      stage[j] = *reinterpret_cast<const bf16x8*>(X + int64_t(n) * CIN + 8 * ch);
+    } else if (i < NHALO + NW) {
+      stage[j] = reinterpret_cast<const bf16x8*>(Wp)[i - NHALO];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NLOAD; ++j) {
+    const int i = tid + NT * j;
+    if (i < NHALO) {
+      tile[i] = stage[j];
+    } else if (i < NHALO + NW) {
+      const int w = i - NHALO;
+      wts[(w / WROW) * WROWP + w % WROW] = stage[j];
+    }
+  }
+  __syncthreads();
+
+  f32x4 acc[RB][4];
+#pragma unroll
+  for (int r = 0; r < RB; ++r)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[r][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  auto load = [&](int s, bf16x8* A, bf16x8* B) {
+    const int k = 32 * s + 8 * lg;
+    const int o = k / CIN, ci = k % CIN;
+#pragma unroll
+    for (int r = 0; r < RB; ++r) A[r] = wts[(16 * r + li) * WROWP + 4 * s + lg];
+    const int da = o / 9, db = (o / 3) % 3, dc = o % 3;   // offsets + 1
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      if (o < 27) {
+        const int t = ((wa + da) * kHaloB + (wb + b + db)) * kHaloC + (li + dc);
+        B[b] = tile[t * CHUNKS + ci / 8];
+      } else {
+        B[b] = bf16x8{};
+      }
+    }
+  };
+  bf16x8 A0[RB], B0[4], A1[RB], B1[4];
+  load(0, A0, B0);
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    bf16x8* A = (s & 1) ? A1 : A0;
+    bf16x8* B = (s & 1) ? B1 : B0;
+    if (s + 1 < KS) load(s + 1, (s & 1) ? A0 : A1, (s & 1) ? B0 : B1);
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[r][b] = mfma(A[r], B[b], acc[r][b]);
+  }
+#pragma unroll
+  for (int r = 0; r < RB; ++r)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int v = ((a0 + wa) * g.H + (b0 + wb + b)) * g.W + c0 + li;
+      const int co0 = 16 * r + 4 * lg;
+      float y[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) y[e] = acc[r][b][e] + (co0 + e < cout_real ? bias[co0 + e] : 0.f);
+      if (mode == 0) {
+        *reinterpret_cast<float4*>(pre + int64_t(v) * COUT + co0) = make_float4(y[0], y[1], y[2], y[3]);
+        __bf16 h[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) h[e] = (__bf16)(co0 + e < cout_real ? gelu(y[e]) : 0.f);
+        *reinterpret_cast<uint2*>(act + int64_t(v) * COUT + co0) = *reinterpret_cast<uint2*>(h);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (co0 + e < cout_real) out[int64_t(v) * cout_real + co0 + e] = y[e];
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Backward data: dX[u][ci] = sum_{o,co} w[co][o][ci] dP[u - o][co]
 // (D[ci][u] = WT[ci][k] * dP[u - o(k)][co(k)], k = o*COUT + co).
 // dP bf16 [V][COUT]; dX fp32 [V][CIN].
@@ -492,8 +615,17 @@ hipError_t launch_conv_fwd(int cin, int cout, const __bf16* X, const __bf16* Wp,
                            const float* bias, int cout_real, const ConvGeom& g, int mode, float* pre,
                            __bf16* act, float* out, hipStream_t s) {
   const unsigned nb = unsigned(int64_t(g.D) * g.H * g.W / 64);
-#define H3_CONV_FWD(CI, CO) \
-  hipLaunchKernelGGL((k_conv_fwd<CI, CO>), dim3(nb), dim3(64), 0, s, X, Wp, KP, bias, cout_real, g, mode, pre, act, out)
+  const bool tiled = g.D % kTileA == 0 && g.H % kTileB == 0 && g.W % kTileC == 0;
+#define H3_CONV_FWD(CI, CO)                                                                                \
+  do {                                                                                                     \
+    if (tiled)                                                                                             \
+      hipLaunchKernelGGL((k_conv_fwd_tiled<CI, CO>), dim3(nb / (kTileA * kTileB * kTileC / 64)),           \
+                         dim3(64 * kTileWaves), 0, s, X, Wp, bias, cout_real,                              \
+                         g, mode, pre, act, out);                                                          \
+    else                                                                                                   \
+      hipLaunchKernelGGL((k_conv_fwd<CI, CO>), dim3(nb), dim3(64), 0, s, X, Wp, KP, bias, cout_real, g,     \
+                         mode, pre, act, out);                                                             \
+  } while (0)
   if (cin == 16 && cout == 32) H3_CONV_FWD(16, 32);
   else if (cin == 32 && cout == 32) H3_CONV_FWD(32, 32);
   else if (cin == 32 && cout == 16) H3_CONV_FWD(32, 16);

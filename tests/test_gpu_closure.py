@@ -70,6 +70,26 @@ def test_conv_forward_matches_torch(cin, cout, gelu):
         assert rel(out, ref) < 1e-5
 
 
+@pytest.mark.parametrize("dhw", [(8, 16, 32), (4, 8, 64), (2, 32, 16), (16, 12, 32)])
+@pytest.mark.parametrize("cin,cout", [(9, 32), (32, 32), (32, 6)])
+def test_conv_forward_tiled_and_fallback_shapes(dhw, cin, cout):
+    """The LDS-tiled forward (D % 4 == 0, H % 8 == 0: 4 x 8 x 16 voxel tiles
+    with their periodic halo) and the per-wave fallback (the other shapes)
+    against torch on anisotropic grids, where a transposed tile axis or a
+    halo wrapped on the wrong axis would show."""
+    d, h, wd = dhw
+    v = d * h * wd
+    L = ConvLayer(dhw, cin, cout, cout == 32, "cuda", torch.Generator().manual_seed(7 * cin + cout))
+    L.b.copy_(torch.linspace(-0.5, 0.5, cout))
+    g = torch.Generator().manual_seed(3)
+    x = torch.zeros(v, L.desc.cin_pad)
+    x[:, :cin] = torch.randn(v, cin, generator=g)
+    x = x.to(torch.bfloat16).cuda()
+    ref = torch_conv(x.float(), bf(L.w), L.b, dhw)
+    got = L.forward(x)[0][:, :cout] if cout == 32 else L.forward(x)
+    assert rel(got, ref) < 1e-5
+
+
 def _torch_grads(x, w, b, gout_pre):
     """Gradients of sum(conv(x) * gout_pre) w.r.t. x, w, b in float32."""
     xr = x.clone().requires_grad_(True)
