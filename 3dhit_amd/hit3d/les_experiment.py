@@ -478,8 +478,52 @@ def run(fine_n=512, les_n=64, kcut=21.0, nu=1 / 2000, dt_dns=2e-3, stride=5, spi
     return res
 
 
+def drift(path, time_units=120.0, variants=("as saved", "gain x0.85", "gain x1.15", "cut k < 2.5", "cut k < 4.5"),
+          seed=5, windows=((10, 40), (40, 80), (80, 120))):
+    """Long-run behaviour of a saved closure: an LES from a random-phase
+    field with the saved target spectrum (resolved energy held at its
+    mean), run for `time_units` with the closure as saved, with its gain
+    scaled, or with its force removed below |k| = c ("cut k < c"); per
+    window, the time-averaged spectrum's error against the target."""
+    out = []
+    les_n = int(json.loads(bytes(np.load(path)["config"]).decode())["les_n"])
+    for variant in variants:
+        grid = make_grid(les_n, dtype=torch.float32)
+        net, target, e_target, config = load_closure(path, grid)
+        dt = config["dt_les"]
+        if variant.startswith("gain x"):
+            with torch.no_grad():
+                net.gain.mul_(float(variant[6:]))
+        elif variant.startswith("cut k < "):
+            net.spectral_shape().mul_((torch.sqrt(grid.k2) >= float(variant[8:])).to(grid.dtype))
+        elif variant != "as saved":
+            raise ValueError(f"unknown variant {variant!r}")
+        kmax = len(target)
+        spec = lambda k: np.where((k >= 0.5) & (k <= kmax + 0.5),  # noqa: E731
+                                  np.interp(k, np.arange(1, kmax + 1), target), 0.0)
+        u = filter_state(random_field(seed, grid, spec), grid, grid, float(config["kcut"]))
+        params = NSParams(nu=config["nu"], forcing=HoldResolvedEnergy(e_target, 1.0))
+        spectra = []
+        les_rollout(u, make_ns_rhs(grid), params, net, dt, int(round(time_units / dt)), every=10,
+                    mask=sphere_mask(grid, float(config["kcut"])),
+                    observe=lambda n, v: spectra.append(_spectrum(v, grid, kmax)))
+        S, per = np.asarray(spectra), 10 * dt
+        row = {"variant": variant, "dt_les": dt, "windows": []}
+        for a, b in windows:
+            if b > time_units + 1e-9:
+                continue
+            err = S[int(round(a / per)):int(round(b / per))].mean(0) / target - 1.0
+            row["windows"].append({"from": a, "to": b, "max_abs_err": float(np.abs(err).max()),
+                                   "rel_err": err.tolist()})
+        _log(f"{variant}: " + ", ".join(f"[{w['from']},{w['to']}] {w['max_abs_err']:.3f}" for w in row["windows"]))
+        out.append(row)
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--drift", default=None, metavar="CLOSURE_NPZ",
+                    help="instead of the experiment: the long-run drift of a saved closure")
     ap.add_argument("--quick", action="store_true", help="small smoke configuration (128^3 -> 32^3)")
     ap.add_argument("--out", default=None, help="write the result JSON here")
     ap.add_argument("--save", default=None, help="write the trained closure (.npz) here")
@@ -491,6 +535,14 @@ def main(argv=None):
     ap.add_argument("--windows", type=int, default=None)
     ap.add_argument("--attractor-windows", dest="attractor_windows", type=int, default=None)
     a = ap.parse_args(argv)
+    if a.drift:
+        res = drift(a.drift)
+        line = json.dumps(res)
+        if a.out:
+            with open(a.out, "w") as f:
+                f.write(line + "\n")
+        print(line, flush=True)
+        return
     kw = {}
     if a.quick:
         kw = dict(fine_n=128, les_n=32, kcut=10.0, nu=1 / 500, dt_dns=2e-3, stride=4, spinup_time=4.0,
