@@ -399,7 +399,7 @@ int rhs_impl(const hit3d_grid& gd, const cpx<T>* u_hat, cpx<T>* du_hat, double n
 template <class T>
 int rk4_impl(const hit3d_grid& gd, cpx<T>* u_hat, double nu, double dt, int64_t nsteps,
              const hit3d_forcing* f, bool pruned, double* diag, void* work, hipStream_t s,
-             PassTimer* tm = nullptr, bool integrating_factor = false) {
+             PassTimer* tm = nullptr, bool integrating_factor = false, cpx<T>* record = nullptr) {
   const Geo g = make_geo(gd, pruned);
   size_t total;
   StepBuffers b = carve_step(gd, g, pruned, true, work, &total);
@@ -429,7 +429,15 @@ int rk4_impl(const hit3d_grid& gd, cpx<T>* u_hat, double nu, double dt, int64_t 
   // and only the last stage's sums need a launch of their own.
   h3::FinalizeJob job{b.partials, 0, finalize_args(gd, f), b.scalars};
   bool pending = false;
+  // record: per step, the state at its start and the inputs of stages 2..4,
+  // in the API layout (the linearization points of the step's adjoint)
+  auto keep = [&](const cpx<T>* src, int64_t n, int slot) -> hipError_t {
+    cpx<T>* dst = record + (4 * n + slot) * 3 * full_fs;
+    if (pruned) return h3::launch_scatter<T>(src, st_fs, dst, full_fs, 3, g, s);
+    return hipMemcpyAsync(dst, src, size_t(3 * full_fs) * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s);
+  };
   for (int64_t n = 0; n < nsteps; ++n) {
+    if (record) H3_HIP(keep(u, n, 0));
     for (int stage = 1; stage <= 4; ++stage) {
       const cpx<T>* us_in = stage == 1 ? u : us;
       int rc = rhs_passes<T>(gd, g, us_in, bufA, bufB, s, tm, pending ? &job : nullptr);
@@ -440,6 +448,7 @@ int rk4_impl(const hit3d_grid& gd, cpx<T>* u_hat, double nu, double dt, int64_t 
       int nb = 0;
       H3_HIP(h3::launch_x_forward_stage<T>(g.nx, integrating_factor ? stage + 4 : stage, a, s, &nb));
       mark(tm, kP5);
+      if (record && stage < 4) H3_HIP(keep(us, n, stage));
       if (sums) {
         job.nblocks = nb;
         pending = true;
@@ -649,24 +658,30 @@ StepVjpBuffers carve_step_vjp(const hit3d_grid& gd, void* work, size_t* total) {
 template <class T>
 int rk4_step_vjp_impl(const hit3d_grid& gd, const cpx<T>* u, const cpx<T>* ub_in, cpx<T>* out,
                       double nu, double dt, const hit3d_forcing* f, void* work, hipStream_t s,
-                      double* pgrad, bool pruned) {
+                      double* pgrad, bool pruned, const cpx<T>* stages = nullptr) {
   size_t total;
   StepVjpBuffers b = carve_step_vjp(gd, work, &total);
   const int64_t n = int64_t(3) * gd.nx * gd.ny * (gd.nz / 2 + 1);
   auto P = [](void* p) { return static_cast<cpx<T>*>(p); };
-  cpx<T>*ub = P(b.ub), *k = P(b.k), *y2 = P(b.y2), *y3 = P(b.y3), *y4 = P(b.y4), *kb = P(b.kb),
-  *yb = P(b.yb);
+  cpx<T>*ub = P(b.ub), *k = P(b.k), *kb = P(b.kb), *yb = P(b.yb);
+  const cpx<T>*y2 = P(b.y2), *y3 = P(b.y3), *y4 = P(b.y4);
   const T hdt = T(0.5 * dt), fdt = T(dt), sdt = T(dt / 6.0), tdt = T(dt / 3.0);
   H3_HIP(hipMemcpyAsync(ub, ub_in, n * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s));
-  // stage inputs (integrators.py:34-37); the pruned transforms when u has no
-  // content outside the 2/3 box (then neither have the stage inputs)
-  int rc = rhs_impl<T>(gd, u, k, nu, f, pruned, b.rhs_ws, s);
-  if (rc) return rc;
-  H3_HIP(h3::launch_axpby<T>(y2, T(1), u, hdt, k, n, s));
-  if ((rc = rhs_impl<T>(gd, y2, k, nu, f, pruned, b.rhs_ws, s))) return rc;
-  H3_HIP(h3::launch_axpby<T>(y3, T(1), u, hdt, k, n, s));
-  if ((rc = rhs_impl<T>(gd, y3, k, nu, f, pruned, b.rhs_ws, s))) return rc;
-  H3_HIP(h3::launch_axpby<T>(y4, T(1), u, fdt, k, n, s));
+  int rc = 0;
+  if (stages) {  // the caller kept the stage inputs of this step
+    y2 = stages;
+    y3 = stages + n;
+    y4 = stages + 2 * n;
+  } else {
+    // stage inputs (integrators.py:34-37); the pruned transforms when u has no
+    // content outside the 2/3 box (then neither have the stage inputs)
+    if ((rc = rhs_impl<T>(gd, u, k, nu, f, pruned, b.rhs_ws, s))) return rc;
+    H3_HIP(h3::launch_axpby<T>(P(b.y2), T(1), u, hdt, k, n, s));
+    if ((rc = rhs_impl<T>(gd, y2, k, nu, f, pruned, b.rhs_ws, s))) return rc;
+    H3_HIP(h3::launch_axpby<T>(P(b.y3), T(1), u, hdt, k, n, s));
+    if ((rc = rhs_impl<T>(gd, y3, k, nu, f, pruned, b.rhs_ws, s))) return rc;
+    H3_HIP(h3::launch_axpby<T>(P(b.y4), T(1), u, fdt, k, n, s));
+  }
   // reverse sweep: kbar_i from the combination weights, then stage adjoints 4..1
   H3_HIP(h3::launch_axpby<T>(kb, sdt, ub, T(0), ub, n, s));
   if ((rc = rhs_vjp_impl<T>(gd, y4, kb, yb, nu, f, b.vjp_ws, s, pgrad))) return rc;
@@ -944,6 +959,30 @@ int hit3d_rk4_steps(const hit3d_grid* grid, void* u_hat, double nu, double dt, i
                           pruned, diag, work, s);
 }
 
+int hit3d_rk4_steps_record(const hit3d_grid* grid, void* u_hat, double nu, double dt, int64_t nsteps,
+                           double t0, const hit3d_forcing* forcing, int32_t dealiased, void* record,
+                           void* work, size_t work_bytes, void* stream) {
+  (void)t0;
+  H3_TRY(check_grid(grid));
+  H3_TRY(check_forcing(forcing));
+  H3_TRY(check_ptr(u_hat, "u_hat"));
+  H3_TRY(check_ptr(record, "record"));
+  if (record == u_hat) return fail(HIT3D_ERR_INVALID, "record must not alias u_hat");
+  if (nsteps < 0) return fail(HIT3D_ERR_INVALID, "nsteps must be >= 0");
+  if (!std::isfinite(dt) || !std::isfinite(nu))
+    return fail(HIT3D_ERR_INVALID, "dt and nu must be finite");
+  const bool pruned = dealiased != 0;
+  size_t need;
+  carve_step(*grid, make_geo(*grid, pruned), pruned, true, nullptr, &need);
+  H3_TRY(check_work(work, work_bytes, need));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (grid->precision == 4)
+    return rk4_impl<float>(*grid, static_cast<cpx<float>*>(u_hat), nu, dt, nsteps, forcing, pruned,
+                           nullptr, work, s, nullptr, false, static_cast<cpx<float>*>(record));
+  return rk4_impl<double>(*grid, static_cast<cpx<double>*>(u_hat), nu, dt, nsteps, forcing, pruned,
+                          nullptr, work, s, nullptr, false, static_cast<cpx<double>*>(record));
+}
+
 int hit3d_rk4_if_steps(const hit3d_grid* grid, void* u_hat, double nu, double dt, int64_t nsteps,
                        double t0, const hit3d_forcing* forcing, int32_t dealiased, double* diag,
                        void* work, size_t work_bytes, void* stream) {
@@ -1154,15 +1193,22 @@ int hit3d_rhs_vjp(const hit3d_grid* grid, const void* u_hat, const void* v, void
 
 int hit3d_rk4_step_vjp(const hit3d_grid* grid, const void* u_hat, const void* ubar_next,
                        void* ubar, double nu, double dt, double t, const hit3d_forcing* forcing,
-                       int32_t dealiased, double* param_grad, void* work, size_t work_bytes,
-                       void* stream) {
+                       int32_t dealiased, const void* stages, double* param_grad, void* work,
+                       size_t work_bytes, void* stream) {
   (void)t;
   H3_TRY(check_grid(grid));
   H3_TRY(check_forcing(forcing));
   H3_TRY(check_ptr(u_hat, "u_hat"));
   H3_TRY(check_ptr(ubar_next, "ubar_next"));
   H3_TRY(check_ptr(ubar, "ubar"));
+  if (stages) H3_TRY(check_ptr(stages, "stages"));
   if (ubar == u_hat) return fail(HIT3D_ERR_INVALID, "ubar must not alias u_hat");
+  if (stages) {
+    const char* lo = static_cast<const char*>(stages);
+    const char* o = static_cast<const char*>(ubar);
+    if (o + state_bytes(*grid) > lo && o < lo + 3 * state_bytes(*grid))
+      return fail(HIT3D_ERR_INVALID, "ubar must not overlap stages");
+  }
   if (!std::isfinite(dt) || !std::isfinite(nu))
     return fail(HIT3D_ERR_INVALID, "dt and nu must be finite");
   size_t need;
@@ -1173,11 +1219,11 @@ int hit3d_rk4_step_vjp(const hit3d_grid* grid, const void* u_hat, const void* ub
     return rk4_step_vjp_impl<float>(*grid, static_cast<const cpx<float>*>(u_hat),
                                     static_cast<const cpx<float>*>(ubar_next),
                                     static_cast<cpx<float>*>(ubar), nu, dt, forcing, work, s,
-                                    param_grad, dealiased != 0);
+                                    param_grad, dealiased != 0, static_cast<const cpx<float>*>(stages));
   return rk4_step_vjp_impl<double>(*grid, static_cast<const cpx<double>*>(u_hat),
                                    static_cast<const cpx<double>*>(ubar_next),
                                    static_cast<cpx<double>*>(ubar), nu, dt, forcing, work, s,
-                                   param_grad, dealiased != 0);
+                                   param_grad, dealiased != 0, static_cast<const cpx<double>*>(stages));
 }
 
 }  // extern "C"

@@ -29,6 +29,7 @@ import torch
 from . import _lib
 from .forcing import BandForcing, LinearForcing
 from .grid import DEALIASED_MARK, check_spectral, stream
+from .navierstokes import known_dealiased
 from .integrators import rollout as _rollout
 
 __all__ = ["make_ns_rhs_vjp", "rk4_step_vjp", "rollout_vjp", "differentiable_rollout",
@@ -67,15 +68,17 @@ def _pg_ptr(pg, like):
     return pg.data_ptr()
 
 
-def _step_vjp_into(grid, u, ub, params, dt, t, out, param_grad=None):
-    """The step's stage inputs are recomputed with the pruned transforms when
-    u carries the dealiased mark (rollout's checkpoints do): no device read."""
+def _step_vjp_into(grid, u, ub, params, dt, t, out, param_grad=None, stages=None):
+    """Without `stages` the step's stage inputs are recomputed, with the
+    pruned transforms when u carries the dealiased mark (rollout's
+    checkpoints do): no device read."""
     ws = grid.workspace(_lib.OP_RK4_VJP)
     f = params.forcing.desc()
     inside = 1 if getattr(u, DEALIASED_MARK, None) == u._version else 0
     _lib.call("hit3d_rk4_step_vjp", grid.desc, u.data_ptr(), ub.data_ptr(), out.data_ptr(),
               float(params.nu), float(dt), float(t), ctypes.byref(f), inside,
-              _pg_ptr(param_grad, u), ws.data_ptr(), ws.numel(), stream())
+              None if stages is None else stages.data_ptr(), _pg_ptr(param_grad, u), ws.data_ptr(),
+              ws.numel(), stream())
 
 
 def rk4_step_vjp(rhs, u_hat, ubar, params, dt, t, param_grad=None):
@@ -89,22 +92,62 @@ def rk4_step_vjp(rhs, u_hat, ubar, params, dt, t, param_grad=None):
     return out
 
 
-def rollout_vjp(rhs, u_hat, ubar, params, dt, nsteps, t0=0.0, checkpoint=20, param_grad=None):
+# share of the free device memory a segment's record may take
+_RECORD_SHARE = 0.5
+
+
+def _record_segment(rhs, u0, params, dt, nsteps, t0):
+    """[nsteps][4][3][Nx][Ny][Nz/2+1]: per step its start state and the
+    inputs of stages 2..4 (hit3d_rk4_steps_record), or None when that does
+    not fit in device memory."""
+    g = rhs.grid
+    per_step = 4 * u0.numel() * u0.element_size()
+    free, _ = torch.cuda.mem_get_info(u0.device)
+    if nsteps * per_step > _RECORD_SHARE * free:
+        return None
+    rec = torch.empty((nsteps, 4) + tuple(u0.shape), dtype=u0.dtype, device=u0.device)
+    inside = known_dealiased(u0, g)
+    u = u0.clone()
+    pruned = 1 if inside else 0
+    ws = g.workspace(_lib.OP_RK4, 3, pruned)
+    f = params.forcing.desc()
+    _lib.call("hit3d_rk4_steps_record", g.desc, u.data_ptr(), float(params.nu), float(dt), int(nsteps),
+              float(t0), ctypes.byref(f), pruned, rec.data_ptr(), ws.data_ptr(), ws.numel(), stream())
+    return rec
+
+
+def rollout_vjp(rhs, u_hat, ubar, params, dt, nsteps, t0=0.0, checkpoint=20, param_grad=None,
+                checkpoints=None):
     """Cotangent of rollout's input given the cotangent of its output, with
-    checkpointed recomputation (memory: nsteps/checkpoint + checkpoint
-    states); the parameter derivatives of every step are added to
-    `param_grad` when given."""
+    checkpointed recomputation: every segment of `checkpoint` steps is run
+    again once, keeping each step's start state and stage inputs (when that
+    record fits in half the free device memory; otherwise the states only,
+    and each step's adjoint recomputes its stage inputs). Memory:
+    nsteps/checkpoint states plus one segment's record. `checkpoints` maps
+    segment starts to states the forward pass already kept (as
+    differentiable_rollout does); missing ones are recomputed. The
+    parameter derivatives of every step are added to `param_grad` when
+    given."""
     g = rhs.grid
     if checkpoint < 1:
         raise ValueError("checkpoint must be >= 1")
     u0 = check_spectral(u_hat, g, 3)
     ub = check_spectral(ubar, g, 3, "ubar").clone()
     starts = list(range(0, int(nsteps), int(checkpoint)))
-    ckpt = {0: u0}
+    ckpt = dict(checkpoints or {})
+    ckpt[0] = u0
     for a, b in zip(starts[:-1], starts[1:]):
-        ckpt[b] = _rollout(rhs, ckpt[a], params, dt, b - a, t0 + a * dt)
+        if b not in ckpt:
+            ckpt[b] = _rollout(rhs, ckpt[a], params, dt, b - a, t0 + a * dt)
     for s in reversed(starts):
         e = min(s + checkpoint, int(nsteps))
+        rec = _record_segment(rhs, ckpt[s], params, dt, e - s, t0 + s * dt)
+        if rec is not None:
+            for n in range(e - 1, s - 1, -1):
+                _step_vjp_into(g, rec[n - s, 0], ub, params, dt, t0 + n * dt, ub, param_grad,
+                               stages=rec[n - s, 1])
+            del rec
+            continue
         states = [ckpt[s]]
         for n in range(s, e - 1):
             states.append(_rollout(rhs, states[-1], params, dt, 1, t0 + n * dt))
@@ -124,9 +167,18 @@ class _Rollout(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, u_hat, nu, coef, rhs, params, dt, nsteps, t0, checkpoint):
+        # the forward runs segment by segment and keeps each segment's start
+        # state: the backward's checkpoints, not computed twice
+        u = u_hat.detach()
+        keep = {}
+        for s in range(0, nsteps, checkpoint):
+            if s > 0:
+                keep[s] = u
+            u = _rollout(rhs, u, params, dt, min(checkpoint, nsteps - s), t0 + s * dt)
         ctx.save_for_backward(u_hat)
         ctx.cfg = (rhs, params, dt, nsteps, t0, checkpoint)
-        return _rollout(rhs, u_hat.detach(), params, dt, nsteps, t0)
+        ctx.keep = keep
+        return u if nsteps > 0 else u.clone()
 
     @staticmethod
     def backward(ctx, grad_out):
@@ -134,7 +186,9 @@ class _Rollout(torch.autograd.Function):
         rhs, params, dt, nsteps, t0, checkpoint = ctx.cfg
         need_nu, need_coef = ctx.needs_input_grad[1], ctx.needs_input_grad[2]
         pg = param_grad_buffer(u0.device) if (need_nu or need_coef) else None
-        g = rollout_vjp(rhs, u0, grad_out.contiguous(), params, dt, nsteps, t0, checkpoint, pg)
+        g = rollout_vjp(rhs, u0, grad_out.contiguous(), params, dt, nsteps, t0, checkpoint, pg,
+                        checkpoints=ctx.keep)
+        ctx.keep = None
         g_nu = g_coef = None
         if pg is not None:
             host = pg.cpu()

@@ -195,16 +195,30 @@ int hit3d_rhs_vjp(const hit3d_grid* grid, const void* u_hat, const void* v, void
 
 /* Adjoint of one classic RK4 step (integrators.py:32-38): from the step's
    start state u_hat and the cotangent ubar_next of its result, writes the
-   cotangent ubar of u_hat. Recomputes the stage inputs, then runs the four
-   stage adjoints in reverse. ubar may alias ubar_next. `dealiased` = 1
-   promises that u_hat has no content outside the 2/3 box (as for hit3d_rhs):
-   the stage inputs are then recomputed with the pruned transforms. `param_grad`
-   (device, 3 doubles, or NULL) accumulates the step's parameter derivatives
-   as in hit3d_rhs_vjp, summed over the four stages. */
+   cotangent ubar of u_hat. Takes the stage inputs from `stages` (device, the
+   three states y2, y3, y4 one after the other, as hit3d_rk4_steps_record
+   keeps them) or, when `stages` is NULL, recomputes them; then runs the four
+   stage adjoints in reverse. ubar may alias ubar_next, not u_hat or stages.
+   `dealiased` = 1 promises that u_hat has no content outside the 2/3 box (as
+   for hit3d_rhs): the stage inputs are then recomputed with the pruned
+   transforms. `param_grad` (device, 3 doubles, or NULL) accumulates the
+   step's parameter derivatives as in hit3d_rhs_vjp, summed over the four
+   stages. */
 int hit3d_rk4_step_vjp(const hit3d_grid* grid, const void* u_hat, const void* ubar_next,
                        void* ubar, double nu, double dt, double t, const hit3d_forcing* forcing,
-                       int32_t dealiased, double* param_grad, void* work, size_t work_bytes,
-                       void* stream);
+                       int32_t dealiased, const void* stages, double* param_grad, void* work,
+                       size_t work_bytes, void* stream);
+
+/* hit3d_rk4_steps that also keeps what the step adjoint linearizes about
+   (the forward half of checkpointed reverse mode; the reference gets it from
+   jax's scan over rk4_step, integrators.py:41-53): for step n of the call,
+   record[4n] = the state at the step's start and record[4n + 1 .. 4n + 3] =
+   the inputs y2, y3, y4 of stages 2..4 (integrators.py:35-37), each a full
+   state in the API layout (record holds 4 * nsteps states). Same arguments,
+   workspace and time stepping as hit3d_rk4_steps, without diag. */
+int hit3d_rk4_steps_record(const hit3d_grid* grid, void* u_hat, double nu, double dt, int64_t nsteps,
+                           double t0, const hit3d_forcing* forcing, int32_t dealiased, void* record,
+                           void* work, size_t work_bytes, void* stream);
 
 /* Measurement kernel (no reference counterpart): dst = src over `bytes`
    (a multiple of 16; both pointers 16-byte aligned) by a plain streaming

@@ -64,6 +64,61 @@ def test_rk4_step_and_rollout_vjp_match_oracle(fi):
         assert rel(h3.rollout_vjp(grhs, dev(u), dev(ub), gp, 2e-3, 7, checkpoint=ck), ref) < 1e-11
 
 
+@pytest.mark.parametrize("dealiased", [True, False])
+def test_rk4_steps_record_keeps_states_and_stage_inputs(dealiased):
+    """hit3d_rk4_steps_record: slot 0 of step n is the state after n steps
+    (bit for bit the rollout's), slots 1..3 the stage inputs of the oracle's
+    RK4 step (integrators.py:35-37) from that state; dense inputs included."""
+    import ctypes
+    from hit3d import _lib
+    from hit3d.grid import stream
+    og, g = O.make_grid(16), h3.make_grid(16, dtype=torch.float64)
+    u = O.random_field(8, og, O.config_spectrum)
+    if not dealiased:
+        u = u + 1e-3 * np.abs(u).max() * _rand_c(9, u.shape)
+    of, gf = FORCINGS[4]
+    op, gp = O.NSParams(nu=0.02, forcing=of), h3.NSParams(nu=0.02, forcing=gf)
+    dt, n = 2e-3, 3
+    rec = torch.empty((n, 4) + u.shape, dtype=torch.complex128, device="cuda")
+    x = dev(u)
+    pruned = 1 if dealiased else 0
+    ws = g.workspace(_lib.OP_RK4, 3, pruned)
+    f = gf.desc()
+    _lib.call("hit3d_rk4_steps_record", g.desc, x.data_ptr(), 0.02, dt, n, 0.0, ctypes.byref(f), pruned,
+              rec.data_ptr(), ws.data_ptr(), ws.numel(), stream())
+    assert torch.equal(x, h3.rollout(h3.make_ns_rhs(g), dev(u), gp, dt, n))
+    orhs, un = O.make_ns_rhs(og), u
+    for step in range(n):
+        got = rec[step].cpu().numpy()
+        assert torch.equal(rec[step, 0], h3.rollout(h3.make_ns_rhs(g), dev(u), gp, dt, step))
+        k1 = orhs(un, op, 0.0)
+        y2 = un + 0.5 * dt * k1
+        k2 = orhs(y2, op, 0.0)
+        y3 = un + 0.5 * dt * k2
+        y4 = un + dt * orhs(y3, op, 0.0)
+        for slot, ref in ((1, y2), (2, y3), (3, y4)):
+            assert rel(got[slot], ref) < 1e-13
+        un = O.rollout(orhs, un, op, dt, 1)
+
+
+def test_rollout_vjp_record_and_recompute_agree(monkeypatch):
+    """The recorded segments (stage inputs kept by the forward recompute) and
+    the recompute-per-step path (taken when a record does not fit) give the
+    same cotangent and parameter gradients."""
+    import hit3d.adjoint as A
+    g = h3.make_grid(32, dtype=torch.float64)
+    u = h3.random_field(10, g, h3.config_spectrum)
+    ub = dev(_rand_c(11, tuple(u.shape)))
+    params = h3.NSParams(nu=0.02, forcing=h3.BandForcing(eps=0.3, kmin=1.0, kmax=3.0))
+    rhs = h3.make_ns_rhs(g)
+    pg1, pg2 = A.param_grad_buffer(u.device), A.param_grad_buffer(u.device)
+    a = h3.rollout_vjp(rhs, u, ub, params, 2e-3, 9, checkpoint=4, param_grad=pg1)
+    monkeypatch.setattr(A, "_RECORD_SHARE", 0.0)
+    b = h3.rollout_vjp(rhs, u, ub, params, 2e-3, 9, checkpoint=4, param_grad=pg2)
+    assert rel(a, b.cpu().numpy()) < 1e-13
+    assert float((pg1 - pg2).abs().max() / pg2.abs().max()) < 1e-12
+
+
 def _shell_energy_weights(g, lo, hi):
     """w / (2 Ntot^2) on modes whose shell s has lo <= s < hi (the energy in
     those shells is sum(W |u|^2))."""
