@@ -470,9 +470,9 @@ int rk4_impl(const hit3d_grid& gd, cpx<T>* u_hat, double nu, double dt, int64_t 
   return HIT3D_OK;
 }
 
+// x- then y-inverse of nf dense fields (in -> work, then in place)
 template <class T>
-int to_physical_impl(const hit3d_grid& gd, const cpx<T>* in, T* out, int nf, cpx<T>* work,
-                     hipStream_t s) {
+int xy_inverse_impl(const hit3d_grid& gd, const cpx<T>* in, int nf, cpx<T>* work, hipStream_t s) {
   const Geo g = make_geo(gd, false);
   const int64_t fs = int64_t(g.nx) * g.ny * g.nzh;
   h3::StridedArgs<T> x{};
@@ -497,19 +497,14 @@ int to_physical_impl(const hit3d_grid& gd, const cpx<T>* in, T* out, int nf, cpx
   y.in_k = y.out_k = g.ny;
   y.in_p = y.out_p = (g.ny + 1) / 2;
   H3_HIP(h3::launch_strided<T>(g.ny, +1, y, s));
-  const double n = double(gd.nx) * double(gd.ny) * double(gd.nz);
-  H3_HIP(h3::launch_z_c2r<T>(g.nz, work, fs, out, int64_t(g.nx) * g.ny * g.nz, nf,
-                             int64_t(g.nx) * g.ny, g.nzh, T(1.0 / n), s));
   return HIT3D_OK;
 }
 
+// y- then x-forward of nf dense fields (work in place, then work -> out)
 template <class T>
-int to_spectral_impl(const hit3d_grid& gd, const T* in, cpx<T>* out, int nf, cpx<T>* work,
-                     hipStream_t s) {
+int yx_forward_impl(const hit3d_grid& gd, cpx<T>* work, cpx<T>* out, int nf, hipStream_t s) {
   const Geo g = make_geo(gd, false);
   const int64_t fs = int64_t(g.nx) * g.ny * g.nzh;
-  H3_HIP(h3::launch_z_r2c<T>(g.nz, in, int64_t(g.nx) * g.ny * g.nz, work, fs, nf,
-                             int64_t(g.nx) * g.ny, g.nzh, s));
   h3::StridedArgs<T> y{};
   y.in = work;
   y.out = work;
@@ -535,14 +530,35 @@ int to_spectral_impl(const hit3d_grid& gd, const T* in, cpx<T>* out, int nf, cpx
   return HIT3D_OK;
 }
 
+template <class T>
+int to_physical_impl(const hit3d_grid& gd, const cpx<T>* in, T* out, int nf, cpx<T>* work,
+                     hipStream_t s) {
+  const Geo g = make_geo(gd, false);
+  const int64_t fs = int64_t(g.nx) * g.ny * g.nzh;
+  int rc = xy_inverse_impl<T>(gd, in, nf, work, s);
+  if (rc) return rc;
+  const double n = double(gd.nx) * double(gd.ny) * double(gd.nz);
+  H3_HIP(h3::launch_z_c2r<T>(g.nz, work, fs, out, int64_t(g.nx) * g.ny * g.nz, nf,
+                             int64_t(g.nx) * g.ny, g.nzh, T(1.0 / n), s));
+  return HIT3D_OK;
+}
+
+template <class T>
+int to_spectral_impl(const hit3d_grid& gd, const T* in, cpx<T>* out, int nf, cpx<T>* work,
+                     hipStream_t s) {
+  const Geo g = make_geo(gd, false);
+  const int64_t fs = int64_t(g.nx) * g.ny * g.nzh;
+  H3_HIP(h3::launch_z_r2c<T>(g.nz, in, int64_t(g.nx) * g.ny * g.nz, work, fs, nf,
+                             int64_t(g.nx) * g.ny, g.nzh, s));
+  return yx_forward_impl<T>(gd, work, out, nf, s);
+}
+
 // ---- adjoint (VJP) of the RHS and of one RK4 step --------------------------
 
 struct VjpBuffers {
   void* vp = nullptr;    // [3] P M v
   void* S = nullptr;     // [9] transform inputs, later [6] transformed cross products
-  void* tw = nullptr;    // [9] transform workspace
-  void* ph = nullptr;    // [9] real physical u, w, g
-  void* cr = nullptr;    // [6] real w x g, g x u
+  void* tw = nullptr;    // [9] transform workspace (the z pass works in place)
   double* partials = nullptr;
   double* scalars = nullptr;
   double* dot = nullptr;
@@ -553,13 +569,10 @@ VjpBuffers carve_vjp(const hit3d_grid& gd, void* work, size_t* total) {
   c.base = static_cast<char*>(work);
   const size_t cb = cbytes(gd);
   const size_t fs = size_t(gd.nx) * gd.ny * (gd.nz / 2 + 1);
-  const size_t pr = size_t(gd.nx) * gd.ny * gd.nz * size_t(gd.precision);
   VjpBuffers b;
   b.vp = c.take(3 * fs * cb);
   b.S = c.take(9 * fs * cb);
   b.tw = c.take(9 * fs * cb);
-  b.ph = c.take(9 * pr);
-  b.cr = c.take(6 * pr);
   b.partials = static_cast<double*>(c.take(size_t(3) * h3::kReduceBlocks * sizeof(double)));
   b.scalars = static_cast<double*>(c.take(16 * sizeof(double)));
   b.dot = static_cast<double*>(c.take(16 * sizeof(double)));
@@ -574,20 +587,18 @@ int rhs_vjp_impl(const hit3d_grid& gd, const cpx<T>* u, const cpx<T>* v, cpx<T>*
   VjpBuffers b = carve_vjp(gd, work, &total);
   const Geo g = make_geo(gd, false);
   const int64_t fs = int64_t(g.nx) * g.ny * g.nzh;
-  const int64_t np = int64_t(g.nx) * g.ny * g.nz;
   const double ntot = double(gd.nx) * double(gd.ny) * double(gd.nz);
   cpx<T>* vp = static_cast<cpx<T>*>(b.vp);
   cpx<T>* S = static_cast<cpx<T>*>(b.S);
   cpx<T>* tw = static_cast<cpx<T>*>(b.tw);
-  T* ph = static_cast<T*>(b.ph);
-  T* cr = static_cast<T*>(b.cr);
   H3_HIP(h3::launch_dealias_project<T>(v, vp, fs, 3, g, 1, 1, s));
   H3_HIP(h3::launch_adj_prep<T>(u, vp, S, fs, g, ntot, s));
-  int rc = to_physical_impl<T>(gd, S, ph, 9, tw, s);
+  // u, w = K u, g to physical space (x, y, then z fused with w x g, g x u
+  // and the z-forward), then back along y and x into S[0..5]
+  int rc = xy_inverse_impl<T>(gd, S, 9, tw, s);
   if (rc) return rc;
-  H3_HIP(h3::launch_adj_cross<T>(ph, np, cr, np, np, s));
-  rc = to_spectral_impl<T>(gd, cr, S, 6, tw, s);
-  if (rc) return rc;
+  H3_HIP(h3::launch_z_adj<T>(g.nz, tw, fs, int64_t(g.nx) * g.ny, g.nzh, T(1.0 / ntot), s));
+  if ((rc = yx_forward_impl<T>(gd, tw, S, 6, s))) return rc;
   T kmin2 = T(1), kmax2 = T(0);
   h3::AdjCombine c{};
   c.fkind = f ? f->kind : HIT3D_FORCING_NONE;

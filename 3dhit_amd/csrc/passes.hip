@@ -414,6 +414,69 @@ __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_r2c(const T* __re
 }
 
 // ---------------------------------------------------------------------------
+// Adjoint z pass (dense layout, in place): per z pencil, c2r of the nine
+// fields u, w = K u, g in the pairs (u0 + i u1), (u2 + i w0), (w1 + i w2),
+// (g0 + i g1), g2, scaled by `scale` (1/Ntot), the cross products a = w x g
+// and b = g x u, and r2c of (a0 + i a1), (a2 + i b0), (b1 + i b2) into
+// fields 0..5. The same pairs, scaling and products as z c2r -> cross -> z
+// r2c over physical buffers, without the 15 real fields' round trip.
+// ---------------------------------------------------------------------------
+template <int N, class T>
+__global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_adj(cpx<T>* buf, int64_t fs, int64_t npencils,
+                                                   int nzh, T scale) {
+  using B = PencilBlock<N>;
+  using F = typename B::F;
+  static_assert(B::THREADS == 64, "WaveSync exchanges need one-wave blocks");
+  constexpr int R = B::R, Q = B::Q, PW = B::PW;
+  __shared__ cpx<T> lds[B::LDS];
+  const int q = threadIdx.x % Q, p = threadIdx.x / Q;
+  const int64_t pp = int64_t(blockIdx.x) * PW + p;
+  const bool active = pp < npencils;
+  const int64_t base = (active ? pp : 0) * nzh;
+  cpx<T>* lp = lds + p * F::kSlots;
+  SlotPencil<T> L{lp};
+  const int kzl = active ? nzh : 0;  // inactive pencils load nothing
+  cpx<T> tw[R];
+  F::load_tw(tw, q);
+  cpx<T> z[5][R];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      z[c][r] = herm_pair<N, T>(buf + 2 * c * fs + base, c < 4 ? buf + (2 * c + 1) * fs + base : nullptr,
+                                q + Q * r, kzl);
+    F::template run<+1, WaveSync>(z[c], q, L, tw);
+  }
+  cpx<T> X[3][R];
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+    const T u0 = z[0][s].x * scale, u1 = z[0][s].y * scale, u2 = z[1][s].x * scale;
+    const T w0 = z[1][s].y * scale, w1 = z[2][s].x * scale, w2 = z[2][s].y * scale;
+    const T g0 = z[3][s].x * scale, g1 = z[3][s].y * scale, g2 = z[4][s].x * scale;
+    const int r = F::out_slot_as_input(s);
+    X[0][r] = {w1 * g2 - w2 * g1, w2 * g0 - w0 * g2};
+    X[1][r] = {w0 * g1 - w1 * g0, g1 * u2 - g2 * u1};
+    X[2][r] = {g2 * u0 - g0 * u2, g0 * u1 - g1 * u0};
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    F::template run<-1, WaveSync>(X[c], q, L, tw);
+    WaveSync::sync();
+#pragma unroll
+    for (int s = 0; s < R; ++s) lp[F::out_index(q, s)] = X[c][s];
+    WaveSync::sync();
+    if (active) {
+      for (int k = q; k < nzh; k += Q) {
+        cpx<T> a, b;
+        split_pair(lp[k], lp[(N - k) & (N - 1)], a, b);
+        buf[2 * c * fs + base + k] = a;
+        buf[(2 * c + 1) * fs + base + k] = b;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Deterministic block reduction of three doubles (blockDim a multiple of 64, <= 1024):
 // xor-butterfly inside each wave, then the four wave sums in fixed order.
 // ---------------------------------------------------------------------------
@@ -676,6 +739,17 @@ hipError_t launch_z_r2c(int N, const T* in, int64_t in_fs, cpx<T>* out, int64_t 
 }
 
 template <class T>
+hipError_t launch_z_adj(int N, cpx<T>* buf, int64_t fs, int64_t npencils, int nzh, T scale,
+                        hipStream_t s) {
+  H3_DISPATCH_N(N, {
+    const unsigned nb = ceil_div(npencils, PencilBlock<NN>::PW);
+    hipLaunchKernelGGL((k_z_adj<NN, T>), dim3(nb), dim3(PencilBlock<NN>::THREADS), 0, s, buf, fs, npencils,
+                       nzh, scale);
+  });
+  return hipGetLastError();
+}
+
+template <class T>
 hipError_t launch_x_forward_stage(int N, int mode, const StageArgs<T>& a, hipStream_t s,
                                   int* nblocks_out) {
   const int64_t J = int64_t(a.g.ky) * a.g.kzs;
@@ -709,6 +783,7 @@ hipError_t launch_x_forward_stage(int N, int mode, const StageArgs<T>& a, hipStr
                                       int, T, hipStream_t);                                      \
   template hipError_t launch_z_r2c<T>(int, const T*, int64_t, cpx<T>*, int64_t, int, int64_t,   \
                                       int, hipStream_t);                                         \
+  template hipError_t launch_z_adj<T>(int, cpx<T>*, int64_t, int64_t, int, T, hipStream_t);           \
   template hipError_t launch_x_forward_stage<T>(int, int, const StageArgs<T>&, hipStream_t, int*);
 
 H3_INSTANTIATE(H3_REAL)
