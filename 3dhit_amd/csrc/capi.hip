@@ -556,7 +556,6 @@ int to_spectral_impl(const hit3d_grid& gd, const T* in, cpx<T>* out, int nf, cpx
 // ---- adjoint (VJP) of the RHS and of one RK4 step --------------------------
 
 struct VjpBuffers {
-  void* vp = nullptr;    // [3] P M v
   void* S = nullptr;     // [9] transform inputs, later [6] transformed cross products
   void* tw = nullptr;    // [9] transform workspace (the z pass works in place)
   double* partials = nullptr;
@@ -570,7 +569,6 @@ VjpBuffers carve_vjp(const hit3d_grid& gd, void* work, size_t* total) {
   const size_t cb = cbytes(gd);
   const size_t fs = size_t(gd.nx) * gd.ny * (gd.nz / 2 + 1);
   VjpBuffers b;
-  b.vp = c.take(3 * fs * cb);
   b.S = c.take(9 * fs * cb);
   b.tw = c.take(9 * fs * cb);
   b.partials = static_cast<double*>(c.take(size_t(3) * h3::kReduceBlocks * sizeof(double)));
@@ -580,19 +578,20 @@ VjpBuffers carve_vjp(const hit3d_grid& gd, void* work, size_t* total) {
   return b;
 }
 
+// The RHS VJP at u for the cotangent v into `out`, or through the epilogue
+// `ep` (accumulated, and the next RK4 stage's cotangent formed) when given.
 template <class T>
 int rhs_vjp_impl(const hit3d_grid& gd, const cpx<T>* u, const cpx<T>* v, cpx<T>* out, double nu,
-                 const hit3d_forcing* f, void* work, hipStream_t s, double* pgrad = nullptr) {
+                 const hit3d_forcing* f, void* work, hipStream_t s, double* pgrad = nullptr,
+                 const h3::AdjEpilogue<T>* ep = nullptr) {
   size_t total;
   VjpBuffers b = carve_vjp(gd, work, &total);
   const Geo g = make_geo(gd, false);
   const int64_t fs = int64_t(g.nx) * g.ny * g.nzh;
   const double ntot = double(gd.nx) * double(gd.ny) * double(gd.nz);
-  cpx<T>* vp = static_cast<cpx<T>*>(b.vp);
   cpx<T>* S = static_cast<cpx<T>*>(b.S);
   cpx<T>* tw = static_cast<cpx<T>*>(b.tw);
-  H3_HIP(h3::launch_dealias_project<T>(v, vp, fs, 3, g, 1, 1, s));
-  H3_HIP(h3::launch_adj_prep<T>(u, vp, S, fs, g, ntot, s));
+  H3_HIP(h3::launch_adj_prep<T>(u, v, S, fs, g, ntot, s));
   // u, w = K u, g to physical space (x, y, then z fused with w x g, g x u
   // and the z-forward), then back along y and x into S[0..5]
   int rc = xy_inverse_impl<T>(gd, S, 9, tw, s);
@@ -619,7 +618,7 @@ int rhs_vjp_impl(const hit3d_grid& gd, const cpx<T>* u, const cpx<T>* v, cpx<T>*
     c.tau = f->tau;
     rc = reduce_state<T>(gd, g, u, f, b.partials, b.scalars, s);  // coef, S_E, S_Om, S_band
     if (rc) return rc;
-    H3_HIP(h3::launch_dot_partials<T>(u, vp, fs, g, kmin2, kmax2, band ? 1 : 0, b.partials, s));
+    H3_HIP(h3::launch_dot_partials<T>(u, v, fs, g, kmin2, kmax2, band ? 1 : 0, b.partials, s));
     FinalizeArgs fa{};
     fa.kind = HIT3D_FORCING_NONE;
     fa.inv_ntot2 = 1.0;
@@ -627,10 +626,12 @@ int rhs_vjp_impl(const hit3d_grid& gd, const cpx<T>* u, const cpx<T>* v, cpx<T>*
                        b.dot);  // dot[1] = the raw sum
     H3_HIP(hipGetLastError());
   }
-  H3_HIP(h3::launch_adj_combine<T>(S, v, vp, u, out, fs, g, kmin2, kmax2, c, s));
+  // parameter derivatives first: the epilogue may overwrite v
   if (pgrad)  // b.scalars[3] = 2E_band of u (reduce_state above, band forcing)
     H3_HIP(h3::launch_param_grad<T>(v, u, fs, g, kmin2, kmax2, c.fkind, b.scalars + 3, c.tiny,
                                     b.partials, pgrad, s));
+  H3_HIP(h3::launch_adj_combine<T>(S, v, u, out, fs, g, kmin2, kmax2, c,
+                                   ep ? *ep : h3::AdjEpilogue<T>{}, s));
   return HIT3D_OK;
 }
 
@@ -639,7 +640,7 @@ size_t state_bytes(const hit3d_grid& gd) {
 }
 
 struct StepVjpBuffers {
-  void *ub, *k, *y2, *y3, *y4, *kb, *yb, *rhs_ws, *vjp_ws;
+  void *ub, *k, *y2, *y3, *y4, *kb, *rhs_ws, *vjp_ws;
   size_t rhs_bytes, vjp_bytes;
 };
 
@@ -654,7 +655,6 @@ StepVjpBuffers carve_step_vjp(const hit3d_grid& gd, void* work, size_t* total) {
   b.y3 = c.take(st);
   b.y4 = c.take(st);
   b.kb = c.take(st);
-  b.yb = c.take(st);
   size_t dense = 0, pruned = 0;  // the stage recomputation runs either layout
   carve_step(gd, make_geo(gd, false), false, false, nullptr, &dense);
   carve_step(gd, make_geo(gd, true), true, false, nullptr, &pruned);
@@ -674,7 +674,7 @@ int rk4_step_vjp_impl(const hit3d_grid& gd, const cpx<T>* u, const cpx<T>* ub_in
   StepVjpBuffers b = carve_step_vjp(gd, work, &total);
   const int64_t n = int64_t(3) * gd.nx * gd.ny * (gd.nz / 2 + 1);
   auto P = [](void* p) { return static_cast<cpx<T>*>(p); };
-  cpx<T>*ub = P(b.ub), *k = P(b.k), *kb = P(b.kb), *yb = P(b.yb);
+  cpx<T>*ub = P(b.ub), *k = P(b.k), *kb = P(b.kb);
   const cpx<T>*y2 = P(b.y2), *y3 = P(b.y3), *y4 = P(b.y4);
   const T hdt = T(0.5 * dt), fdt = T(dt), sdt = T(dt / 6.0), tdt = T(dt / 3.0);
   H3_HIP(hipMemcpyAsync(ub, ub_in, n * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s));
@@ -693,19 +693,26 @@ int rk4_step_vjp_impl(const hit3d_grid& gd, const cpx<T>* u, const cpx<T>* ub_in
     if ((rc = rhs_impl<T>(gd, y3, k, nu, f, pruned, b.rhs_ws, s))) return rc;
     H3_HIP(h3::launch_axpby<T>(P(b.y4), T(1), u, fdt, k, n, s));
   }
-  // reverse sweep: kbar_i from the combination weights, then stage adjoints 4..1
+  // reverse sweep: kbar_i from the combination weights, then stage adjoints
+  // 4..1, each accumulating its value into `out` and forming the next
+  // kbar in its epilogue (kbar_3 = dt/3 ub + dt ybar_4, kbar_2 = dt/3 ub +
+  // dt/2 ybar_3, kbar_1 = dt/6 ub + dt/2 ybar_2)
   H3_HIP(h3::launch_axpby<T>(kb, sdt, ub, T(0), ub, n, s));
-  if ((rc = rhs_vjp_impl<T>(gd, y4, kb, yb, nu, f, b.vjp_ws, s, pgrad))) return rc;
-  H3_HIP(h3::launch_axpby<T>(out, T(1), ub, T(1), yb, n, s));
-  H3_HIP(h3::launch_axpby<T>(kb, tdt, ub, fdt, yb, n, s));
-  if ((rc = rhs_vjp_impl<T>(gd, y3, kb, yb, nu, f, b.vjp_ws, s, pgrad))) return rc;
-  H3_HIP(h3::launch_axpby<T>(out, T(1), out, T(1), yb, n, s));
-  H3_HIP(h3::launch_axpby<T>(kb, tdt, ub, hdt, yb, n, s));
-  if ((rc = rhs_vjp_impl<T>(gd, y2, kb, yb, nu, f, b.vjp_ws, s, pgrad))) return rc;
-  H3_HIP(h3::launch_axpby<T>(out, T(1), out, T(1), yb, n, s));
-  H3_HIP(h3::launch_axpby<T>(kb, sdt, ub, hdt, yb, n, s));
-  if ((rc = rhs_vjp_impl<T>(gd, u, kb, yb, nu, f, b.vjp_ws, s, pgrad))) return rc;
-  H3_HIP(h3::launch_axpby<T>(out, T(1), out, T(1), yb, n, s));
+  h3::AdjEpilogue<T> ep{};
+  ep.acc = out;
+  ep.acc_base = ub;
+  ep.kbn = kb;
+  ep.ub = ub;
+  ep.alpha = tdt;
+  ep.beta = fdt;
+  if ((rc = rhs_vjp_impl<T>(gd, y4, kb, nullptr, nu, f, b.vjp_ws, s, pgrad, &ep))) return rc;
+  ep.acc_base = out;
+  ep.beta = hdt;
+  if ((rc = rhs_vjp_impl<T>(gd, y3, kb, nullptr, nu, f, b.vjp_ws, s, pgrad, &ep))) return rc;
+  ep.alpha = sdt;
+  if ((rc = rhs_vjp_impl<T>(gd, y2, kb, nullptr, nu, f, b.vjp_ws, s, pgrad, &ep))) return rc;
+  ep.kbn = nullptr;
+  if ((rc = rhs_vjp_impl<T>(gd, u, kb, nullptr, nu, f, b.vjp_ws, s, pgrad, &ep))) return rc;
   return HIT3D_OK;
 }
 

@@ -469,9 +469,25 @@ __global__ __launch_bounds__(kBlock) void k_tables(Geo g, T* k2o, T* ik2o, uint8
 // with g = Ntot irfftn(vp / w) the adjoint of rfftn, K = i k x (self-adjoint)
 // and w the Hermitian weight of the mode's kz plane.
 
+// P M v at one mode: the 2/3 mask, then the Leray projection, with the
+// arithmetic of k_dealias_project (so both give the same bits)
+template <class T>
+__device__ __forceinline__ void masked_projection(const cpx<T>* __restrict__ v, int64_t fs, int64_t idx,
+                                                  const ModeIdx& m, const Geo& g, T kx, T ky, T kz,
+                                                  cpx<T> out[3]) {
+  const bool keep = in_box(m.nx, m.ny, m.nz, g);
+  const cpx<T> u0 = keep ? v[idx] : czero<T>(), u1 = keep ? v[fs + idx] : czero<T>(),
+               u2 = keep ? v[2 * fs + idx] : czero<T>();
+  const T ik2 = inv_k2_of(k2_of(kx, ky, kz));
+  const cpx<T> div = ik2 * (kx * u0 + ky * u1 + kz * u2);
+  out[0] = u0 - kx * div;
+  out[1] = u1 - ky * div;
+  out[2] = u2 - kz * div;
+}
+
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_adj_prep(const cpx<T>* __restrict__ u,
-                                                     const cpx<T>* __restrict__ vp,
+                                                     const cpx<T>* __restrict__ v,
                                                      cpx<T>* __restrict__ S, int64_t fs, Geo g,
                                                      double ntot) {
   const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
@@ -488,7 +504,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_prep(const cpx<T>* __restrict__ 
   S[4 * fs + idx] = mul_i(kz * u0 - kx * u2);
   S[5 * fs + idx] = mul_i(kx * u1 - ky * u0);
   const T sc = T(ntot / hermitian_weight(m.nz, g.nz));
-  for (int c = 0; c < 3; ++c) S[(6 + c) * fs + idx] = sc * vp[c * fs + idx];
+  cpx<T> vp[3];
+  masked_projection(v, fs, idx, m, g, kx, ky, kz, vp);
+  for (int c = 0; c < 3; ++c) S[(6 + c) * fs + idx] = sc * vp[c];
 }
 
 template <class T>
@@ -510,22 +528,25 @@ __global__ __launch_bounds__(kBlock) void k_adj_cross(const T* __restrict__ ph, 
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_dot_partials(const cpx<T>* __restrict__ x,
-                                                         const cpx<T>* __restrict__ y, int64_t fs,
+                                                         const cpx<T>* __restrict__ v, int64_t fs,
                                                          Geo g, T kmin2, T kmax2, int use_band,
                                                          double* __restrict__ partials) {
   const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
   double s = 0.0;
   for (int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x; idx < n;
        idx += int64_t(gridDim.x) * kBlock) {
+    const ModeIdx m = mode_of(idx, g);
+    const T kx = wavenumber<T>(g.k0x, m.nx), ky = wavenumber<T>(g.k0y, m.ny),
+            kz = wavenumber<T>(g.k0z, m.nz);
     if (use_band) {
-      const ModeIdx m = mode_of(idx, g);
-      const T k2 = k2_of(wavenumber<T>(g.k0x, m.nx), wavenumber<T>(g.k0y, m.ny),
-                         wavenumber<T>(g.k0z, m.nz));
+      const T k2 = k2_of(kx, ky, kz);
       if (!(kmin2 <= k2 && k2 <= kmax2)) continue;
     }
+    cpx<T> vp[3];
+    masked_projection(v, fs, idx, m, g, kx, ky, kz, vp);
     for (int c = 0; c < 3; ++c) {
-      const cpx<T> a = x[c * fs + idx], b = y[c * fs + idx];
-      s += double(a.x) * b.x + double(a.y) * b.y;
+      const cpx<T> a = x[c * fs + idx];
+      s += double(a.x) * vp[c].x + double(a.y) * vp[c].y;
     }
   }
   block_sum3m(s, 0.0, 0.0, partials + 3 * blockIdx.x);
@@ -635,11 +656,11 @@ __global__ __launch_bounds__(kBlock) void k_hold_apply(const cpx<T>* __restrict_
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_adj_combine(const cpx<T>* __restrict__ AB,
-                                                        const cpx<T>* __restrict__ v,
-                                                        const cpx<T>* __restrict__ vp,
+                                                        const cpx<T>* v,
                                                         const cpx<T>* __restrict__ u,
                                                         cpx<T>* __restrict__ out, int64_t fs,
-                                                        Geo g, T kmin2, T kmax2, AdjCombine c) {
+                                                        Geo g, T kmin2, T kmax2, AdjCombine c,
+                                                        AdjEpilogue<T> ep) {
   const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
   const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (idx >= n) return;
@@ -673,9 +694,19 @@ __global__ __launch_bounds__(kBlock) void k_adj_combine(const cpx<T>* __restrict
       gu = T((*c.dot) * 2.0 * w * c.inv_ntot2 * (c.fnu / sE * double(k2) + dA_dE));
     }
   }
+  cpx<T> vp[3];
+  masked_projection(v, fs, idx, m, g, kx, ky, kz, vp);
   for (int i = 0; i < 3; ++i) {
     const int64_t o = i * fs + idx;
-    out[o] = sc * r[i] - nk2 * v[o] + fc * vp[o] + gu * u[o];
+    const cpx<T> val = sc * r[i] - nk2 * v[o] + fc * vp[i] + gu * u[o];
+    // every read of entry o (v, ub, acc_base) precedes this thread's writes
+    // of entry o, so kbn may alias v and acc may alias acc_base
+    const cpx<T> ubo = ep.kbn ? ep.ub[o] : czero<T>();
+    if (ep.acc)
+      ep.acc[o] = ep.acc_base[o] + val;
+    else
+      out[o] = val;
+    if (ep.kbn) ep.kbn[o] = ep.alpha * ubo + ep.beta * val;
   }
 }
 
@@ -788,10 +819,10 @@ hipError_t launch_tables(const Geo& g, T* k2, T* inv_k2, uint8_t* mask, int32_t*
 }
 
 template <class T>
-hipError_t launch_adj_prep(const cpx<T>* u, const cpx<T>* vp, cpx<T>* S, int64_t fs, const Geo& g,
+hipError_t launch_adj_prep(const cpx<T>* u, const cpx<T>* v, cpx<T>* S, int64_t fs, const Geo& g,
                            double ntot, hipStream_t s) {
   const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
-  hipLaunchKernelGGL((k_adj_prep<T>), dim3(nblocks_for(n)), dim3(kBlock), 0, s, u, vp, S, fs, g,
+  hipLaunchKernelGGL((k_adj_prep<T>), dim3(nblocks_for(n)), dim3(kBlock), 0, s, u, v, S, fs, g,
                      ntot);
   return hipGetLastError();
 }
@@ -840,12 +871,12 @@ hipError_t launch_hold_apply(const cpx<T>* u, const cpx<T>* du, cpx<T>* out, int
 }
 
 template <class T>
-hipError_t launch_adj_combine(const cpx<T>* AB, const cpx<T>* v, const cpx<T>* vp,
-                              const cpx<T>* u, cpx<T>* out, int64_t fs, const Geo& g, T kmin2,
-                              T kmax2, const AdjCombine& c, hipStream_t s) {
+hipError_t launch_adj_combine(const cpx<T>* AB, const cpx<T>* v, const cpx<T>* u, cpx<T>* out,
+                              int64_t fs, const Geo& g, T kmin2, T kmax2, const AdjCombine& c,
+                              const AdjEpilogue<T>& ep, hipStream_t s) {
   const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
-  hipLaunchKernelGGL((k_adj_combine<T>), dim3(nblocks_for(n)), dim3(kBlock), 0, s, AB, v, vp, u,
-                     out, fs, g, kmin2, kmax2, c);
+  hipLaunchKernelGGL((k_adj_combine<T>), dim3(nblocks_for(n)), dim3(kBlock), 0, s, AB, v, u, out, fs, g,
+                     kmin2, kmax2, c, ep);
   return hipGetLastError();
 }
 
@@ -862,9 +893,9 @@ hipError_t launch_axpby(cpx<T>* out, T a, const cpx<T>* x, T b, const cpx<T>* y,
   template hipError_t launch_adj_cross<T>(const T*, int64_t, T*, int64_t, int64_t, hipStream_t);  \
   template hipError_t launch_dot_partials<T>(const cpx<T>*, const cpx<T>*, int64_t, const Geo&, T, \
                                              T, int, double*, hipStream_t);                        \
-  template hipError_t launch_adj_combine<T>(const cpx<T>*, const cpx<T>*, const cpx<T>*,          \
-                                            const cpx<T>*, cpx<T>*, int64_t, const Geo&, T, T,     \
-                                            const AdjCombine&, hipStream_t);                       \
+  template hipError_t launch_adj_combine<T>(const cpx<T>*, const cpx<T>*, const cpx<T>*, cpx<T>*, \
+                                            int64_t, const Geo&, T, T, const AdjCombine&,          \
+                                            const AdjEpilogue<T>&, hipStream_t);                   \
   template hipError_t launch_axpby<T>(cpx<T>*, T, const cpx<T>*, T, const cpx<T>*, int64_t,       \
                                       hipStream_t);                                                       \
   template hipError_t launch_hold_partials<T>(const cpx<T>*, const cpx<T>*, int64_t, const Geo&,  \

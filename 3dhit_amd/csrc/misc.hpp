@@ -54,15 +54,15 @@ hipError_t launch_tables(const Geo& g, T* k2, T* inv_k2, uint8_t* mask, int32_t*
                          hipStream_t s);
 
 // ---- adjoint building blocks (full layout, dense Geo) ----
-// S[0..2] = u, S[3..5] = i k x u, S[6..8] = vp * (ntot / w)
+// S[0..2] = u, S[3..5] = i k x u, S[6..8] = (P M v) * (ntot / w)
 template <class T>
-hipError_t launch_adj_prep(const cpx<T>* u, const cpx<T>* vp, cpx<T>* S, int64_t fs, const Geo& g,
+hipError_t launch_adj_prep(const cpx<T>* u, const cpx<T>* v, cpx<T>* S, int64_t fs, const Geo& g,
                            double ntot, hipStream_t s);
 // from physical u (0..2), w (3..5), g (6..8): out 0..2 = w x g, out 3..5 = g x u
 template <class T>
 hipError_t launch_adj_cross(const T* ph, int64_t pfs, T* out, int64_t ofs, int64_t n,
                             hipStream_t s);
-// partials[kReduceBlocks][3]: (sum Re(conj(x) y) [over the band if use_band], 0, 0)
+// partials[kReduceBlocks][3]: (sum Re(conj(x) P M y) [over the band if use_band], 0, 0)
 template <class T>
 hipError_t launch_dot_partials(const cpx<T>* x, const cpx<T>* y, int64_t fs, const Geo& g,
                                T kmin2, T kmax2, int use_band, double* partials, hipStream_t s);
@@ -92,11 +92,22 @@ struct AdjCombine {
   const double* scalars;  // [coef, S_E, S_Omega, S_band] of the state (normalised by 1/Ntot^2)
   const double* dot;      // <band u, vp> (BAND) or <vp, u> (LUNDGREN)
 };
-// ubar = (w/Ntot)(A + i k x B) - nu k2 v + forcing adjoint
+// Where the combined value goes: with acc set, acc = acc_base + value (else
+// out = value); with kbn set also kbn = alpha ub + beta value (the next RK4
+// stage's cotangent). kbn may alias v, acc may alias acc_base.
 template <class T>
-hipError_t launch_adj_combine(const cpx<T>* AB, const cpx<T>* v, const cpx<T>* vp,
-                              const cpx<T>* u, cpx<T>* out, int64_t fs, const Geo& g, T kmin2,
-                              T kmax2, const AdjCombine& c, hipStream_t s);
+struct AdjEpilogue {
+  cpx<T>* acc = nullptr;
+  const cpx<T>* acc_base = nullptr;
+  cpx<T>* kbn = nullptr;
+  const cpx<T>* ub = nullptr;
+  T alpha = T(0), beta = T(0);
+};
+// value = (w/Ntot)(A + i k x B) - nu k2 v + forcing adjoint (P M v formed per mode)
+template <class T>
+hipError_t launch_adj_combine(const cpx<T>* AB, const cpx<T>* v, const cpx<T>* u, cpx<T>* out,
+                              int64_t fs, const Geo& g, T kmin2, T kmax2, const AdjCombine& c,
+                              const AdjEpilogue<T>& ep, hipStream_t s);
 // out = a x + b y over n complex values
 template <class T>
 hipError_t launch_axpby(cpx<T>* out, T a, const cpx<T>* x, T b, const cpx<T>* y, int64_t n,
