@@ -568,9 +568,14 @@ VjpBuffers carve_vjp(const hit3d_grid& gd, void* work, size_t* total) {
   c.base = static_cast<char*>(work);
   const size_t cb = cbytes(gd);
   const size_t fs = size_t(gd.nx) * gd.ny * (gd.nz / 2 + 1);
+  // the pruned path's compact, x-inverse and y-inverse layouts (row padding
+  // can make them larger than a dense field on small grids)
+  const Geo gp = make_geo(gd, true);
+  const size_t c_fs = size_t(gp.kx) * gp.jp, a_fs = size_t(gp.nx) * gp.jp;
+  const size_t b_fs = size_t(gp.nx) * gp.ny * gp.kzs;
   VjpBuffers b;
-  b.S = c.take(9 * fs * cb);
-  b.tw = c.take(9 * fs * cb);
+  b.S = c.take(9 * std::max({fs, c_fs, b_fs}) * cb);
+  b.tw = c.take(9 * std::max(fs, a_fs) * cb);
   b.partials = static_cast<double*>(c.take(size_t(3) * h3::kReduceBlocks * sizeof(double)));
   b.scalars = static_cast<double*>(c.take(16 * sizeof(double)));
   b.dot = static_cast<double*>(c.take(16 * sizeof(double)));
@@ -580,10 +585,14 @@ VjpBuffers carve_vjp(const hit3d_grid& gd, void* work, size_t* total) {
 
 // The RHS VJP at u for the cotangent v into `out`, or through the epilogue
 // `ep` (accumulated, and the next RK4 stage's cotangent formed) when given.
+// `pruned`: u has no content outside the 2/3 box, so u, K u and P M v go
+// through the pruned inverse transforms (compact prep, x and y passes over
+// the retained rows, z pass on kz-mode pencils); the forward transforms of
+// the cross products stay dense (ubar has content at every mode).
 template <class T>
 int rhs_vjp_impl(const hit3d_grid& gd, const cpx<T>* u, const cpx<T>* v, cpx<T>* out, double nu,
                  const hit3d_forcing* f, void* work, hipStream_t s, double* pgrad = nullptr,
-                 const h3::AdjEpilogue<T>* ep = nullptr) {
+                 const h3::AdjEpilogue<T>* ep = nullptr, bool pruned = false) {
   size_t total;
   VjpBuffers b = carve_vjp(gd, work, &total);
   const Geo g = make_geo(gd, false);
@@ -591,12 +600,55 @@ int rhs_vjp_impl(const hit3d_grid& gd, const cpx<T>* u, const cpx<T>* v, cpx<T>*
   const double ntot = double(gd.nx) * double(gd.ny) * double(gd.nz);
   cpx<T>* S = static_cast<cpx<T>*>(b.S);
   cpx<T>* tw = static_cast<cpx<T>*>(b.tw);
-  H3_HIP(h3::launch_adj_prep<T>(u, v, S, fs, g, ntot, s));
-  // u, w = K u, g to physical space (x, y, then z fused with w x g, g x u
-  // and the z-forward), then back along y and x into S[0..5]
-  int rc = xy_inverse_impl<T>(gd, S, 9, tw, s);
-  if (rc) return rc;
-  H3_HIP(h3::launch_z_adj<T>(g.nz, tw, fs, int64_t(g.nx) * g.ny, g.nzh, T(1.0 / ntot), s));
+  const int64_t npencils = int64_t(g.nx) * g.ny;
+  int rc = 0;
+  if (pruned) {
+    // S: compact prep [9][Kx][jp], then the y-inverse output [9][Nx][Ny][kzs];
+    // tw: the x-inverse output [9][Nx][jp], then the z pass output (dense)
+    const Geo gp = make_geo(gd, true);
+    const int64_t c_fs = int64_t(gp.kx) * gp.jp, a_fs = int64_t(gp.nx) * gp.jp;
+    const int64_t b_fs = int64_t(gp.nx) * gp.ny * gp.kzs;
+    H3_HIP(h3::launch_adj_prep_compact<T>(u, v, fs, S, c_fs, gp, ntot, s));
+    h3::StridedArgs<T> x{};
+    x.in = S;
+    x.out = tw;
+    x.in_fs = c_fs;
+    x.out_fs = a_fs;
+    x.in_slab = x.out_slab = 0;
+    x.B = gp.jp;
+    x.J = gp.jp;
+    x.nf = 9;
+    x.in_k = gp.kx;
+    x.in_p = gp.px;
+    x.out_k = gp.nx;
+    x.out_p = (gp.nx + 1) / 2;
+    H3_HIP(h3::launch_strided<T>(gp.nx, +1, x, s));
+    h3::StridedArgs<T> y{};
+    y.in = tw;
+    y.out = S;
+    y.in_fs = a_fs;
+    y.out_fs = b_fs;
+    y.in_slab = gp.jp;
+    y.out_slab = int64_t(gp.ny) * gp.kzs;
+    y.B = gp.kzs;
+    y.J = gp.nx * gp.kzs;
+    y.nf = 9;
+    y.in_k = gp.ky;
+    y.in_p = gp.py;
+    y.out_k = gp.ny;
+    y.out_p = (gp.ny + 1) / 2;
+    H3_HIP(h3::launch_strided<T>(gp.ny, +1, y, s));
+    H3_HIP(h3::launch_z_adj<T>(g.nz, S, b_fs, gp.kz, gp.kzs, tw, fs, g.nzh, npencils, T(1.0 / ntot),
+                               true, s));
+  } else {
+    // u, w = K u, g to physical space (x, y, then z fused with w x g, g x u
+    // and the z-forward), in place in tw
+    H3_HIP(h3::launch_adj_prep<T>(u, v, S, fs, g, ntot, s));
+    if ((rc = xy_inverse_impl<T>(gd, S, 9, tw, s))) return rc;
+    H3_HIP(h3::launch_z_adj<T>(g.nz, tw, fs, g.nzh, g.nzh, tw, fs, g.nzh, npencils, T(1.0 / ntot),
+                               false, s));
+  }
+  // the cross products back along y and x into S[0..5]
   if ((rc = yx_forward_impl<T>(gd, tw, S, 6, s))) return rc;
   T kmin2 = T(1), kmax2 = T(0);
   h3::AdjCombine c{};
@@ -705,14 +757,14 @@ int rk4_step_vjp_impl(const hit3d_grid& gd, const cpx<T>* u, const cpx<T>* ub_in
   ep.ub = ub;
   ep.alpha = tdt;
   ep.beta = fdt;
-  if ((rc = rhs_vjp_impl<T>(gd, y4, kb, nullptr, nu, f, b.vjp_ws, s, pgrad, &ep))) return rc;
+  if ((rc = rhs_vjp_impl<T>(gd, y4, kb, nullptr, nu, f, b.vjp_ws, s, pgrad, &ep, pruned))) return rc;
   ep.acc_base = out;
   ep.beta = hdt;
-  if ((rc = rhs_vjp_impl<T>(gd, y3, kb, nullptr, nu, f, b.vjp_ws, s, pgrad, &ep))) return rc;
+  if ((rc = rhs_vjp_impl<T>(gd, y3, kb, nullptr, nu, f, b.vjp_ws, s, pgrad, &ep, pruned))) return rc;
   ep.alpha = sdt;
-  if ((rc = rhs_vjp_impl<T>(gd, y2, kb, nullptr, nu, f, b.vjp_ws, s, pgrad, &ep))) return rc;
+  if ((rc = rhs_vjp_impl<T>(gd, y2, kb, nullptr, nu, f, b.vjp_ws, s, pgrad, &ep, pruned))) return rc;
   ep.kbn = nullptr;
-  if ((rc = rhs_vjp_impl<T>(gd, u, kb, nullptr, nu, f, b.vjp_ws, s, pgrad, &ep))) return rc;
+  if ((rc = rhs_vjp_impl<T>(gd, u, kb, nullptr, nu, f, b.vjp_ws, s, pgrad, &ep, pruned))) return rc;
   return HIT3D_OK;
 }
 

@@ -509,6 +509,42 @@ __global__ __launch_bounds__(kBlock) void k_adj_prep(const cpx<T>* __restrict__ 
   for (int c = 0; c < 3; ++c) S[(6 + c) * fs + idx] = sc * vp[c];
 }
 
+// k_adj_prep into the compact layout of the pruned transforms ([Kx][jp] per
+// field, padding slots zero): for a state with no content outside the 2/3
+// box, u, i k x u and P M v have none either
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_adj_prep_compact(const cpx<T>* __restrict__ u,
+                                                             const cpx<T>* __restrict__ v,
+                                                             int64_t full_fs, cpx<T>* __restrict__ S,
+                                                             int64_t comp_fs, Geo g, double ntot) {
+  const int64_t n = int64_t(g.kx) * g.jp;
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= n) return;
+  const int ix = int(idx / g.jp);
+  const int j = int(idx - int64_t(ix) * g.jp);
+  const int iy = j / g.kzs, iz = j - iy * g.kzs;
+  if (!(iy < g.ky && iz < g.kz)) {  // padding slot
+    for (int f = 0; f < 9; ++f) S[f * comp_fs + idx] = czero<T>();
+    return;
+  }
+  const int x = row_of_compact(ix, g.nx, g.kx, g.px), y = row_of_compact(iy, g.ny, g.ky, g.py);
+  const int64_t src = (int64_t(x) * g.ny + y) * g.nzh + iz;
+  const ModeIdx m = mode_of(src, g);
+  const T kx = wavenumber<T>(g.k0x, m.nx), ky = wavenumber<T>(g.k0y, m.ny),
+          kz = wavenumber<T>(g.k0z, m.nz);
+  const cpx<T> u0 = u[src], u1 = u[full_fs + src], u2 = u[2 * full_fs + src];
+  S[idx] = u0;
+  S[comp_fs + idx] = u1;
+  S[2 * comp_fs + idx] = u2;
+  S[3 * comp_fs + idx] = mul_i(ky * u2 - kz * u1);
+  S[4 * comp_fs + idx] = mul_i(kz * u0 - kx * u2);
+  S[5 * comp_fs + idx] = mul_i(kx * u1 - ky * u0);
+  const T sc = T(ntot / hermitian_weight(m.nz, g.nz));
+  cpx<T> vp[3];
+  masked_projection(v, full_fs, src, m, g, kx, ky, kz, vp);
+  for (int c = 0; c < 3; ++c) S[(6 + c) * comp_fs + idx] = sc * vp[c];
+}
+
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_adj_cross(const T* __restrict__ ph, int64_t pfs,
                                                       T* __restrict__ out, int64_t ofs,
@@ -828,6 +864,15 @@ hipError_t launch_adj_prep(const cpx<T>* u, const cpx<T>* v, cpx<T>* S, int64_t 
 }
 
 template <class T>
+hipError_t launch_adj_prep_compact(const cpx<T>* u, const cpx<T>* v, int64_t full_fs, cpx<T>* S,
+                                   int64_t comp_fs, const Geo& g, double ntot, hipStream_t s) {
+  const int64_t n = int64_t(g.kx) * g.jp;
+  hipLaunchKernelGGL((k_adj_prep_compact<T>), dim3(nblocks_for(n)), dim3(kBlock), 0, s, u, v, full_fs, S,
+                     comp_fs, g, ntot);
+  return hipGetLastError();
+}
+
+template <class T>
 hipError_t launch_adj_cross(const T* ph, int64_t pfs, T* out, int64_t ofs, int64_t n,
                             hipStream_t s) {
   hipLaunchKernelGGL((k_adj_cross<T>), dim3(nblocks_for(n)), dim3(kBlock), 0, s, ph, pfs, out, ofs,
@@ -891,6 +936,8 @@ hipError_t launch_axpby(cpx<T>* out, T a, const cpx<T>* x, T b, const cpx<T>* y,
   template hipError_t launch_adj_prep<T>(const cpx<T>*, const cpx<T>*, cpx<T>*, int64_t,          \
                                          const Geo&, double, hipStream_t);                         \
   template hipError_t launch_adj_cross<T>(const T*, int64_t, T*, int64_t, int64_t, hipStream_t);  \
+  template hipError_t launch_adj_prep_compact<T>(const cpx<T>*, const cpx<T>*, int64_t, cpx<T>*,  \
+                                                 int64_t, const Geo&, double, hipStream_t);        \
   template hipError_t launch_dot_partials<T>(const cpx<T>*, const cpx<T>*, int64_t, const Geo&, T, \
                                              T, int, double*, hipStream_t);                        \
   template hipError_t launch_adj_combine<T>(const cpx<T>*, const cpx<T>*, const cpx<T>*, cpx<T>*, \

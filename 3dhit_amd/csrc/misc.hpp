@@ -58,6 +58,11 @@ hipError_t launch_tables(const Geo& g, T* k2, T* inv_k2, uint8_t* mask, int32_t*
 template <class T>
 hipError_t launch_adj_prep(const cpx<T>* u, const cpx<T>* v, cpx<T>* S, int64_t fs, const Geo& g,
                            double ntot, hipStream_t s);
+// the same nine fields in the compact layout of pruned Geo g (u without
+// content outside the 2/3 box)
+template <class T>
+hipError_t launch_adj_prep_compact(const cpx<T>* u, const cpx<T>* v, int64_t full_fs, cpx<T>* S,
+                                   int64_t comp_fs, const Geo& g, double ntot, hipStream_t s);
 // from physical u (0..2), w (3..5), g (6..8): out 0..2 = w x g, out 3..5 = g x u
 template <class T>
 hipError_t launch_adj_cross(const T* ph, int64_t pfs, T* out, int64_t ofs, int64_t n,

@@ -414,16 +414,20 @@ __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_r2c(const T* __re
 }
 
 // ---------------------------------------------------------------------------
-// Adjoint z pass (dense layout, in place): per z pencil, c2r of the nine
-// fields u, w = K u, g in the pairs (u0 + i u1), (u2 + i w0), (w1 + i w2),
-// (g0 + i g1), g2, scaled by `scale` (1/Ntot), the cross products a = w x g
-// and b = g x u, and r2c of (a0 + i a1), (a2 + i b0), (b1 + i b2) into
-// fields 0..5. The same pairs, scaling and products as z c2r -> cross -> z
-// r2c over physical buffers, without the 15 real fields' round trip.
+// Adjoint z pass: per z pencil, c2r of the nine fields u, w = K u, g in the
+// pairs (u0 + i u1), (u2 + i w0), (w1 + i w2), (g0 + i g1), g2, scaled by
+// `scale` (1/Ntot), the cross products a = w x g and b = g x u, and r2c of
+// (a0 + i a1), (a2 + i b0), (b1 + i b2) into out fields 0..5 (all nzh modes).
+// Input pencils hold kz modes kzs apart (the pruned layout, PRUNED: entry
+// slots in the 2/3 truncation's zero band are compile-time zeros) or nzh
+// (dense; then out may equal in: each pencil's loads precede its stores).
+// The same pairs, scaling and products as z c2r -> cross -> z r2c over
+// physical buffers, without the 15 real fields' round trip.
 // ---------------------------------------------------------------------------
-template <int N, class T>
-__global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_adj(cpx<T>* buf, int64_t fs, int64_t npencils,
-                                                   int nzh, T scale) {
+template <int N, class T, bool PRUNED>
+__global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_adj(const cpx<T>* in, int64_t in_fs, int kz,
+                                                   int kzs, cpx<T>* out, int64_t out_fs, int nzh,
+                                                   int64_t npencils, T scale) {
   using B = PencilBlock<N>;
   using F = typename B::F;
   static_assert(B::THREADS == 64, "WaveSync exchanges need one-wave blocks");
@@ -432,10 +436,11 @@ __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_adj(cpx<T>* buf, 
   const int q = threadIdx.x % Q, p = threadIdx.x / Q;
   const int64_t pp = int64_t(blockIdx.x) * PW + p;
   const bool active = pp < npencils;
-  const int64_t base = (active ? pp : 0) * nzh;
+  const int64_t ib = (active ? pp : 0) * kzs, ob = (active ? pp : 0) * nzh;
   cpx<T>* lp = lds + p * F::kSlots;
   SlotPencil<T> L{lp};
-  const int kzl = active ? nzh : 0;  // inactive pencils load nothing
+  const int kzl = active ? kz : 0;  // inactive pencils load nothing
+  constexpr unsigned kZeroIn = PRUNED ? ZeroBand<F, N>::kEntry : 0u;
   cpx<T> tw[R];
   F::load_tw(tw, q);
   cpx<T> z[5][R];
@@ -443,8 +448,10 @@ __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_adj(cpx<T>* buf, 
   for (int c = 0; c < 5; ++c) {
 #pragma unroll
     for (int r = 0; r < R; ++r)
-      z[c][r] = herm_pair<N, T>(buf + 2 * c * fs + base, c < 4 ? buf + (2 * c + 1) * fs + base : nullptr,
-                                q + Q * r, kzl);
+      z[c][r] = ((kZeroIn >> r) & 1u) ? czero<T>()
+                                      : herm_pair<N, T>(in + 2 * c * in_fs + ib,
+                                                        c < 4 ? in + (2 * c + 1) * in_fs + ib : nullptr,
+                                                        q + Q * r, kzl);
     F::template run<+1, WaveSync>(z[c], q, L, tw);
   }
   cpx<T> X[3][R];
@@ -469,8 +476,8 @@ __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_adj(cpx<T>* buf, 
       for (int k = q; k < nzh; k += Q) {
         cpx<T> a, b;
         split_pair(lp[k], lp[(N - k) & (N - 1)], a, b);
-        buf[2 * c * fs + base + k] = a;
-        buf[(2 * c + 1) * fs + base + k] = b;
+        out[2 * c * out_fs + ob + k] = a;
+        out[(2 * c + 1) * out_fs + ob + k] = b;
       }
     }
   }
@@ -739,12 +746,17 @@ hipError_t launch_z_r2c(int N, const T* in, int64_t in_fs, cpx<T>* out, int64_t 
 }
 
 template <class T>
-hipError_t launch_z_adj(int N, cpx<T>* buf, int64_t fs, int64_t npencils, int nzh, T scale,
+hipError_t launch_z_adj(int N, const cpx<T>* in, int64_t in_fs, int kz, int kzs, cpx<T>* out,
+                        int64_t out_fs, int nzh, int64_t npencils, T scale, bool pruned,
                         hipStream_t s) {
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(npencils, PencilBlock<NN>::PW);
-    hipLaunchKernelGGL((k_z_adj<NN, T>), dim3(nb), dim3(PencilBlock<NN>::THREADS), 0, s, buf, fs, npencils,
-                       nzh, scale);
+    if (pruned)
+      hipLaunchKernelGGL((k_z_adj<NN, T, true>), dim3(nb), dim3(PencilBlock<NN>::THREADS), 0, s, in, in_fs, kz,
+                         kzs, out, out_fs, nzh, npencils, scale);
+    else
+      hipLaunchKernelGGL((k_z_adj<NN, T, false>), dim3(nb), dim3(PencilBlock<NN>::THREADS), 0, s, in, in_fs,
+                         kz, kzs, out, out_fs, nzh, npencils, scale);
   });
   return hipGetLastError();
 }
@@ -783,7 +795,8 @@ hipError_t launch_x_forward_stage(int N, int mode, const StageArgs<T>& a, hipStr
                                       int, T, hipStream_t);                                      \
   template hipError_t launch_z_r2c<T>(int, const T*, int64_t, cpx<T>*, int64_t, int, int64_t,   \
                                       int, hipStream_t);                                         \
-  template hipError_t launch_z_adj<T>(int, cpx<T>*, int64_t, int64_t, int, T, hipStream_t);           \
+  template hipError_t launch_z_adj<T>(int, const cpx<T>*, int64_t, int, int, cpx<T>*, int64_t, int,   \
+                                      int64_t, T, bool, hipStream_t);                              \
   template hipError_t launch_x_forward_stage<T>(int, int, const StageArgs<T>&, hipStream_t, int*);
 
 H3_INSTANTIATE(H3_REAL)

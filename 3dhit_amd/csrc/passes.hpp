@@ -178,9 +178,11 @@ hipError_t launch_z_c2r(int N, const cpx<T>* in, int64_t in_fs, T* out, int64_t 
 template <class T>
 hipError_t launch_z_r2c(int N, const T* in, int64_t in_fs, cpx<T>* out, int64_t out_fs, int nf,
                         int64_t npencils, int kz, hipStream_t s);
-// adjoint z pass: nine fields in (dense), c2r, w x g and g x u, r2c into fields 0..5
+// adjoint z pass: nine fields in (pencils of kz modes kzs apart), c2r,
+// w x g and g x u, r2c into out fields 0..5 (pencils of nzh modes)
 template <class T>
-hipError_t launch_z_adj(int N, cpx<T>* buf, int64_t fs, int64_t npencils, int nzh, T scale,
+hipError_t launch_z_adj(int N, const cpx<T>* in, int64_t in_fs, int kz, int kzs, cpx<T>* out,
+                        int64_t out_fs, int nzh, int64_t npencils, T scale, bool pruned,
                         hipStream_t s);
 template <class T>
 hipError_t launch_x_forward_stage(int N, int mode, const StageArgs<T>& a, hipStream_t s,

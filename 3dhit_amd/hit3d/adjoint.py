@@ -68,13 +68,16 @@ def _pg_ptr(pg, like):
     return pg.data_ptr()
 
 
-def _step_vjp_into(grid, u, ub, params, dt, t, out, param_grad=None, stages=None):
-    """Without `stages` the step's stage inputs are recomputed, with the
-    pruned transforms when u carries the dealiased mark (rollout's
-    checkpoints do): no device read."""
+def _step_vjp_into(grid, u, ub, params, dt, t, out, param_grad=None, stages=None, dealiased=None):
+    """Without `stages` the step's stage inputs are recomputed. The pruned
+    transforms (stage recomputation and the inverse transforms of the
+    stage adjoints) are taken when u carries the dealiased mark (rollout's
+    checkpoints do) or `dealiased` says so: no device read."""
     ws = grid.workspace(_lib.OP_RK4_VJP)
     f = params.forcing.desc()
-    inside = 1 if getattr(u, DEALIASED_MARK, None) == u._version else 0
+    if dealiased is None:
+        dealiased = getattr(u, DEALIASED_MARK, None) == u._version
+    inside = 1 if dealiased else 0
     _lib.call("hit3d_rk4_step_vjp", grid.desc, u.data_ptr(), ub.data_ptr(), out.data_ptr(),
               float(params.nu), float(dt), float(t), ctypes.byref(f), inside,
               None if stages is None else stages.data_ptr(), _pg_ptr(param_grad, u), ws.data_ptr(),
@@ -97,14 +100,15 @@ _RECORD_SHARE = 0.5
 
 
 def _record_segment(rhs, u0, params, dt, nsteps, t0):
-    """[nsteps][4][3][Nx][Ny][Nz/2+1]: per step its start state and the
-    inputs of stages 2..4 (hit3d_rk4_steps_record), or None when that does
-    not fit in device memory."""
+    """([nsteps][4][3][Nx][Ny][Nz/2+1], dealiased): per step its start state
+    and the inputs of stages 2..4 (hit3d_rk4_steps_record), and whether they
+    lie inside the 2/3 box; (None, None) when the record does not fit in
+    device memory."""
     g = rhs.grid
     per_step = 4 * u0.numel() * u0.element_size()
     free, _ = torch.cuda.mem_get_info(u0.device)
     if nsteps * per_step > _RECORD_SHARE * free:
-        return None
+        return None, None
     rec = torch.empty((nsteps, 4) + tuple(u0.shape), dtype=u0.dtype, device=u0.device)
     inside = known_dealiased(u0, g)
     u = u0.clone()
@@ -113,7 +117,7 @@ def _record_segment(rhs, u0, params, dt, nsteps, t0):
     f = params.forcing.desc()
     _lib.call("hit3d_rk4_steps_record", g.desc, u.data_ptr(), float(params.nu), float(dt), int(nsteps),
               float(t0), ctypes.byref(f), pruned, rec.data_ptr(), ws.data_ptr(), ws.numel(), stream())
-    return rec
+    return rec, inside
 
 
 def rollout_vjp(rhs, u_hat, ubar, params, dt, nsteps, t0=0.0, checkpoint=20, param_grad=None,
@@ -141,11 +145,11 @@ def rollout_vjp(rhs, u_hat, ubar, params, dt, nsteps, t0=0.0, checkpoint=20, par
             ckpt[b] = _rollout(rhs, ckpt[a], params, dt, b - a, t0 + a * dt)
     for s in reversed(starts):
         e = min(s + checkpoint, int(nsteps))
-        rec = _record_segment(rhs, ckpt[s], params, dt, e - s, t0 + s * dt)
+        rec, inside = _record_segment(rhs, ckpt[s], params, dt, e - s, t0 + s * dt)
         if rec is not None:
             for n in range(e - 1, s - 1, -1):
                 _step_vjp_into(g, rec[n - s, 0], ub, params, dt, t0 + n * dt, ub, param_grad,
-                               stages=rec[n - s, 1])
+                               stages=rec[n - s, 1], dealiased=inside)
             del rec
             continue
         states = [ckpt[s]]

@@ -199,11 +199,13 @@ int hit3d_rhs_vjp(const hit3d_grid* grid, const void* u_hat, const void* v, void
    three states y2, y3, y4 one after the other, as hit3d_rk4_steps_record
    keeps them) or, when `stages` is NULL, recomputes them; then runs the four
    stage adjoints in reverse. ubar may alias ubar_next, not u_hat or stages.
-   `dealiased` = 1 promises that u_hat has no content outside the 2/3 box (as
-   for hit3d_rhs): the stage inputs are then recomputed with the pruned
-   transforms. `param_grad` (device, 3 doubles, or NULL) accumulates the
-   step's parameter derivatives as in hit3d_rhs_vjp, summed over the four
-   stages. */
+   `dealiased` = 1 promises that u_hat and the stage inputs have no content
+   outside the 2/3 box (as for hit3d_rhs; hit3d_rk4_steps_record run with
+   dealiased = 1 keeps such stages): the stage inputs are then recomputed
+   with the pruned transforms and the stage adjoints transform u, K u and the
+   projected cotangent with the pruned inverse passes. `param_grad` (device,
+   3 doubles, or NULL) accumulates the step's parameter derivatives as in
+   hit3d_rhs_vjp, summed over the four stages. */
 int hit3d_rk4_step_vjp(const hit3d_grid* grid, const void* u_hat, const void* ubar_next,
                        void* ubar, double nu, double dt, double t, const hit3d_forcing* forcing,
                        int32_t dealiased, const void* stages, double* param_grad, void* work,
