@@ -1114,6 +1114,37 @@ int hit3d_energy_dissipation(const hit3d_grid* grid, const void* u_hat, double n
                                  static_cast<hipStream_t>(stream));
 }
 
+int hit3d_forcing_term(const hit3d_grid* grid, const void* u_hat, const hit3d_forcing* forcing,
+                       void* out, void* work, size_t work_bytes, void* stream) {
+  H3_TRY(check_grid(grid));
+  H3_TRY(check_forcing(forcing));
+  H3_TRY(check_ptr(u_hat, "u_hat"));
+  H3_TRY(check_ptr(out, "out"));
+  if (out == u_hat) return fail(HIT3D_ERR_INVALID, "out must not alias u_hat");
+  H3_TRY(check_work(work, work_bytes, reduce_bytes(*grid, nullptr, nullptr, nullptr)));
+  double *partials, *scalars;
+  reduce_bytes(*grid, &partials, &scalars, work);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const Geo g = make_geo(*grid, false);
+  const int64_t fs = int64_t(g.nx) * g.ny * g.nzh;
+  const int kind = forcing ? forcing->kind : HIT3D_FORCING_NONE;
+  const bool band = kind == HIT3D_FORCING_BAND;
+  auto run = [&](auto tag) -> int {
+    using T = decltype(tag);
+    const cpx<T>* u = static_cast<const cpx<T>*>(u_hat);
+    // the coefficient of BAND / LUNDGREN from the FP64 sums of u, on the device
+    if (needs_sums(forcing)) H3_TRY(reduce_state<T>(*grid, g, u, forcing, partials, scalars, s));
+    const T c = kind == HIT3D_FORCING_LINEAR ? T(forcing->a) : T(0);
+    H3_HIP(h3::launch_forcing_term<T>(u, static_cast<cpx<T>*>(out), fs, g, band ? 1 : 0, c,
+                                      needs_sums(forcing) ? scalars : nullptr,
+                                      band ? T(forcing->kmin * forcing->kmin) : T(1),
+                                      band ? T(forcing->kmax * forcing->kmax) : T(0), s));
+    return HIT3D_OK;
+  };
+  if (grid->precision == 4) return run(float{});
+  return run(double{});
+}
+
 int hit3d_energy_dissipation_noalloc(const hit3d_grid* grid, const void* u_hat, double nu,
                                      double kmin, double kmax, double* out, void* stream) {
   H3_TRY(check_grid(grid));

@@ -746,6 +746,27 @@ __global__ __launch_bounds__(kBlock) void k_adj_combine(const cpx<T>* __restrict
   }
 }
 
+// The forcing term f(u) of the forcing plug-ins (forcing.py:74-163) over the
+// full layout: out = c u (uniform: LINEAR with c = A, LUNDGREN with the
+// device coefficient) or c [kmin2 <= k2 <= kmax2] u (BAND, device c).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_forcing_term(const cpx<T>* __restrict__ u,
+                                                         cpx<T>* __restrict__ out, int64_t fs, Geo g,
+                                                         int band, T coef_const,
+                                                         const double* __restrict__ coef_dev, T kmin2,
+                                                         T kmax2) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= n) return;
+  T c = coef_dev ? T(*coef_dev) : coef_const;
+  if (band) {
+    const ModeIdx m = mode_of(idx, g);
+    const T k2 = k2_of(wavenumber<T>(g.k0x, m.nx), wavenumber<T>(g.k0y, m.ny), wavenumber<T>(g.k0z, m.nz));
+    if (!(kmin2 <= k2 && k2 <= kmax2)) c = T(0);
+  }
+  for (int f = 0; f < 3; ++f) out[f * fs + idx] = c * u[f * fs + idx];
+}
+
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_axpby(cpx<T>* out, T a, const cpx<T>* x, T b,
                                                   const cpx<T>* y, int64_t n) {
@@ -926,6 +947,15 @@ hipError_t launch_adj_combine(const cpx<T>* AB, const cpx<T>* v, const cpx<T>* u
 }
 
 template <class T>
+hipError_t launch_forcing_term(const cpx<T>* u, cpx<T>* out, int64_t fs, const Geo& g, int band,
+                               T coef_const, const double* coef_dev, T kmin2, T kmax2, hipStream_t s) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nzh;
+  hipLaunchKernelGGL((k_forcing_term<T>), dim3(nblocks_for(n)), dim3(kBlock), 0, s, u, out, fs, g, band,
+                     coef_const, coef_dev, kmin2, kmax2);
+  return hipGetLastError();
+}
+
+template <class T>
 hipError_t launch_axpby(cpx<T>* out, T a, const cpx<T>* x, T b, const cpx<T>* y, int64_t n,
                         hipStream_t s) {
   hipLaunchKernelGGL((k_axpby<T>), dim3(nblocks_for(n)), dim3(kBlock), 0, s, out, a, x, b, y, n);
@@ -943,6 +973,8 @@ hipError_t launch_axpby(cpx<T>* out, T a, const cpx<T>* x, T b, const cpx<T>* y,
   template hipError_t launch_adj_combine<T>(const cpx<T>*, const cpx<T>*, const cpx<T>*, cpx<T>*, \
                                             int64_t, const Geo&, T, T, const AdjCombine&,          \
                                             const AdjEpilogue<T>&, hipStream_t);                   \
+  template hipError_t launch_forcing_term<T>(const cpx<T>*, cpx<T>*, int64_t, const Geo&, int, T,  \
+                                             const double*, T, T, hipStream_t);                    \
   template hipError_t launch_axpby<T>(cpx<T>*, T, const cpx<T>*, T, const cpx<T>*, int64_t,       \
                                       hipStream_t);                                                       \
   template hipError_t launch_hold_partials<T>(const cpx<T>*, const cpx<T>*, int64_t, const Geo&,  \

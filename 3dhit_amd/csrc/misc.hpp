@@ -113,6 +113,11 @@ template <class T>
 hipError_t launch_adj_combine(const cpx<T>* AB, const cpx<T>* v, const cpx<T>* u, cpx<T>* out,
                               int64_t fs, const Geo& g, T kmin2, T kmax2, const AdjCombine& c,
                               const AdjEpilogue<T>& ep, hipStream_t s);
+// forcing term over the full layout: out = c u, or c [kmin2 <= k2 <= kmax2] u
+// when band; c = *coef_dev when coef_dev is set, else coef_const
+template <class T>
+hipError_t launch_forcing_term(const cpx<T>* u, cpx<T>* out, int64_t fs, const Geo& g, int band,
+                               T coef_const, const double* coef_dev, T kmin2, T kmax2, hipStream_t s);
 // out = a x + b y over n complex values
 template <class T>
 hipError_t launch_axpby(cpx<T>* out, T a, const cpx<T>* x, T b, const cpx<T>* y, int64_t n,

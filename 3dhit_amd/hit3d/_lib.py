@@ -64,6 +64,7 @@ SIGNATURES = {
     "hit3d_rk4_if_steps": [_G, _P, _D, _D, _I64, _D, _F, _I32, _P, _P, _SZ, _P],
     "hit3d_shell_spectrum": [_G, _P, _I32, _I32, _P, _P, _SZ, _P],
     "hit3d_energy_dissipation": [_G, _P, _D, _D, _D, _P, _P, _SZ, _P],
+    "hit3d_forcing_term": [_G, _P, _F, _P, _P, _SZ, _P],
     "hit3d_energy_dissipation_noalloc": [_G, _P, _D, _D, _D, _P, _P],
     "hit3d_alias_energy": [_G, _P, _P, _P, _SZ, _P],
     "hit3d_uniform_noise": [_G, ctypes.c_uint64, _P, _P],

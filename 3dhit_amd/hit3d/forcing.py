@@ -5,11 +5,14 @@ Inside `rhs` / `rk4_step` / `rollout` the forcing never runs as separate
 tensor code: its descriptor (`desc`) is handed to the fused HIP stage
 kernel, and the coefficients that depend on the state (BandForcing's
 eps/2E_band, LundgrenForcing's A) are reduced in FP64 on the device at every
-RK4 stage. `term` and `injection` are the standalone API forms.
+RK4 stage. `term` and `injection` are the standalone API forms; `term` runs
+the library's hit3d_forcing_term (coefficient and field on the device, no
+host read).
 """
 
 from __future__ import annotations
 
+import ctypes
 import math
 from dataclasses import dataclass
 
@@ -18,6 +21,7 @@ import torch
 
 from . import _lib
 from .diagnostics import quadratic_sums
+from .grid import check_spectral, stream
 
 __all__ = ["NoForcing", "LinearForcing", "BandForcing", "LundgrenForcing"]
 
@@ -28,6 +32,17 @@ def _fmt(x):
 
 def _tiny(grid):
     return float(np.finfo(np.float32 if grid.dtype == torch.float32 else np.float64).eps)
+
+
+def _device_term(forcing, u_hat, grid):
+    """f(u_hat) of `forcing` by hit3d_forcing_term (a new state tensor)."""
+    u = check_spectral(u_hat, grid, 3)
+    out = torch.empty_like(u)
+    ws = grid.workspace(_lib.OP_REDUCE)
+    f = forcing.desc()
+    _lib.call("hit3d_forcing_term", grid.desc, u.data_ptr(), ctypes.byref(f), out.data_ptr(),
+              ws.data_ptr(), ws.numel(), stream())
+    return out
 
 
 @dataclass(frozen=True)
@@ -61,7 +76,7 @@ class LinearForcing:
         return _lib.ForcingDesc(_lib.FORCING_LINEAR, 0, float(self.A))
 
     def term(self, u_hat, grid, t):
-        return float(self.A) * u_hat
+        return _device_term(self, u_hat, grid)
 
     def injection(self, u_hat, grid):
         return float(self.A) * 2.0 * float(quadratic_sums(u_hat, grid)[0])
@@ -91,13 +106,7 @@ class BandForcing:
         return 2.0 * float(quadratic_sums(u_hat, grid, 0.0, self.kmin, self.kmax)[3])
 
     def term(self, u_hat, grid, t):
-        twoE = self._two_e_band(u_hat, grid)
-        c = float(self.eps) / twoE if twoE > _tiny(grid) else 0.0
-        k2 = grid.k2
-        lo = torch.tensor(self.kmin**2, dtype=grid.dtype)
-        hi = torch.tensor(self.kmax**2, dtype=grid.dtype)
-        mask = (lo.to(k2.device) <= k2) & (k2 <= hi.to(k2.device))
-        return (c * mask.to(u_hat.dtype)) * u_hat
+        return _device_term(self, u_hat, grid)
 
     def injection(self, u_hat, grid):
         return float(self.eps) if self._two_e_band(u_hat, grid) > _tiny(grid) else 0.0
@@ -138,7 +147,7 @@ class LundgrenForcing:
         return rate / twoE
 
     def term(self, u_hat, grid, t):
-        return self.coefficient(u_hat, grid) * u_hat
+        return _device_term(self, u_hat, grid)
 
     def injection(self, u_hat, grid):
         return self.coefficient(u_hat, grid) * 2.0 * float(quadratic_sums(u_hat, grid)[0])

@@ -222,6 +222,16 @@ int hit3d_rk4_steps_record(const hit3d_grid* grid, void* u_hat, double nu, doubl
                            double t0, const hit3d_forcing* forcing, int32_t dealiased, void* record,
                            void* work, size_t work_bytes, void* stream);
 
+/* The forcing plug-in's term(u_hat, grid, t) (forcing.py:5-20, 55-163):
+   out = f(u_hat), the field the RHS adds before dealiasing and projection —
+   0 (NONE), A u (LINEAR), (eps / 2E_band) [kmin^2 <= k2 <= kmax^2] u with the
+   finfo(dtype).eps guard (BAND), A(u) u of the constant-energy forcing
+   (LUNDGREN). The coefficients come from FP64 sums of u_hat reduced on the
+   device (no host read). Workspace of HIT3D_OP_REDUCE. out must not alias
+   u_hat. */
+int hit3d_forcing_term(const hit3d_grid* grid, const void* u_hat, const hit3d_forcing* forcing,
+                       void* out, void* work, size_t work_bytes, void* stream);
+
 /* Measurement kernel (no reference counterpart): dst = src over `bytes`
    (a multiple of 16; both pointers 16-byte aligned) by a plain streaming
    copy, with non-temporal loads and stores when `nontemporal` is non-zero.
