@@ -546,23 +546,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_prep_compact(const cpx<T>* __res
 }
 
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_adj_cross(const T* __restrict__ ph, int64_t pfs,
-                                                      T* __restrict__ out, int64_t ofs,
-                                                      int64_t n) {
-  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const T u0 = ph[i], u1 = ph[pfs + i], u2 = ph[2 * pfs + i];
-  const T w0 = ph[3 * pfs + i], w1 = ph[4 * pfs + i], w2 = ph[5 * pfs + i];
-  const T g0 = ph[6 * pfs + i], g1 = ph[7 * pfs + i], g2 = ph[8 * pfs + i];
-  out[i] = w1 * g2 - w2 * g1;
-  out[ofs + i] = w2 * g0 - w0 * g2;
-  out[2 * ofs + i] = w0 * g1 - w1 * g0;
-  out[3 * ofs + i] = g1 * u2 - g2 * u1;
-  out[4 * ofs + i] = g2 * u0 - g0 * u2;
-  out[5 * ofs + i] = g0 * u1 - g1 * u0;
-}
-
-template <class T>
 __global__ __launch_bounds__(kBlock) void k_dot_partials(const cpx<T>* __restrict__ x,
                                                          const cpx<T>* __restrict__ v, int64_t fs,
                                                          Geo g, T kmin2, T kmax2, int use_band,
@@ -894,14 +877,6 @@ hipError_t launch_adj_prep_compact(const cpx<T>* u, const cpx<T>* v, int64_t ful
 }
 
 template <class T>
-hipError_t launch_adj_cross(const T* ph, int64_t pfs, T* out, int64_t ofs, int64_t n,
-                            hipStream_t s) {
-  hipLaunchKernelGGL((k_adj_cross<T>), dim3(nblocks_for(n)), dim3(kBlock), 0, s, ph, pfs, out, ofs,
-                     n);
-  return hipGetLastError();
-}
-
-template <class T>
 hipError_t launch_dot_partials(const cpx<T>* x, const cpx<T>* y, int64_t fs, const Geo& g,
                                T kmin2, T kmax2, int use_band, double* partials, hipStream_t s) {
   hipLaunchKernelGGL((k_dot_partials<T>), dim3(kReduceBlocks), dim3(kBlock), 0, s, x, y, fs, g,
@@ -965,7 +940,6 @@ hipError_t launch_axpby(cpx<T>* out, T a, const cpx<T>* x, T b, const cpx<T>* y,
 #define H3_INSTANTIATE_ADJ(T)                                                                      \
   template hipError_t launch_adj_prep<T>(const cpx<T>*, const cpx<T>*, cpx<T>*, int64_t,          \
                                          const Geo&, double, hipStream_t);                         \
-  template hipError_t launch_adj_cross<T>(const T*, int64_t, T*, int64_t, int64_t, hipStream_t);  \
   template hipError_t launch_adj_prep_compact<T>(const cpx<T>*, const cpx<T>*, int64_t, cpx<T>*,  \
                                                  int64_t, const Geo&, double, hipStream_t);        \
   template hipError_t launch_dot_partials<T>(const cpx<T>*, const cpx<T>*, int64_t, const Geo&, T, \

@@ -63,10 +63,6 @@ hipError_t launch_adj_prep(const cpx<T>* u, const cpx<T>* v, cpx<T>* S, int64_t 
 template <class T>
 hipError_t launch_adj_prep_compact(const cpx<T>* u, const cpx<T>* v, int64_t full_fs, cpx<T>* S,
                                    int64_t comp_fs, const Geo& g, double ntot, hipStream_t s);
-// from physical u (0..2), w (3..5), g (6..8): out 0..2 = w x g, out 3..5 = g x u
-template <class T>
-hipError_t launch_adj_cross(const T* ph, int64_t pfs, T* out, int64_t ofs, int64_t n,
-                            hipStream_t s);
 // partials[kReduceBlocks][3]: (sum Re(conj(x) P M y) [over the band if use_band], 0, 0)
 template <class T>
 hipError_t launch_dot_partials(const cpx<T>* x, const cpx<T>* y, int64_t fs, const Geo& g,
