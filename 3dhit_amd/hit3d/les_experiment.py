@@ -87,7 +87,12 @@ def _e_omega(u, grid):
     return float(s[0]), float(s[2])
 
 
-def generate(fine_n, les_n, kcut, nu, dt_dns, stride, spinup_time, nsnap, seed, holdout_time=0.0):
+def generate(fine_n, les_n, kcut, nu, dt_dns, stride, spinup_time, nsnap, seed, holdout_time=0.0,
+             record_training=True):
+    """The DNS, its filtered training record and the held-out record. With
+    record_training=False the DNS still runs through the training window in
+    the same calls (so the held-out record is the same), without filtering
+    or keeping its snapshots."""
     fine = make_grid(fine_n, dtype=torch.float32)
     coarse = make_grid(les_n, dtype=torch.float32)
     rhs = make_ns_rhs(fine)
@@ -104,9 +109,9 @@ def generate(fine_n, les_n, kcut, nu, dt_dns, stride, spinup_time, nsnap, seed, 
         done += n
         E, Om = _e_omega(u, fine)
         _log(f"  spin-up step {done}/{nspin}: E = {E:.4f}, Omega = {Om:.2f}, {1e3 * dns_s / done:.2f} ms/step")
-    shape = (nsnap, 3, les_n, les_n, les_n // 2 + 1)
+    shape = (nsnap if record_training else 0, 3, les_n, les_n, les_n // 2 + 1)
     ubar = torch.empty(shape, dtype=coarse.complex_dtype, device=coarse.device)
-    tau = torch.empty((nsnap, 6) + shape[2:], dtype=coarse.complex_dtype, device=coarse.device)
+    tau = torch.empty((shape[0], 6) + shape[2:], dtype=coarse.complex_dtype, device=coarse.device)
     coef = np.empty(nsnap)
     e_res = np.empty(nsnap)
     t0 = nspin * dt_dns
@@ -114,6 +119,8 @@ def generate(fine_n, les_n, kcut, nu, dt_dns, stride, spinup_time, nsnap, seed, 
     for s in range(nsnap):
         u, sec = _timed(lambda: rollout(rhs, u, params, dt_dns, stride, t0 + s * stride * dt_dns))
         dns_steps_s += sec
+        if not record_training:
+            continue
         (ub, th), sec = _timed(lambda: subgrid_stress(u, fine, coarse, kcut))
         stress_s += sec
         ubar[s], tau[s] = ub, th
@@ -143,7 +150,7 @@ def generate(fine_n, les_n, kcut, nu, dt_dns, stride, spinup_time, nsnap, seed, 
     del u
     return {"fine": fine, "coarse": coarse, "ubar": ubar, "tau": tau, "coef": coef, "e_res": e_res,
             "dns_ms_per_step": 1e3 * dns_steps_s / ((nsnap + nhold) * stride),
-            "stress_ms": 1e3 * stress_s / nsnap, "t_start": t0,
+            "stress_ms": 1e3 * stress_s / max(shape[0], 1), "t_start": t0,
             "hold_start": hold_start, "hold_spectra": hold_spectra, "hold_energy": hold_e[:nhold]}
 
 
@@ -478,6 +485,47 @@ def run(fine_n=512, les_n=64, kcut=21.0, nu=1 / 2000, dt_dns=2e-3, stride=5, spi
     return res
 
 
+def evaluate_saved(path, windows=(2000,), seed=0, tol=0.05):
+    """A saved closure against held-out data regenerated from its
+    configuration: the DNS (seed `seed`, the experiment's default) is run
+    again through spin-up and the training window in the same calls, then
+    the held-out record is taken; an LES from the held-out start state is
+    compared with it over each window of LES steps in `windows` (the
+    benchmark configuration judges 2000 steps) and over the whole held-out
+    record. Beside each: the same LES without the closure, and the held-out
+    DNS record's own first-half-versus-second-half difference over that
+    window (its sampling noise at that averaging length)."""
+    cfg = json.loads(bytes(np.load(path)["config"]).decode())
+    kcut, kmax, dt = float(cfg["kcut"]), int(cfg["kcut"]), float(cfg["dt_les"])
+    data = generate(cfg["fine_n"], cfg["les_n"], kcut, cfg["nu"], cfg["dt_dns"], cfg["stride"],
+                    cfg["spinup_time"], cfg["nsnap"], seed, cfg["holdout_time"], record_training=False)
+    coarse = data["coarse"]
+    net, target, e_target, _ = load_closure(path, coarse)
+    params = NSParams(nu=cfg["nu"], forcing=HoldResolvedEnergy(e_target, 1.0))
+    mask = sphere_mask(coarse, kcut)
+    hold = data["hold_spectra"]
+    ref = np.concatenate([_spectrum(data["hold_start"], coarse, kmax)[None], hold[1:]], axis=0)
+    nfull = ref.shape[0] - 1
+    res = {"closure": str(path), "config": cfg, "seed": seed, "windows": []}
+    for w in sorted({min(int(x), nfull) for x in windows} | {nfull}):
+        cl = evaluate(net, coarse, data["hold_start"], ref, params, dt, w, kmax, mask)
+        bare = evaluate(None, coarse, data["hold_start"], ref, params, dt, w, kmax, mask)
+        rows = ref[10:10 * (w // 10) + 1:10]
+        h = len(rows) // 2
+        noise = np.abs(rows[:h].mean(0) / rows[h:].mean(0) - 1.0)
+        err = np.asarray(cl["rel_err"])
+        row = {"les_steps": w, "time_units": w * dt, "max_rel_err": cl["max_rel_err"],
+               "worst_k": cl["worst_k"], "max_rel_err_k_ge_3": float(err[2:].max()),
+               "rel_err": cl["rel_err"], "no_closure_max_rel_err": bare["max_rel_err"],
+               "dns_half_vs_half_max": float(noise.max()), "dns_half_vs_half": noise.tolist(),
+               "accepted": bool(cl["max_rel_err"] <= tol)}
+        res["windows"].append(row)
+        _log(f"held-out window of {w} LES steps ({w * dt:g} time units): closure {cl['max_rel_err']:.3f} "
+             f"(k = {cl['worst_k']}; k >= 3: {row['max_rel_err_k_ge_3']:.3f}), no closure "
+             f"{bare['max_rel_err']:.3f}; the DNS record's own half-vs-half {noise.max():.3f}")
+    return res
+
+
 def drift(path, time_units=120.0, variants=("as saved", "gain x0.85", "gain x1.15", "cut k < 2.5", "cut k < 4.5"),
           seed=5, windows=((10, 40), (40, 80), (80, 120))):
     """Long-run behaviour of a saved closure: an LES from a random-phase
@@ -524,6 +572,11 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--drift", default=None, metavar="CLOSURE_NPZ",
                     help="instead of the experiment: the long-run drift of a saved closure")
+    ap.add_argument("--evaluate", default=None, metavar="CLOSURE_NPZ",
+                    help="instead of the experiment: a saved closure against held-out data "
+                         "regenerated from its configuration")
+    ap.add_argument("--eval-steps", dest="eval_steps", type=int, default=2000,
+                    help="LES steps of the judged held-out window (with --evaluate)")
     ap.add_argument("--quick", action="store_true", help="small smoke configuration (128^3 -> 32^3)")
     ap.add_argument("--out", default=None, help="write the result JSON here")
     ap.add_argument("--save", default=None, help="write the trained closure (.npz) here")
@@ -535,8 +588,8 @@ def main(argv=None):
     ap.add_argument("--windows", type=int, default=None)
     ap.add_argument("--attractor-windows", dest="attractor_windows", type=int, default=None)
     a = ap.parse_args(argv)
-    if a.drift:
-        res = drift(a.drift)
+    if a.drift or a.evaluate:
+        res = drift(a.drift) if a.drift else evaluate_saved(a.evaluate, windows=(a.eval_steps,))
         line = json.dumps(res)
         if a.out:
             with open(a.out, "w") as f:

@@ -81,3 +81,26 @@ def test_workspace_sizes_and_null_checks():
     rc = lib.hit3d_rhs(ctypes.byref(g), ctypes.c_void_p(256), ctypes.c_void_p(512), 0.1,
                        ctypes.byref(bad), 0.0, 1, None, 0, None)
     assert rc == L.HIT3D_ERR_INVALID and "forcing" in L.last_error()
+
+
+def test_adjoint_record_and_forcing_term_argument_checks():
+    """Argument checks of hit3d_rk4_steps_record, hit3d_rk4_step_vjp's
+    stages and hit3d_forcing_term: they fail before any device work (the
+    pointers below are never dereferenced)."""
+    from hit3d import _lib as L
+    lib = _lib()
+    g = _grid(16, 16, 16)
+    f = L.ForcingDesc(L.FORCING_NONE, 0)
+    P = ctypes.c_void_p
+    rc = lib.hit3d_rk4_steps_record(ctypes.byref(g), P(4096), 0.1, 1e-3, 2, 0.0, ctypes.byref(f), 1, None,
+                                    None, 0, None)
+    assert rc == L.HIT3D_ERR_INVALID and "record" in L.last_error()
+    rc = lib.hit3d_rk4_steps_record(ctypes.byref(g), P(4096), 0.1, 1e-3, 2, 0.0, ctypes.byref(f), 1, P(4096),
+                                    None, 0, None)
+    assert rc == L.HIT3D_ERR_INVALID and "alias" in L.last_error()
+    state = 3 * 16 * 16 * 9 * 8                       # one complex64 state of a 16^3 grid
+    rc = lib.hit3d_rk4_step_vjp(ctypes.byref(g), P(1 << 20), P(2 << 20), P((3 << 20) + state), 0.1, 1e-3,
+                                0.0, ctypes.byref(f), 1, P(3 << 20), None, None, 0, None)
+    assert rc == L.HIT3D_ERR_INVALID and "stages" in L.last_error()   # ubar inside the stage inputs
+    rc = lib.hit3d_forcing_term(ctypes.byref(g), P(4096), ctypes.byref(f), P(4096), None, 0, None)
+    assert rc == L.HIT3D_ERR_INVALID and "alias" in L.last_error()
