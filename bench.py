@@ -357,6 +357,8 @@ def les_speed(torch, h3, fine_n=512, les_n=64, stride=5, steps=40, dt_dns=2e-3):
             "max_rel_err": None if rec is None else rec["max_rel_err"],
             "max_rel_err_k_ge_3": None if rec is None else rec["max_rel_err_k_ge_3"],
             "dns_record_to_record_max": None if rec is None else rec["dns_record_to_record_max"],
+            "judged_window_steps": None if rec is None else rec.get("judged_window_steps"),
+            "judged_window_max_rel_err": None if rec is None else rec.get("judged_window_max_rel_err"),
             "source": None if rec is None else rec["source"]}
 
 
