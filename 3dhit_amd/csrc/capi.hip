@@ -1270,8 +1270,8 @@ int hit3d_profile_rk4(const hit3d_grid* grid, void* u_hat, double nu, double dt,
 }
 
 int hit3d_rhs_vjp(const hit3d_grid* grid, const void* u_hat, const void* v, void* ubar, double nu,
-                  const hit3d_forcing* forcing, double t, double* param_grad, void* work,
-                  size_t work_bytes, void* stream) {
+                  const hit3d_forcing* forcing, double t, int32_t dealiased, double* param_grad,
+                  void* work, size_t work_bytes, void* stream) {
   (void)t;
   H3_TRY(check_grid(grid));
   H3_TRY(check_forcing(forcing));
@@ -1286,10 +1286,10 @@ int hit3d_rhs_vjp(const hit3d_grid* grid, const void* u_hat, const void* v, void
   if (grid->precision == 4)
     return rhs_vjp_impl<float>(*grid, static_cast<const cpx<float>*>(u_hat),
                                static_cast<const cpx<float>*>(v), static_cast<cpx<float>*>(ubar),
-                               nu, forcing, work, s, param_grad);
+                               nu, forcing, work, s, param_grad, nullptr, dealiased != 0);
   return rhs_vjp_impl<double>(*grid, static_cast<const cpx<double>*>(u_hat),
                               static_cast<const cpx<double>*>(v), static_cast<cpx<double>*>(ubar),
-                              nu, forcing, work, s, param_grad);
+                              nu, forcing, work, s, param_grad, nullptr, dealiased != 0);
 }
 
 int hit3d_rk4_step_vjp(const hit3d_grid* grid, const void* u_hat, const void* ubar_next,

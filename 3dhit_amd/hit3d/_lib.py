@@ -71,7 +71,7 @@ SIGNATURES = {
     "hit3d_shell_scale": [_G, _P, _I64, _P, _I64, _P],
     "hit3d_stream_copy": [_P, _P, _SZ, _I32, _P],
     "hit3d_profile_rk4": [_G, _P, _D, _D, _I64, _F, _I32, _P, _P, _P, _SZ, _P],
-    "hit3d_rhs_vjp": [_G, _P, _P, _P, _D, _F, _D, _P, _P, _SZ, _P],
+    "hit3d_rhs_vjp": [_G, _P, _P, _P, _D, _F, _D, _I32, _P, _P, _SZ, _P],
     "hit3d_rk4_step_vjp": [_G, _P, _P, _P, _D, _D, _D, _F, _I32, _P, _P, _P, _SZ, _P],
     "hit3d_rk4_steps_record": [_G, _P, _D, _D, _I64, _D, _F, _I32, _P, _P, _SZ, _P],
     "hit3d_conv_sizes": [_C, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_SZ)],

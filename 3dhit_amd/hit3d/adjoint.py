@@ -39,17 +39,21 @@ __all__ = ["make_ns_rhs_vjp", "rk4_step_vjp", "rollout_vjp", "differentiable_rol
 def make_ns_rhs_vjp(grid):
     """v -> (d rhs / d u_hat)^* v at u_hat, for `rhs = make_ns_rhs(grid)`."""
 
-    def vjp(u_hat, v, params, t, param_grad=None):
+    def vjp(u_hat, v, params, t, param_grad=None, dealiased=None):
         """`param_grad` (device float64, 3 entries): the derivatives of
-        Re<v, rhs(u_hat)> in nu, A, eps are added to it."""
+        Re<v, rhs(u_hat)> in nu, A, eps are added to it. The pruned inverse
+        transforms are taken when u_hat carries the dealiased mark or
+        `dealiased` says so (no device read)."""
         u = check_spectral(u_hat, grid, 3)
         vv = check_spectral(v, grid, 3, "v")
         out = torch.empty_like(u)
         ws = grid.workspace(_lib.OP_RHS_VJP)
         f = params.forcing.desc()
+        if dealiased is None:
+            dealiased = getattr(u, DEALIASED_MARK, None) == u._version
         _lib.call("hit3d_rhs_vjp", grid.desc, u.data_ptr(), vv.data_ptr(), out.data_ptr(),
-                  float(params.nu), ctypes.byref(f), float(t), _pg_ptr(param_grad, u),
-                  ws.data_ptr(), ws.numel(), stream())
+                  float(params.nu), ctypes.byref(f), float(t), 1 if dealiased else 0,
+                  _pg_ptr(param_grad, u), ws.data_ptr(), ws.numel(), stream())
         return out
 
     return vjp

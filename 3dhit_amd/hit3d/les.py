@@ -198,7 +198,8 @@ class _NSRhs(torch.autograd.Function):
     def backward(ctx, g):
         (u,) = ctx.saved_tensors
         from .adjoint import make_ns_rhs_vjp
-        return make_ns_rhs_vjp(ctx.rhs.grid)(u, g.contiguous(), ctx.params, ctx.t), None, None, None
+        return make_ns_rhs_vjp(ctx.rhs.grid)(u, g.contiguous(), ctx.params, ctx.t, dealiased=True), \
+            None, None, None
 
 
 # ---------------------------------------------------------------------------

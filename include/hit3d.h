@@ -183,15 +183,18 @@ int hit3d_shell_scale(const hit3d_grid* grid, void* u_hat, int64_t ncomp, const 
    written out by hand. Gradient convention: for a real J, dJ = Re sum over
    all entries of conj(ubar) * du_hat (jax's cotangent is conj(ubar)).
    Includes the derivative of the state-dependent forcing coefficients
-   (BAND, LUNDGREN). Dense transforms; ubar must not alias u_hat or v.
+   (BAND, LUNDGREN). `dealiased` = 1 promises that u_hat has no content
+   outside the 2/3 box (as for hit3d_rhs): u, K u and the projected cotangent
+   then take the pruned inverse transforms (ubar is formed at every mode
+   either way). ubar must not alias u_hat or v.
    If `param_grad` (device, 3 doubles) is not NULL, the derivatives of
    Re sum conj(v) rhs(u_hat) with respect to the physical scalars are ADDED
    to it: [d/dnu, d/dA (LINEAR forcing; else 0), d/deps (BAND; else 0)] —
    the gradients jax.grad takes through NSParams' leaves
    (navierstokes.py:40-59, tests/test_grad.py:43-45, 59-74). */
 int hit3d_rhs_vjp(const hit3d_grid* grid, const void* u_hat, const void* v, void* ubar, double nu,
-                  const hit3d_forcing* forcing, double t, double* param_grad, void* work,
-                  size_t work_bytes, void* stream);
+                  const hit3d_forcing* forcing, double t, int32_t dealiased, double* param_grad,
+                  void* work, size_t work_bytes, void* stream);
 
 /* Adjoint of one classic RK4 step (integrators.py:32-38): from the step's
    start state u_hat and the cotangent ubar_next of its result, writes the
