@@ -160,6 +160,7 @@ struct Fft2 {
   static constexpr int R = 1 << ((LOG + 1) / 2);
   static constexpr int Q = N / R;
   static constexpr int M = R / Q;  // 1 or 2
+  static constexpr int NPTS = N;
   static constexpr int kSlots = R * (Q + 1);
   static_assert(R * Q == N && (M == 1 || M == 2), "N must be a power of two");
 
@@ -176,20 +177,20 @@ struct Fft2 {
     for (int k1 = 0; k1 < R; ++k1) tw[k1] = twiddle<N, -1, T>(q * k1);
   }
 
-  template <int DIR, class Sync = BlockSync, class T, class Slot>
-  __device__ static __forceinline__ void run(cpx<T>* v, int q, Slot slot, const cpx<T>* tw) {
+  template <int DIR, class Sync = BlockSync, class T, class Slot, class TW>
+  __device__ static __forceinline__ void run(cpx<T>* v, int q, Slot slot, const TW& tw) {
     dft<R, DIR>(v);
 #pragma unroll
     for (int k1 = 1; k1 < R; ++k1) v[k1] = apply_tw<T, DIR>(v[k1], tw[k1]);
     Sync::sync();
 #pragma unroll
-    for (int k1 = 0; k1 < R; ++k1) *slot(k1 * (Q + 1) + q) = v[k1];
+    for (int k1 = 0; k1 < R; ++k1) *slot.at(q, k1 * (Q + 1)) = v[k1];
     Sync::sync();
 #pragma unroll
     for (int m = 0; m < M; ++m) {
       cpx<T> u[Q];
 #pragma unroll
-      for (int qq = 0; qq < Q; ++qq) u[qq] = *slot((q + Q * m) * (Q + 1) + qq);
+      for (int qq = 0; qq < Q; ++qq) u[qq] = *slot.at(q * (Q + 1), Q * m * (Q + 1) + qq);
       dft<Q, DIR>(u);
 #pragma unroll
       for (int k2 = 0; k2 < Q; ++k2) v[m * Q + k2] = u[k2];
@@ -200,6 +201,7 @@ struct Fft2 {
 template <int N, int QT = 64>
 struct Fft3 {
   static constexpr int Q = QT, R = N / QT;
+  static constexpr int NPTS = N;
   static constexpr int Q2 = QT / R;       // points of the middle DFT (over q2)
   static constexpr int M1 = R / Q2;       // middle DFTs per thread
   static constexpr int E1 = QT + 1;       // k1 stride of the first exchange
@@ -215,14 +217,14 @@ struct Fft3 {
     for (int k1 = 0; k1 < R; ++k1) tw[k1] = twiddle<N, -1, T>(q * k1);
   }
 
-  template <int DIR, class Sync = BlockSync, class T, class Slot>
-  __device__ static __forceinline__ void run(cpx<T>* v, int q, Slot slot, const cpx<T>* tw) {
+  template <int DIR, class Sync = BlockSync, class T, class Slot, class TW>
+  __device__ static __forceinline__ void run(cpx<T>* v, int q, Slot slot, const TW& tw) {
     dft<R, DIR>(v);
 #pragma unroll
     for (int k1 = 1; k1 < R; ++k1) v[k1] = apply_tw<T, DIR>(v[k1], tw[k1]);
     Sync::sync();
 #pragma unroll
-    for (int k1 = 0; k1 < R; ++k1) *slot(k1 * E1 + q) = v[k1];
+    for (int k1 = 0; k1 < R; ++k1) *slot.at(q, k1 * E1) = v[k1];
     Sync::sync();
     // middle DFTs: this thread takes k1 = q % R and q1 = q / R + Q2 m
     const int k1 = q % R, g = q / R;
@@ -230,7 +232,7 @@ struct Fft3 {
 #pragma unroll
     for (int m = 0; m < M1; ++m)
 #pragma unroll
-      for (int q2 = 0; q2 < Q2; ++q2) u[m][q2] = *slot(k1 * E1 + (g + Q2 * m) + R * q2);
+      for (int q2 = 0; q2 < Q2; ++q2) u[m][q2] = *slot.at(k1 * E1 + g, Q2 * m + R * q2);
     Sync::sync();
 #pragma unroll
     for (int m = 0; m < M1; ++m) {
@@ -239,15 +241,31 @@ struct Fft3 {
 #pragma unroll
       for (int j = 0; j < Q2; ++j) {
         const cpx<T> w = j ? apply_tw<T, DIR>(u[m][j], twiddle<QT, -1, T>(q1 * j)) : u[m][j];
-        *slot((k1 + R * j) * (R + 1) + q1) = w;
+        *slot.at(k1 * (R + 1) + g, R * j * (R + 1) + Q2 * m) = w;
       }
     }
     Sync::sync();
     // last DFT over q1: thread q = k1' + R j' holds X[q + Q s]
 #pragma unroll
-    for (int q1 = 0; q1 < R; ++q1) v[q1] = *slot(q * (R + 1) + q1);
+    for (int q1 = 0; q1 < R; ++q1) v[q1] = *slot.at(q * (R + 1), q1);
     dft<R, DIR>(v);
   }
+};
+
+// First-stage twiddles w_N^{q k1} of thread q: held in VGPRs (loaded once
+// per kernel), or, for kernels that cannot spare the registers, read from the
+// table at each use (the table stays in the vector L1).
+template <class F, class T, bool LAZY = false>
+struct Twiddles {
+  cpx<T> w[F::R];
+  __device__ __forceinline__ explicit Twiddles(int q) { F::load_tw(w, q); }
+  __device__ __forceinline__ cpx<T> operator[](int k1) const { return w[k1]; }
+};
+template <class F, class T>
+struct Twiddles<F, T, true> {
+  int q;
+  __device__ __forceinline__ explicit Twiddles(int q_) : q(q_) {}
+  __device__ __forceinline__ cpx<T> operator[](int k1) const { return twiddle<F::NPTS, -1, T>(q * k1); }
 };
 
 // strided passes: Fft2 up to 256 points (16 threads per column keeps 16
@@ -286,11 +304,17 @@ template <int TC, class T>
 struct SlotStrided {
   cpx<T>* base;  // already offset by column c
   __device__ __forceinline__ cpx<T>* operator()(int i) const { return base + i * TC; }
+  // slot lane + c, c a compile-time constant after unrolling: the lane part
+  // is one address register and c * TC * sizeof(cpx) the instruction's
+  // immediate offset (a constant folded inside the product would instead be
+  // materialised in a register per slot)
+  __device__ __forceinline__ cpx<T>* at(int lane, int c) const { return (base + lane * TC) + c * TC; }
 };
 template <class T>
 struct SlotPencil {
   cpx<T>* base;  // already offset by pencil p
   __device__ __forceinline__ cpx<T>* operator()(int i) const { return base + i; }
+  __device__ __forceinline__ cpx<T>* at(int lane, int c) const { return (base + lane) + c; }
 };
 
 // Strided passes tile columns across workgroups, and neighbouring tiles share

@@ -199,6 +199,143 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
 }
 
 // ---------------------------------------------------------------------------
+// Register-lean forms of the 1024-point float x-inverse and plain strided
+// passes. The forms above hold three or four fields' values per thread
+// (~240 VGPRs, 2 waves per SIMD) on 8-column workgroups, so every row segment
+// is half a 128-byte line and the other half belongs to another workgroup.
+// These hold at most three transforms' values per thread (two in the plain
+// pass) within 128 VGPRs, which allows 16-column workgroups of 16 waves
+// (4 waves per SIMD, 136 KB of LDS): each row segment is a whole line.
+// Addresses are a wave-uniform field base plus a 32-bit per-lane byte offset
+// (the launchers take these kernels only when every offset fits), so no
+// 64-bit per-lane address is held across the field sequence.
+// ---------------------------------------------------------------------------
+// A copy of x the compiler cannot see through (an empty asm statement): the
+// lane-dependent values derived from it (row offsets, LDS addresses, middle-
+// stage twiddles) are recomputed in each iteration of the field sequence
+// instead of being hoisted out of it and held in registers.
+__device__ __forceinline__ int opaque(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
+template <class T>
+__device__ __forceinline__ cpx<T> ld_off(const cpx<T>* base, uint32_t off) {
+  return *reinterpret_cast<const cpx<T>*>(reinterpret_cast<const char*>(base) + off);
+}
+template <class T>
+__device__ __forceinline__ void st_off(cpx<T>* base, uint32_t off, cpx<T> v) {
+  *reinterpret_cast<cpx<T>*>(reinterpret_cast<char*>(base) + off) = v;
+}
+
+template <int N, int DIR, class T>
+__global__ __launch_bounds__(LeanShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_strided_lean(StridedArgs<T> a) {
+  using S = LeanShape<N>;
+  using F = typename S::F;
+  constexpr int R = S::R, Q = S::Q, TC = S::TC;
+  constexpr uint32_t E = sizeof(cpx<T>);
+  __shared__ cpx<T> lds[S::LDS];
+  const int c = threadIdx.x % TC, q = threadIdx.x / TC;
+  const int j = xcd_tile(blockIdx.x, gridDim.x) * TC + c;
+  const bool active = j < a.J;
+  const int jj = active ? j : 0;
+  const int ao = jj / a.B, bo = jj - ao * a.B;
+  const uint32_t cin = uint32_t(int64_t(ao) * a.in_slab + bo) * E;
+  const uint32_t cout = uint32_t(int64_t(ao) * a.out_slab + bo) * E;
+  const uint32_t rs = uint32_t(a.B) * E;
+  SlotStrided<TC, T> L{lds + c};
+  const Twiddles<F, T> tw(q);
+  for (int f = 0; f < a.nf; ++f) {
+    const int qv = opaque(q);
+    const cpx<T>* src = a.in + f * a.in_fs;
+    cpx<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int ir = compact_of_row(qv + Q * r, N, a.in_k, a.in_p);
+      v[r] = (active && ir >= 0) ? ld_off(src, cin + uint32_t(ir) * rs) : czero<T>();
+    }
+    F::template run<DIR>(v, qv, L, tw);
+    cpx<T>* dst = a.out + f * a.out_fs;
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      const int o = compact_of_row(F::out_index(qv, s), N, a.out_k, a.out_p);
+      if (active && o >= 0) st_off(dst, cout + uint32_t(o) * rs, v[s]);
+    }
+  }
+}
+
+// P1, lean: three arrays (u0 stays in registers for both curl parts; the
+// twiddles come from the table).
+template <int N, class T>
+__global__ __launch_bounds__(LeanShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_x_inverse_curl_lean(
+    const cpx<T>* __restrict__ us, int64_t us_fs, cpx<T>* __restrict__ out, int64_t out_fs, Geo g,
+    FinalizeJob job) {
+  using S = LeanShape<N>;
+  using F = typename S::F;
+  constexpr int R = S::R, Q = S::Q, TC = S::TC;
+  constexpr uint32_t E = sizeof(cpx<T>);
+  if (job.partials != nullptr && blockIdx.x == 0) finalize_in_block(job);   // uniform per block
+  __shared__ cpx<T> lds[S::LDS];
+  const int c = threadIdx.x % TC, q = threadIdx.x / TC;
+  const int j = xcd_tile(blockIdx.x, gridDim.x) * TC + c;
+  const bool active = j < g.ky * g.kzs;   // padding columns (iz >= kz) carry zeros
+  const int jj = active ? j : 0;
+  const int iy = jj / g.kzs, iz = jj - iy * g.kzs;
+  const T kyv = wavenumber<T>(g.k0y, freq_of(iy, g.ky, g.py));
+  const T kzv = wavenumber<T>(g.k0z, iz);
+  const uint32_t cj = uint32_t(jj) * E, rs = uint32_t(g.jp) * E;
+  SlotStrided<TC, T> L{lds + c};
+  const Twiddles<F, T, true> tw(q);
+  // every step takes its own opaque copy of q (see opaque())
+  auto row = [&](int qv, int r) { return compact_of_row(qv + Q * r, N, g.kx, g.px); };
+  auto kx = [&](int qv, int r) {
+    const int ir = row(qv, r);
+    return wavenumber<T>(g.k0x, ir >= 0 ? freq_of(ir, g.kx, g.px) : 0);
+  };
+  auto load = [&](int f, cpx<T>* v) {
+    const int qv = opaque(q);
+    const cpx<T>* src = us + f * us_fs;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int ir = row(qv, r);
+      v[r] = (active && ir >= 0) ? ld_off(src, cj + uint32_t(ir) * rs) : czero<T>();
+    }
+  };
+  auto store = [&](int f, const cpx<T>* v) {
+    if (!active) return;
+    const int qv = opaque(q);
+    cpx<T>* dst = out + f * out_fs;
+#pragma unroll
+    for (int s = 0; s < R; ++s) st_off(dst, cj + uint32_t(F::out_index(qv, s)) * rs, v[s]);
+  };
+  auto fft = [&](cpx<T>* v) { const int qv = opaque(q); F::template run<+1>(v, qv, L, Twiddles<F, T, true>(qv)); };
+  cpx<T> a[R], b[R], w[R];
+  load(0, a);
+  load(2, b);
+  {
+    const int qv = opaque(q);
+#pragma unroll
+    for (int r = 0; r < R; ++r) w[r] = mul_i(kzv * a[r] - kx(qv, r) * b[r]);   // w1
+  }
+  fft(w);
+  store(3, w);
+  fft(b);
+  store(2, b);
+  load(1, b);
+  {
+    const int qv = opaque(q);
+#pragma unroll
+    for (int r = 0; r < R; ++r) w[r] = mul_i(kx(qv, r) * b[r] - kyv * a[r]);   // w2
+  }
+  fft(w);
+  store(4, w);
+  fft(b);
+  store(1, b);
+  fft(a);
+  store(0, a);
+}
+
+// ---------------------------------------------------------------------------
 // z-axis helpers: Hermitian extension of two half spectra U, W into the
 // spectrum of u + i w (c2r semantics: only Re of the kz = 0 and Nyquist bins).
 // ---------------------------------------------------------------------------
@@ -684,9 +821,28 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
 
 static inline unsigned ceil_div(int64_t a, int64_t b) { return unsigned((a + b - 1) / b); }
 
+// The lean kernels address with 32-bit byte offsets inside a field.
+static inline bool fits_u32(int64_t max_elem, size_t esize) {
+  return max_elem >= 0 && uint64_t(max_elem + 1) * esize <= (uint64_t(1) << 32);
+}
+template <class T>
+static constexpr bool lean_type() { return sizeof(T) == 4 && H3_LEAN1024; }
+
 template <class T>
 hipError_t launch_strided(int N, int dir, const StridedArgs<T>& a, hipStream_t s) {
   if (a.J <= 0 || a.nf <= 0) return hipSuccess;
+  if constexpr (lean_type<T>()) {
+    const int64_t ao = (a.J - 1) / a.B;
+    const bool fit = fits_u32(ao * a.in_slab + (a.B - 1) + int64_t(N - 1) * a.B, sizeof(cpx<T>)) &&
+                     fits_u32(ao * a.out_slab + (a.B - 1) + int64_t(N - 1) * a.B, sizeof(cpx<T>));
+    if (N == 1024 && fit) {
+      constexpr int NN = 1024;
+      const unsigned nb = ceil_div(a.J, LeanShape<NN>::TC);
+      if (dir < 0) hipLaunchKernelGGL((k_strided_lean<NN, -1, T>), dim3(nb), dim3(LeanShape<NN>::THREADS), 0, s, a);
+      else hipLaunchKernelGGL((k_strided_lean<NN, +1, T>), dim3(nb), dim3(LeanShape<NN>::THREADS), 0, s, a);
+      return hipGetLastError();
+    }
+  }
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(a.J, StridedShape<NN>::TC);
     if (dir < 0) hipLaunchKernelGGL((k_strided<NN, -1, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a);
@@ -701,6 +857,16 @@ hipError_t launch_x_inverse_curl(int N, const cpx<T>* us, int64_t us_fs, cpx<T>*
                                  const FinalizeJob* job) {
   FinalizeJob j{};
   if (job) j = *job;
+  if constexpr (lean_type<T>()) {
+    if (N == 1024 && fits_u32(int64_t(g.kx) * g.jp, sizeof(cpx<T>)) &&
+        fits_u32(int64_t(g.nx) * g.jp, sizeof(cpx<T>))) {
+      constexpr int NN = 1024;
+      const unsigned nb = ceil_div(int64_t(g.ky) * g.kzs, LeanShape<NN>::TC);
+      hipLaunchKernelGGL((k_x_inverse_curl_lean<NN, T>), dim3(nb), dim3(LeanShape<NN>::THREADS), 0, s, us, us_fs,
+                         out, out_fs, g, j);
+      return hipGetLastError();
+    }
+  }
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(int64_t(g.ky) * g.kzs, StridedShape<NN>::TC);
     hipLaunchKernelGGL((k_x_inverse_curl<NN, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, us, us_fs, out,

@@ -9,6 +9,12 @@ namespace h3 {
 constexpr int kThreads = 256;   // strided passes: 4 waves
 constexpr int kZThreads = 64;   // z passes: one wave, so the pencil exchanges need no cross-wave barrier
 
+// 1 selects the register-lean 1024-point float x-inverse and plain strided
+// passes (passes.hip) wherever their 32-bit offsets fit
+#ifndef H3_LEAN1024
+#define H3_LEAN1024 1
+#endif
+
 // Minimum waves per SIMD the compiler must allow: the 256-point float
 // strided passes run 8-wave workgroups, which fit twice per CU only at 4
 // waves per SIMD (<= 128 VGPRs); double needs twice the registers.
@@ -22,6 +28,15 @@ struct StridedShape {
   using F = FftFor<N>;
   static constexpr int R = F::R, Q = F::Q;
   static constexpr int TC = N == 256 ? 16 : ((kThreads / Q) > 8 ? kThreads / Q : 8);  // columns per workgroup
+  static constexpr int THREADS = TC * Q;
+  static constexpr int LDS = TC * F::kSlots;
+};
+// Workgroup shape of the register-lean 1024-point passes (passes.hip): 16
+// columns (one 128-byte line of complex64 per row) of Q threads.
+template <int N>
+struct LeanShape {
+  using F = FftFor<N>;
+  static constexpr int R = F::R, Q = F::Q, TC = 16;
   static constexpr int THREADS = TC * Q;
   static constexpr int LDS = TC * F::kSlots;
 };
