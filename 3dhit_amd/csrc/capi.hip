@@ -295,7 +295,7 @@ int rhs_passes(const hit3d_grid& gd, const Geo& g, const cpx<T>* us, cpx<T>* buf
                                T(1.0 / (n * n)), s));
   mark(tm, kP3);
   h3::StridedArgs<T> y{};
-  y.in = bufB;
+  y.in = bufB + h3::kZOut * b_fs;
   y.out = bufA;
   y.in_fs = b_fs;
   y.out_fs = a_fs;

@@ -358,6 +358,30 @@ __device__ __forceinline__ cpx<T> herm_pair(const cpx<T>* U, const cpx<T>* W, in
   return mirror ? cpx<T>{ux + wy, wx - uy} : cpx<T>{ux - wy, uy + wx};
 }
 
+// The same extension with the index work done once per element position and
+// shared by several field pairs.
+struct HermIdx {
+  int at;          // stored source index (0 when not stored)
+  bool mirror, stored, cplx;  // cplx: stored and not a real (kz = 0 / Nyquist) mode
+};
+template <int N>
+__device__ __forceinline__ HermIdx herm_idx(int k, int kz) {
+  HermIdx h;
+  h.mirror = 2 * k > N;
+  const int src = h.mirror ? N - k : k;
+  h.stored = src < kz;
+  h.at = h.stored ? src : 0;
+  h.cplx = h.stored && !(src == 0 || 2 * src == N);
+  return h;
+}
+template <class T>
+__device__ __forceinline__ cpx<T> herm_pair_at(const cpx<T>* U, const cpx<T>* W, const HermIdx& h) {
+  const cpx<T> u = U[h.at], w = W[h.at];
+  const T uy = h.cplx ? u.y : T(0), wy = h.cplx ? w.y : T(0);
+  const T ux = h.stored ? u.x : T(0), wx = h.stored ? w.x : T(0);
+  return h.mirror ? cpx<T>{ux + wy, wx - uy} : cpx<T>{ux - wy, uy + wx};
+}
+
 // The 2/3 truncation of the pruned layout (kz = N/3 + 1 stored modes) leaves
 // the modes kz <= k <= N - kz of the Hermitian-extended pencil zero. Bit r of
 // kEntry: entry slot r (k = q + Q r) is in that band for every thread q, so it
@@ -399,10 +423,19 @@ __device__ __forceinline__ void split_pair(cpx<T> ak, cpx<T> bk, cpx<T>& X, cpx<
 
 // ---------------------------------------------------------------------------
 // P3: per z pencil, c2r of (u_j + i w_j) for j = 0..2, u x w scaled by
-// `scale` (= 1/Ntot^2, a power of two), then r2c; output (nl0, nl1, nl2)
-// overwrites fields 0..2 of the same pencils. Pencils are kzs apart; their
-// kz modes are followed by kzs - kz padding slots, written as zeros so that
-// every line of the output is written whole.
+// `scale` (= 1/Ntot^2, a power of two), then r2c. Pencils are kzs apart;
+// their kz modes are followed by kzs - kz padding slots, written as zeros so
+// that every line of the output is written whole. Outputs (nl0, nl1, nl2) go
+// to fields kZOut, kZOut+1, kZOut+2 = 2, 3, 4, in place over inputs the
+// pencil has consumed by then.
+// Up to 512 points: z0 and z1 are transformed first, and nl2 = u0 w1 - u1 w0,
+// which needs only them, is transformed and stored (over w1, field 4) before
+// z2 is loaded, so at most two fields' physical values and one transform are
+// live at a time (256 points: 1303 -> 1260 VALU instructions per wave, 115.8
+// -> 102.9 us per launch at 256^3). At 1024 points all three fields are
+// loaded before the first transform and both products formed after the third
+// (the one-pencil waves there gain more from the early loads: 7.4 ms per
+// launch at 1024^3 against 9.6 ms in the order above).
 // ---------------------------------------------------------------------------
 template <int N, class T, bool PRUNED>
 __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_fused(cpx<T>* buf, int64_t fs, int64_t npencils,
@@ -410,6 +443,7 @@ __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_fused(cpx<T>* buf
   using B = PencilBlock<N>;
   using F = typename B::F;
   static_assert(B::THREADS == 64, "WaveSync exchanges need one-wave blocks");
+  static_assert(kZOut == 2, "the in-place order below writes fields 2, 3, 4");
   constexpr int R = B::R, Q = B::Q, PW = B::PW;
   __shared__ cpx<T> lds[B::LDS];
   const int q = threadIdx.x % Q, p = threadIdx.x / Q;
@@ -421,71 +455,95 @@ __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_fused(cpx<T>* buf
   const int kzl = active ? kz : 0;  // inactive pencils load nothing
   cpx<T> tw[R];
   F::load_tw(tw, q);
-  cpx<T> z0[R], z1[R], z2[R];
-  cpx<T>* zz[3] = {z0, z1, z2};
   using Z = ZeroBand<F, N>;
   constexpr unsigned kZeroIn = PRUNED ? Z::kEntry : 0u, kUnused = PRUNED ? Z::kExit : 0u;
-  if constexpr (N >= 512) {
-    // all three fields' loads ahead of the first transform (measured at
-    // 1024^3: 7.68 -> 7.27 ms; neutral at 256^3, where the interleaved
-    // order below is kept)
+  HermIdx hi[R];
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+  for (int r = 0; r < R; ++r) hi[r] = herm_idx<N>(q + Q * r, kzl);
+  auto load = [&](int c, cpx<T>* z) {
 #pragma unroll
-      for (int r = 0; r < R; ++r)
-        zz[c][r] = ((kZeroIn >> r) & 1u) ? czero<T>()
-                                         : herm_pair<N, T>(buf + c * fs + base, buf + (3 + c) * fs + base,
-                                                           q + Q * r, kzl);
+    for (int r = 0; r < R; ++r)
+      z[r] = ((kZeroIn >> r) & 1u) ? czero<T>()
+                                   : herm_pair_at<T>(buf + c * fs + base, buf + (3 + c) * fs + base, hi[r]);
+  };
+  // nl2 (real) at exit slot s of the inverse transforms, placed at the entry
+  // slot of the forward transform; then its retained modes and the padding
+  auto nl2_of = [&](const cpx<T>* z0, const cpx<T>* z1, cpx<T>* Bv) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) F::template run<+1, WaveSync>(zz[c], q, L, tw);
-  } else {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-        zz[c][r] = ((kZeroIn >> r) & 1u) ? czero<T>()
-                                         : herm_pair<N, T>(buf + c * fs + base, buf + (3 + c) * fs + base,
-                                                           q + Q * r, kzl);
-      F::template run<+1, WaveSync>(zz[c], q, L, tw);
-    }
-  }
-  // physical space: z_c = u_c + i w_c at position out_index(q, s)
-  cpx<T> A[R], Bv[R];
-#pragma unroll
-  for (int s = 0; s < R; ++s) {
-    const T u0 = z0[s].x, u1 = z1[s].x, u2 = z2[s].x;
-    const T w0 = z0[s].y, w1 = z1[s].y, w2 = z2[s].y;
-    const int r = F::out_slot_as_input(s);
-    A[r] = {(u1 * w2 - u2 * w1) * scale, (u2 * w0 - u0 * w2) * scale};
-    Bv[r] = {(u0 * w1 - u1 * w0) * scale, T(0)};
-  }
-  F::template run<-1, WaveSync>(Bv, q, L, tw);
-  F::template run<-1, WaveSync>(A, q, L, tw);
-  if (active) {
+    for (int s = 0; s < R; ++s)
+      Bv[F::out_slot_as_input(s)] = {(z0[s].x * z1[s].y - z1[s].x * z0[s].y) * scale, T(0)};
+  };
+  auto store_nl2 = [&](const cpx<T>* Bv) {
+    if (!active) return;
+    cpx<T>* dst = buf + (kZOut + 2) * fs + base;
 #pragma unroll
     for (int s = 0; s < R; ++s) {
       if ((kUnused >> s) & 1u) continue;
       const int k = F::out_index(q, s);
-      if (k < kz) buf[2 * fs + base + k] = Bv[s];
+      if (k < kz) dst[k] = Bv[s];
     }
-    for (int k = kz + q; k < kzs; k += Q) buf[2 * fs + base + k] = czero<T>();
+    for (int k = kz + q; k < kzs; k += Q) dst[k] = czero<T>();
+  };
+  // physical space: z_c = u_c + i w_c at position out_index(q, s)
+  cpx<T> A[R];
+  if constexpr (N >= 1024) {
+    cpx<T> z0[R], z1[R], z2[R];
+    load(0, z0);
+    load(1, z1);
+    load(2, z2);
+    F::template run<+1, WaveSync>(z0, q, L, tw);
+    F::template run<+1, WaveSync>(z1, q, L, tw);
+    F::template run<+1, WaveSync>(z2, q, L, tw);
+    cpx<T> Bv[R];
+    nl2_of(z0, z1, Bv);
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      const T u0 = z0[s].x, u1 = z1[s].x, u2 = z2[s].x;
+      const T w0 = z0[s].y, w1 = z1[s].y, w2 = z2[s].y;
+      A[F::out_slot_as_input(s)] = {(u1 * w2 - u2 * w1) * scale, (u2 * w0 - u0 * w2) * scale};
+    }
+    F::template run<-1, WaveSync>(Bv, q, L, tw);
+    store_nl2(Bv);
+  } else {
+    cpx<T> z0[R], z1[R];
+    load(0, z0);
+    F::template run<+1, WaveSync>(z0, q, L, tw);
+    load(1, z1);
+    F::template run<+1, WaveSync>(z1, q, L, tw);
+    {
+      cpx<T> Bv[R];
+      nl2_of(z0, z1, Bv);
+      F::template run<-1, WaveSync>(Bv, q, L, tw);
+      store_nl2(Bv);
+    }
+    cpx<T> z2[R];
+    load(2, z2);
+    F::template run<+1, WaveSync>(z2, q, L, tw);
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      const T u0 = z0[s].x, u1 = z1[s].x, u2 = z2[s].x;
+      const T w0 = z0[s].y, w1 = z1[s].y, w2 = z2[s].y;
+      A[F::out_slot_as_input(s)] = {(u1 * w2 - u2 * w1) * scale, (u2 * w0 - u0 * w2) * scale};
+    }
   }
+  F::template run<-1, WaveSync>(A, q, L, tw);
   WaveSync::sync();
 #pragma unroll
   for (int s = 0; s < R; ++s)
     if (!((kUnused >> s) & 1u)) lp[F::out_index(q, s)] = A[s];
   WaveSync::sync();
   if (active) {
+    cpx<T>* d0 = buf + kZOut * fs + base;
+    cpx<T>* d1 = buf + (kZOut + 1) * fs + base;
     for (int k = q; k < kzs; k += Q) {
       cpx<T> X = czero<T>(), Y = czero<T>();
       if (k < kz) split_pair(lp[k], lp[(N - k) & (N - 1)], X, Y);
-      buf[0 * fs + base + k] = X;
-      buf[1 * fs + base + k] = Y;
+      d0[k] = X;
+      d1[k] = Y;
     }
   }
 }
 
-// Standalone z c2r of nf fields, two per complex transform.
 template <int N, class T>
 __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_c2r(const cpx<T>* __restrict__ in, int64_t in_fs,
                                                    T* __restrict__ out, int64_t out_fs, int nf,

@@ -8,6 +8,9 @@ namespace h3 {
 
 constexpr int kThreads = 256;   // strided passes: 4 waves
 constexpr int kZThreads = 64;   // z passes: one wave, so the pencil exchanges need no cross-wave barrier
+// first of the three pencil-buffer fields the fused z pass writes (u x w
+// transformed, in place over consumed inputs)
+constexpr int kZOut = 2;
 
 // 1 selects the register-lean 1024-point float x-inverse and plain strided
 // passes (passes.hip) wherever their 32-bit offsets fit
