@@ -431,10 +431,10 @@ __device__ __forceinline__ void split_pair(cpx<T> ak, cpx<T> bk, cpx<T>& X, cpx<
 // Up to 512 points: z0 and z1 are transformed first, and nl2 = u0 w1 - u1 w0,
 // which needs only them, is transformed and stored (over w1, field 4) before
 // z2 is loaded, so at most two fields' physical values and one transform are
-// live at a time (256 points: 1303 -> 1260 VALU instructions per wave, 115.8
-// -> 102.9 us per launch at 256^3). At 1024 points all three fields are
+// live at a time (256 points: 1303 -> 1260 VALU instructions per wave, about
+// 111 -> 106 us per launch at 256^3). At 1024 points all three fields are
 // loaded before the first transform and both products formed after the third
-// (the one-pencil waves there gain more from the early loads: 7.4 ms per
+// (the one-pencil waves there gain more from the early loads: 7.3 ms per
 // launch at 1024^3 against 9.6 ms in the order above).
 // ---------------------------------------------------------------------------
 template <int N, class T, bool PRUNED>
