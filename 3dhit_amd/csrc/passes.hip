@@ -10,6 +10,8 @@
 // z c2r / r2c kernels.
 #include "passes.hpp"
 
+#include <type_traits>
+
 namespace h3 {
 
 // ---------------------------------------------------------------------------
@@ -228,11 +230,48 @@ __device__ __forceinline__ void st_off(cpx<T>* base, uint32_t off, cpx<T> v) {
   *reinterpret_cast<cpx<T>*>(reinterpret_cast<char*>(base) + off) = v;
 }
 
-template <int N, int DIR, class T>
+// Rows a pruned axis never stores: the 2/3 box keeps K = 2(N/3) + 1 rows,
+// the P = (K + 1)/2 non-negative frequencies first, so rows P .. N-(K-P)-1
+// are absent. Bit r of kIn: entry slot r (row q + Q r) is absent for every
+// thread q, so its value is a compile-time zero; bit s of kOut: exit slot s
+// (row out_index(q, s)) likewise, so its value is never stored or used. At
+// 1024 points 4 of a thread's 16 slots are absent either way, which frees
+// the registers the lean kernels need.
+template <class F, int N>
+struct PrunedRows {
+  static constexpr int K = 2 * (N / 3) + 1, P = (K + 1) / 2;
+  static constexpr bool absent(int row) { return row >= P && row < N - (K - P); }
+  static constexpr unsigned in_mask() {
+    unsigned m = 0;
+    for (int r = 0; r < F::R; ++r) {
+      bool all = true;
+      for (int q = 0; q < F::Q; ++q) all = all && absent(q + F::Q * r);
+      if (all) m |= 1u << r;
+    }
+    return m;
+  }
+  static constexpr unsigned out_mask() {
+    unsigned m = 0;
+    for (int s = 0; s < F::R; ++s) {
+      bool all = true;
+      for (int q = 0; q < F::Q; ++q) all = all && absent(F::out_index(q, s));
+      if (all) m |= 1u << s;
+    }
+    return m;
+  }
+  static constexpr unsigned kIn = in_mask(), kOut = out_mask();
+};
+
+// PIN / POUT: the input / output rows are the pruned layout's (compact index
+// = row of the retained set, as compact_of_row maps them), so the absent
+// slots above are compile-time.
+template <int N, int DIR, class T, bool PIN, bool POUT>
 __global__ __launch_bounds__(LeanShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_strided_lean(StridedArgs<T> a) {
   using S = LeanShape<N>;
   using F = typename S::F;
+  using PR = PrunedRows<F, N>;
   constexpr int R = S::R, Q = S::Q, TC = S::TC;
+  constexpr unsigned kZeroIn = PIN ? PR::kIn : 0u, kDeadOut = POUT ? PR::kOut : 0u;
   constexpr uint32_t E = sizeof(cpx<T>);
   __shared__ cpx<T> lds[S::LDS];
   const int c = threadIdx.x % TC, q = threadIdx.x / TC;
@@ -252,12 +291,13 @@ __global__ __launch_bounds__(LeanShape<N>::THREADS) __attribute__((amdgpu_waves_
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int ir = compact_of_row(qv + Q * r, N, a.in_k, a.in_p);
-      v[r] = (active && ir >= 0) ? ld_off(src, cin + uint32_t(ir) * rs) : czero<T>();
+      v[r] = ((kZeroIn >> r) & 1u) || !(active && ir >= 0) ? czero<T>() : ld_off(src, cin + uint32_t(ir) * rs);
     }
     F::template run<DIR>(v, qv, L, tw);
     cpx<T>* dst = a.out + f * a.out_fs;
 #pragma unroll
     for (int s = 0; s < R; ++s) {
+      if ((kDeadOut >> s) & 1u) continue;
       const int o = compact_of_row(F::out_index(qv, s), N, a.out_k, a.out_p);
       if (active && o >= 0) st_off(dst, cout + uint32_t(o) * rs, v[s]);
     }
@@ -266,13 +306,15 @@ __global__ __launch_bounds__(LeanShape<N>::THREADS) __attribute__((amdgpu_waves_
 
 // P1, lean: three arrays (u0 stays in registers for both curl parts; the
 // twiddles come from the table).
-template <int N, class T>
+template <int N, class T, bool PIN>
 __global__ __launch_bounds__(LeanShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_x_inverse_curl_lean(
     const cpx<T>* __restrict__ us, int64_t us_fs, cpx<T>* __restrict__ out, int64_t out_fs, Geo g,
     FinalizeJob job) {
   using S = LeanShape<N>;
   using F = typename S::F;
+  using PR = PrunedRows<F, N>;
   constexpr int R = S::R, Q = S::Q, TC = S::TC;
+  constexpr unsigned kZeroIn = PIN ? PR::kIn : 0u;
   constexpr uint32_t E = sizeof(cpx<T>);
   if (job.partials != nullptr && blockIdx.x == 0) finalize_in_block(job);   // uniform per block
   __shared__ cpx<T> lds[S::LDS];
@@ -285,7 +327,6 @@ __global__ __launch_bounds__(LeanShape<N>::THREADS) __attribute__((amdgpu_waves_
   const T kzv = wavenumber<T>(g.k0z, iz);
   const uint32_t cj = uint32_t(jj) * E, rs = uint32_t(g.jp) * E;
   SlotStrided<TC, T> L{lds + c};
-  const Twiddles<F, T, true> tw(q);
   // every step takes its own opaque copy of q (see opaque())
   auto row = [&](int qv, int r) { return compact_of_row(qv + Q * r, N, g.kx, g.px); };
   auto kx = [&](int qv, int r) {
@@ -298,7 +339,7 @@ __global__ __launch_bounds__(LeanShape<N>::THREADS) __attribute__((amdgpu_waves_
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int ir = row(qv, r);
-      v[r] = (active && ir >= 0) ? ld_off(src, cj + uint32_t(ir) * rs) : czero<T>();
+      v[r] = ((kZeroIn >> r) & 1u) || !(active && ir >= 0) ? czero<T>() : ld_off(src, cj + uint32_t(ir) * rs);
     }
   };
   auto store = [&](int f, const cpx<T>* v) {
@@ -333,6 +374,75 @@ __global__ __launch_bounds__(LeanShape<N>::THREADS) __attribute__((amdgpu_waves_
   store(1, b);
   fft(a);
   store(0, a);
+}
+
+// P2, lean: three arrays, each field loaded one transform ahead.
+template <int N, class T, bool PIN>
+__global__ __launch_bounds__(LeanShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_y_inverse_curl_lean(
+    const cpx<T>* __restrict__ in, int64_t in_fs, cpx<T>* __restrict__ out, int64_t out_fs, Geo g) {
+  using S = LeanShape<N>;
+  using F = typename S::F;
+  using PR = PrunedRows<F, N>;
+  constexpr int R = S::R, Q = S::Q, TC = S::TC;
+  constexpr unsigned kZeroIn = PIN ? PR::kIn : 0u;
+  constexpr uint32_t E = sizeof(cpx<T>);
+  __shared__ cpx<T> lds[S::LDS];
+  const int c = threadIdx.x % TC, q = threadIdx.x / TC;
+  const int j = xcd_tile(blockIdx.x, gridDim.x) * TC + c;
+  const bool active = j < g.nx * g.kzs;   // padding columns (iz >= kz) carry zeros
+  const int jj = active ? j : 0;
+  const int x = jj / g.kzs, iz = jj - x * g.kzs;
+  const T kzv = wavenumber<T>(g.k0z, iz);
+  const uint32_t cin = uint32_t(int64_t(x) * g.jp + iz) * E;
+  const uint32_t cout = uint32_t(int64_t(x) * g.ny * g.kzs + iz) * E;
+  const uint32_t rs = uint32_t(g.kzs) * E;
+  SlotStrided<TC, T> L{lds + c};
+  // every step takes its own opaque copy of q (see opaque())
+  auto row = [&](int qv, int r) { return compact_of_row(qv + Q * r, N, g.ky, g.py); };
+  auto load = [&](int f, cpx<T>* v) {
+    const int qv = opaque(q);
+    const cpx<T>* src = in + f * in_fs;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int ir = row(qv, r);
+      v[r] = ((kZeroIn >> r) & 1u) || !(active && ir >= 0) ? czero<T>() : ld_off(src, cin + uint32_t(ir) * rs);
+    }
+  };
+  auto store = [&](int f, const cpx<T>* v) {
+    if (!active) return;
+    const int qv = opaque(q);
+    cpx<T>* dst = out + f * out_fs;
+#pragma unroll
+    for (int s = 0; s < R; ++s) st_off(dst, cout + uint32_t(F::out_index(qv, s)) * rs, v[s]);
+  };
+  auto fft = [&](cpx<T>* v) { const int qv = opaque(q); F::template run<+1>(v, qv, L, Twiddles<F, T, true>(qv)); };
+  cpx<T> a[R], b[R], w[R];
+  load(1, a);
+  load(2, b);
+  {
+    const int qv = opaque(q);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int ir = row(qv, r);
+      const T kyv = wavenumber<T>(g.k0y, ir >= 0 ? freq_of(ir, g.ky, g.py) : 0);
+      w[r] = ((kZeroIn >> r) & 1u) ? czero<T>() : mul_i(kyv * b[r] - kzv * a[r]);   // w0
+    }
+  }
+  fft(w);
+  store(3, w);
+  load(0, w);
+  fft(b);
+  store(2, b);
+  load(3, b);
+  fft(a);
+  store(1, a);
+  load(4, a);
+  fft(w);
+  store(0, w);
+  fft(b);
+  store(4, b);
+  fft(a);
+  store(5, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -732,11 +842,21 @@ __device__ __forceinline__ T exp_neg(T x) {
 //   du = P[mask (NL + f)] - nu k2 us          (navierstokes.py:89-95)
 // and the RK4 stage update (integrators.py:34-38).
 // ---------------------------------------------------------------------------
-template <int N, int MODE, class T>
-__global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(MinWaves<N, T>::value))) void k_x_forward_stage(StageArgs<T> a) {
-  using S = StridedShape<N>;
+// LEAN: the 1024-point float form for the pruned layout on 16-column
+// workgroups within 128 VGPRs: rows that the compact state does not hold are
+// compile-time dead (PrunedRows), so a transformed field keeps 12 of its 16
+// values per thread; the fields are loaded one transform ahead, the twiddles
+// read from the table, and the state is addressed by 32-bit offsets from
+// uniform field bases (the launcher checks they fit).
+template <int N, bool LEAN>
+using StageShape = std::conditional_t<LEAN, LeanShape<N>, StridedShape<N>>;
+
+template <int N, int MODE, class T, bool LEAN = false>
+__global__ __launch_bounds__((StageShape<N, LEAN>::THREADS)) __attribute__((amdgpu_waves_per_eu(LEAN ? 4 : MinWaves<N, T>::value))) void k_x_forward_stage(StageArgs<T> a) {
+  using S = StageShape<N, LEAN>;
   using F = typename S::F;
   constexpr int R = S::R, Q = S::Q, TC = S::TC;
+  constexpr unsigned kDeadOut = LEAN ? PrunedRows<F, N>::kOut : 0u;
   __shared__ cpx<T> lds[S::LDS];
   const Geo& g = a.g;
   const int c = threadIdx.x % TC, q = threadIdx.x / TC;
@@ -748,19 +868,56 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
   const T kyv = wavenumber<T>(g.k0y, ny);
   const T kzv = wavenumber<T>(g.k0z, iz);
   SlotStrided<TC, T> L{lds + c};
-  cpx<T> tw[R];
-  F::load_tw(tw, q);
   cpx<T> v0[R], v1[R], v2[R];
   cpx<T>* vv[3] = {v0, v1, v2};
-  // all three fields are loaded before the first transform (see k_z_fused)
+  if constexpr (LEAN) {
+    constexpr uint32_t E = sizeof(cpx<T>);
+    auto load = [&](int f) {
+      const int qv = opaque(q);
+      const cpx<T>* src = a.nl + f * a.nl_fs;
 #pragma unroll
-  for (int f = 0; f < 3; ++f) {
-    const cpx<T>* src = a.nl + f * a.nl_fs + jj;
+      for (int r = 0; r < R; ++r)
+        vv[f][r] = active ? ld_off(src, (uint32_t(qv + Q * r) * uint32_t(g.jp) + uint32_t(jj)) * E) : czero<T>();
+    };
+    auto fft = [&](int f) {
+      const int qv = opaque(q);
+      F::template run<-1>(vv[f], qv, L, Twiddles<F, T, true>(qv));
+    };
+    load(0);
+    load(1);
+    fft(0);
+    load(2);
+    fft(1);
+    fft(2);
+    // the transformed first field waits in LDS (its live slots, thread-major
+    // rows: conflict-free) while the epilogue runs, which frees its registers
+    static_assert(S::LDS >= R * S::THREADS, "parking area");
+    __syncthreads();
 #pragma unroll
-    for (int r = 0; r < R; ++r) vv[f][r] = active ? src[int64_t(q + Q * r) * g.jp] : czero<T>();
+    for (int s = 0; s < R; ++s)
+      if (!((kDeadOut >> s) & 1u)) lds[s * S::THREADS + threadIdx.x] = v0[s];
+  } else {
+    cpx<T> tw[R];
+    F::load_tw(tw, q);
+    // all three fields are loaded before the first transform (see k_z_fused)
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const cpx<T>* src = a.nl + f * a.nl_fs + jj;
+#pragma unroll
+      for (int r = 0; r < R; ++r) vv[f][r] = active ? src[int64_t(q + Q * r) * g.jp] : czero<T>();
+    }
+#pragma unroll
+    for (int f = 0; f < 3; ++f) F::template run<-1>(vv[f], q, L, tw);
   }
-#pragma unroll
-  for (int f = 0; f < 3; ++f) F::template run<-1>(vv[f], q, L, tw);
+  // state accesses of the epilogue: field f at mode offset o
+  auto LD = [&](const cpx<T>* p, int f, int64_t o) -> cpx<T> {
+    if constexpr (LEAN) return ld_off(p + f * a.st_fs, uint32_t(o) * uint32_t(sizeof(cpx<T>)));
+    else return p[o + f * a.st_fs];
+  };
+  auto ST = [&](cpx<T>* p, int f, int64_t o, cpx<T> v) {
+    if constexpr (LEAN) st_off(p + f * a.st_fs, uint32_t(o) * uint32_t(sizeof(cpx<T>)), v);
+    else p[o + f * a.st_fs] = v;
+  };
   T coef = a.coef_const;
   if (a.fterm != kTermNone && a.coef_dev) coef = T(*a.coef_dev);
   const int wgt = hermitian_weight(iz, g.nz);
@@ -768,7 +925,11 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
   const T hdt = T(0.5) * a.dt, sdt = a.dt / T(6), tdt = a.dt / T(3);
 #pragma unroll
   for (int s = 0; s < R; ++s) {
-    const int ix = compact_of_row(F::out_index(q, s), N, g.kx, g.px);
+    if ((kDeadOut >> s) & 1u) continue;
+    // lean: an opaque index per mode keeps the compiler from issuing every
+    // mode's state loads at once (which spills)
+    const int qs = LEAN ? opaque(q) : q;
+    const int ix = compact_of_row(F::out_index(qs, s), N, g.kx, g.px);
     if (!active || ix < 0) continue;
     const int nx = freq_of(ix, g.kx, g.px);
     const T kxv = wavenumber<T>(g.k0x, nx);
@@ -776,11 +937,12 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
     const T ik2 = inv_k2_of(k2);
     const bool keep = in_box(nx, ny, iz, g);
     const int64_t o = int64_t(ix) * g.jp + j;
-    const cpx<T> s0 = a.us[o], s1 = a.us[o + a.st_fs], s2 = a.us[o + 2 * a.st_fs];
+    const cpx<T> s0 = LD(a.us, 0, o), s1 = LD(a.us, 1, o), s2 = LD(a.us, 2, o);
     T fc = T(0);
     if (a.fterm == kTermUniform) fc = coef;
     if (a.fterm == kTermBand) fc = (a.kmin2 <= k2 && k2 <= a.kmax2) ? coef : T(0);
-    cpx<T> d0 = keep ? v0[s] + fc * s0 : czero<T>();
+    const cpx<T> x0 = LEAN ? lds[s * S::THREADS + threadIdx.x] : v0[s];
+    cpx<T> d0 = keep ? x0 + fc * s0 : czero<T>();
     cpx<T> d1 = keep ? v1[s] + fc * s1 : czero<T>();
     cpx<T> d2 = keep ? v2[s] + fc * s2 : czero<T>();
     const cpx<T> div = ik2 * (kxv * d0 + kyv * d1 + kzv * d2);
@@ -802,16 +964,16 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
       //   u_new = E_f u + dt/6 (E_f k1 + 2 E_h (k2 + k3)) + dt/6 k4
       const T eh = exp_neg<T>(T(0.5) * nk2 * a.dt), ef = eh * eh;
       if constexpr (MODE == kStageIf4) {
-        o0 = a.acc[o] + sdt * d0;
-        o1 = a.acc[o + a.st_fs] + sdt * d1;
-        o2 = a.acc[o + 2 * a.st_fs] + sdt * d2;
+        o0 = LD(a.acc, 0, o) + sdt * d0;
+        o1 = LD(a.acc, 1, o) + sdt * d1;
+        o2 = LD(a.acc, 2, o) + sdt * d2;
       } else {
-        const cpx<T> b0 = a.u[o], b1 = a.u[o + a.st_fs], b2 = a.u[o + 2 * a.st_fs];
+        const cpx<T> b0 = LD(a.u, 0, o), b1 = LD(a.u, 1, o), b2 = LD(a.u, 2, o);
         if constexpr (MODE == kStageIf1) {
           o0 = eh * (b0 + hdt * d0); o1 = eh * (b1 + hdt * d1); o2 = eh * (b2 + hdt * d2);
-          a.acc[o] = ef * (b0 + sdt * d0);
-          a.acc[o + a.st_fs] = ef * (b1 + sdt * d1);
-          a.acc[o + 2 * a.st_fs] = ef * (b2 + sdt * d2);
+          ST(a.acc, 0, o, ef * (b0 + sdt * d0));
+          ST(a.acc, 1, o, ef * (b1 + sdt * d1));
+          ST(a.acc, 2, o, ef * (b2 + sdt * d2));
         } else {
           const T tde = tdt * eh;
           if constexpr (MODE == kStageIf2) {
@@ -820,34 +982,34 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
             const T fde = a.dt * eh;
             o0 = ef * b0 + fde * d0; o1 = ef * b1 + fde * d1; o2 = ef * b2 + fde * d2;
           }
-          a.acc[o] = a.acc[o] + tde * d0;
-          a.acc[o + a.st_fs] = a.acc[o + a.st_fs] + tde * d1;
-          a.acc[o + 2 * a.st_fs] = a.acc[o + 2 * a.st_fs] + tde * d2;
+          ST(a.acc, 0, o, LD(a.acc, 0, o) + tde * d0);
+          ST(a.acc, 1, o, LD(a.acc, 1, o) + tde * d1);
+          ST(a.acc, 2, o, LD(a.acc, 2, o) + tde * d2);
         }
       }
     } else if constexpr (MODE == kStageRhs) {
       o0 = d0; o1 = d1; o2 = d2;
     } else if constexpr (MODE == kStage4) {
-      o0 = a.acc[o] + sdt * d0;
-      o1 = a.acc[o + a.st_fs] + sdt * d1;
-      o2 = a.acc[o + 2 * a.st_fs] + sdt * d2;
+      o0 = LD(a.acc, 0, o) + sdt * d0;
+      o1 = LD(a.acc, 1, o) + sdt * d1;
+      o2 = LD(a.acc, 2, o) + sdt * d2;
     } else {
-      const cpx<T> b0 = a.u[o], b1 = a.u[o + a.st_fs], b2 = a.u[o + 2 * a.st_fs];
+      const cpx<T> b0 = LD(a.u, 0, o), b1 = LD(a.u, 1, o), b2 = LD(a.u, 2, o);
       const T h = MODE == kStage3 ? a.dt : hdt;
       o0 = b0 + h * d0; o1 = b1 + h * d1; o2 = b2 + h * d2;
       if constexpr (MODE == kStage1) {
-        a.acc[o] = b0 + sdt * d0;
-        a.acc[o + a.st_fs] = b1 + sdt * d1;
-        a.acc[o + 2 * a.st_fs] = b2 + sdt * d2;
+        ST(a.acc, 0, o, b0 + sdt * d0);
+        ST(a.acc, 1, o, b1 + sdt * d1);
+        ST(a.acc, 2, o, b2 + sdt * d2);
       } else {
-        a.acc[o] = a.acc[o] + tdt * d0;
-        a.acc[o + a.st_fs] = a.acc[o + a.st_fs] + tdt * d1;
-        a.acc[o + 2 * a.st_fs] = a.acc[o + 2 * a.st_fs] + tdt * d2;
+        ST(a.acc, 0, o, LD(a.acc, 0, o) + tdt * d0);
+        ST(a.acc, 1, o, LD(a.acc, 1, o) + tdt * d1);
+        ST(a.acc, 2, o, LD(a.acc, 2, o) + tdt * d2);
       }
     }
-    a.out[o] = o0;
-    a.out[o + a.st_fs] = o1;
-    a.out[o + 2 * a.st_fs] = o2;
+    ST(a.out, 0, o, o0);
+    ST(a.out, 1, o, o1);
+    ST(a.out, 2, o, o2);
     if (a.partials) {
       const double e = double(o0.x) * o0.x + double(o0.y) * o0.y + double(o1.x) * o1.x +
                        double(o1.y) * o1.y + double(o2.x) * o2.x + double(o2.y) * o2.y;
@@ -886,6 +1048,16 @@ static inline bool fits_u32(int64_t max_elem, size_t esize) {
 template <class T>
 static constexpr bool lean_type() { return sizeof(T) == 4 && H3_LEAN1024; }
 
+// the pruned layout's retained row count along an axis of N points
+static inline bool pruned_rows(int N, int k, int p) { return k == 2 * (N / 3) + 1 && p == (k + 1) / 2 && k < N; }
+
+template <int DIR, class T, bool PIN, bool POUT>
+static void launch_strided_lean(const StridedArgs<T>& a, hipStream_t s) {
+  constexpr int NN = 1024;
+  const unsigned nb = ceil_div(a.J, LeanShape<NN>::TC);
+  hipLaunchKernelGGL((k_strided_lean<NN, DIR, T, PIN, POUT>), dim3(nb), dim3(LeanShape<NN>::THREADS), 0, s, a);
+}
+
 template <class T>
 hipError_t launch_strided(int N, int dir, const StridedArgs<T>& a, hipStream_t s) {
   if (a.J <= 0 || a.nf <= 0) return hipSuccess;
@@ -894,10 +1066,14 @@ hipError_t launch_strided(int N, int dir, const StridedArgs<T>& a, hipStream_t s
     const bool fit = fits_u32(ao * a.in_slab + (a.B - 1) + int64_t(N - 1) * a.B, sizeof(cpx<T>)) &&
                      fits_u32(ao * a.out_slab + (a.B - 1) + int64_t(N - 1) * a.B, sizeof(cpx<T>));
     if (N == 1024 && fit) {
-      constexpr int NN = 1024;
-      const unsigned nb = ceil_div(a.J, LeanShape<NN>::TC);
-      if (dir < 0) hipLaunchKernelGGL((k_strided_lean<NN, -1, T>), dim3(nb), dim3(LeanShape<NN>::THREADS), 0, s, a);
-      else hipLaunchKernelGGL((k_strided_lean<NN, +1, T>), dim3(nb), dim3(LeanShape<NN>::THREADS), 0, s, a);
+      const bool pin = pruned_rows(N, a.in_k, a.in_p), pout = pruned_rows(N, a.out_k, a.out_p);
+      if (dir < 0) {
+        if (pin) { if (pout) launch_strided_lean<-1, T, true, true>(a, s); else launch_strided_lean<-1, T, true, false>(a, s); }
+        else { if (pout) launch_strided_lean<-1, T, false, true>(a, s); else launch_strided_lean<-1, T, false, false>(a, s); }
+      } else {
+        if (pin) { if (pout) launch_strided_lean<+1, T, true, true>(a, s); else launch_strided_lean<+1, T, true, false>(a, s); }
+        else { if (pout) launch_strided_lean<+1, T, false, true>(a, s); else launch_strided_lean<+1, T, false, false>(a, s); }
+      }
       return hipGetLastError();
     }
   }
@@ -920,8 +1096,12 @@ hipError_t launch_x_inverse_curl(int N, const cpx<T>* us, int64_t us_fs, cpx<T>*
         fits_u32(int64_t(g.nx) * g.jp, sizeof(cpx<T>))) {
       constexpr int NN = 1024;
       const unsigned nb = ceil_div(int64_t(g.ky) * g.kzs, LeanShape<NN>::TC);
-      hipLaunchKernelGGL((k_x_inverse_curl_lean<NN, T>), dim3(nb), dim3(LeanShape<NN>::THREADS), 0, s, us, us_fs,
-                         out, out_fs, g, j);
+      if (pruned_rows(N, g.kx, g.px))
+        hipLaunchKernelGGL((k_x_inverse_curl_lean<NN, T, true>), dim3(nb), dim3(LeanShape<NN>::THREADS), 0, s, us,
+                           us_fs, out, out_fs, g, j);
+      else
+        hipLaunchKernelGGL((k_x_inverse_curl_lean<NN, T, false>), dim3(nb), dim3(LeanShape<NN>::THREADS), 0, s, us,
+                           us_fs, out, out_fs, g, j);
       return hipGetLastError();
     }
   }
@@ -936,6 +1116,20 @@ hipError_t launch_x_inverse_curl(int N, const cpx<T>* us, int64_t us_fs, cpx<T>*
 template <class T>
 hipError_t launch_y_inverse_curl(int N, const cpx<T>* in, int64_t in_fs, cpx<T>* out,
                                  int64_t out_fs, const Geo& g, hipStream_t s) {
+  if constexpr (lean_type<T>()) {
+    if (N == 1024 && fits_u32(int64_t(g.nx) * g.jp, sizeof(cpx<T>)) &&
+        fits_u32(int64_t(g.nx) * g.ny * g.kzs, sizeof(cpx<T>))) {
+      constexpr int NN = 1024;
+      const unsigned nb = ceil_div(int64_t(g.nx) * g.kzs, LeanShape<NN>::TC);
+      if (pruned_rows(N, g.ky, g.py))
+        hipLaunchKernelGGL((k_y_inverse_curl_lean<NN, T, true>), dim3(nb), dim3(LeanShape<NN>::THREADS), 0, s, in,
+                           in_fs, out, out_fs, g);
+      else
+        hipLaunchKernelGGL((k_y_inverse_curl_lean<NN, T, false>), dim3(nb), dim3(LeanShape<NN>::THREADS), 0, s, in,
+                           in_fs, out, out_fs, g);
+      return hipGetLastError();
+    }
+  }
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(int64_t(g.nx) * g.kzs, StridedShape<NN>::TC);
     hipLaunchKernelGGL((k_y_inverse_curl<NN, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, in, in_fs, out,
@@ -1004,6 +1198,28 @@ template <class T>
 hipError_t launch_x_forward_stage(int N, int mode, const StageArgs<T>& a, hipStream_t s,
                                   int* nblocks_out) {
   const int64_t J = int64_t(a.g.ky) * a.g.kzs;
+  if constexpr (lean_type<T>()) {
+    if (N == 1024 && pruned_rows(N, a.g.kx, a.g.px) && fits_u32(int64_t(a.g.nx) * a.g.jp, sizeof(cpx<T>)) &&
+        fits_u32(int64_t(a.g.kx) * a.g.jp, sizeof(cpx<T>))) {
+      constexpr int NN = 1024;
+      const unsigned nb = ceil_div(J, LeanShape<NN>::TC);
+      if (nblocks_out) *nblocks_out = int(nb);
+      const dim3 blk(LeanShape<NN>::THREADS);
+      switch (mode) {
+        case kStageRhs: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageRhs, T, true>), dim3(nb), blk, 0, s, a); break;
+        case kStage1: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage1, T, true>), dim3(nb), blk, 0, s, a); break;
+        case kStage2: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage2, T, true>), dim3(nb), blk, 0, s, a); break;
+        case kStage3: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage3, T, true>), dim3(nb), blk, 0, s, a); break;
+        case kStage4: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage4, T, true>), dim3(nb), blk, 0, s, a); break;
+        case kStageIf1: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageIf1, T, true>), dim3(nb), blk, 0, s, a); break;
+        case kStageIf2: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageIf2, T, true>), dim3(nb), blk, 0, s, a); break;
+        case kStageIf3: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageIf3, T, true>), dim3(nb), blk, 0, s, a); break;
+        case kStageIf4: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageIf4, T, true>), dim3(nb), blk, 0, s, a); break;
+        default: return hipErrorInvalidValue;
+      }
+      return hipGetLastError();
+    }
+  }
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(J, StridedShape<NN>::TC);
     if (nblocks_out) *nblocks_out = int(nb);
