@@ -14,14 +14,52 @@
 
 namespace h3 {
 
+// Rows a pruned axis never stores: the 2/3 box keeps K = 2(N/3) + 1 rows,
+// the P = (K + 1)/2 non-negative frequencies first, so rows P .. N-(K-P)-1
+// are absent. Bit r of kIn: entry slot r (row q + Q r) is absent for every
+// thread q, so its value is a compile-time zero; bit s of kOut: exit slot s
+// (row out_index(q, s)) likewise, so its value is never stored or used. At
+// 1024 points 4 of a thread's 16 slots are absent either way, which frees
+// the registers the lean kernels need.
+template <class F, int N>
+struct PrunedRows {
+  static constexpr int K = 2 * (N / 3) + 1, P = (K + 1) / 2;
+  static constexpr bool absent(int row) { return row >= P && row < N - (K - P); }
+  static constexpr unsigned in_mask() {
+    unsigned m = 0;
+    for (int r = 0; r < F::R; ++r) {
+      bool all = true;
+      for (int q = 0; q < F::Q; ++q) all = all && absent(q + F::Q * r);
+      if (all) m |= 1u << r;
+    }
+    return m;
+  }
+  static constexpr unsigned out_mask() {
+    unsigned m = 0;
+    for (int s = 0; s < F::R; ++s) {
+      bool all = true;
+      for (int q = 0; q < F::Q; ++q) all = all && absent(F::out_index(q, s));
+      if (all) m |= 1u << s;
+    }
+    return m;
+  }
+  static constexpr unsigned kIn = in_mask(), kOut = out_mask();
+};
+
+// Point counts whose fat kernels have pruned-row instances (at 1024 the lean
+// kernels take the pruned layout; below 128 no slot is absent for every thread)
+template <int N>
+constexpr bool kPrunedN = N == 128 || N == 256 || N == 512;
+
 // ---------------------------------------------------------------------------
 // Plain strided pass (any axis, either direction)
 // ---------------------------------------------------------------------------
-template <int N, int DIR, class T>
+template <int N, int DIR, class T, bool PIN = false, bool POUT = false>
 __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(MinWaves<N, T>::value))) void k_strided(StridedArgs<T> a) {
   using S = StridedShape<N>;
   using F = typename S::F;
   constexpr int R = S::R, Q = S::Q, TC = S::TC;
+  constexpr unsigned kZeroIn = PIN ? PrunedRows<F, N>::kIn : 0u, kDeadOut = POUT ? PrunedRows<F, N>::kOut : 0u;
   __shared__ cpx<T> lds[S::LDS];
   const int c = threadIdx.x % TC, q = threadIdx.x / TC;
   const int j = xcd_tile(blockIdx.x, gridDim.x) * TC + c;
@@ -39,7 +77,8 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
   auto load = [&](int f, cpx<T>* v) {
     const cpx<T>* src = in0 + f * a.in_fs;
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = irow[r] >= 0 ? src[int64_t(irow[r]) * a.B] : czero<T>();
+    for (int r = 0; r < R; ++r)
+      v[r] = !((kZeroIn >> r) & 1u) && irow[r] >= 0 ? src[int64_t(irow[r]) * a.B] : czero<T>();
   };
   // field f+1 is loaded before field f's transform, whose barriers would
   // otherwise hold the loads back
@@ -54,6 +93,7 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
     cpx<T>* dst = out0 + f * a.out_fs;
 #pragma unroll
     for (int s = 0; s < R; ++s) {
+      if ((kDeadOut >> s) & 1u) continue;
       const int o = compact_of_row(F::out_index(q, s), N, a.out_k, a.out_p);
       if (active && o >= 0) dst[int64_t(o) * a.B] = v[s];
     }
@@ -65,7 +105,7 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
 //   w1 = i(kz u0 - kx u2), w2 = i(kx u1 - ky u0)   (navierstokes.py:73-77)
 // out fields: [u0, u1, u2, w1, w2]
 // ---------------------------------------------------------------------------
-template <int N, class T>
+template <int N, class T, bool PIN = false>
 __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(MinWaves<N, T>::value))) void k_x_inverse_curl(const cpx<T>* __restrict__ us,
                                                             int64_t us_fs,
                                                             cpx<T>* __restrict__ out,
@@ -74,6 +114,7 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
   using S = StridedShape<N>;
   using F = typename S::F;
   constexpr int R = S::R, Q = S::Q, TC = S::TC;
+  constexpr unsigned kZeroIn = PIN ? PrunedRows<F, N>::kIn : 0u;
   static_assert(S::THREADS % 64 == 0 && S::THREADS <= 1024, "finalize_in_block needs whole waves");
   if (job.partials != nullptr && blockIdx.x == 0) finalize_in_block(job);   // uniform per block
   __shared__ cpx<T> lds[S::LDS];
@@ -99,7 +140,8 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
   const cpx<T>* u2 = u1 + us_fs;
   auto load = [&](const cpx<T>* p, cpx<T>* v) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = irow[r] >= 0 ? p[int64_t(irow[r]) * g.jp] : czero<T>();
+    for (int r = 0; r < R; ++r)
+      v[r] = !((kZeroIn >> r) & 1u) && irow[r] >= 0 ? p[int64_t(irow[r]) * g.jp] : czero<T>();
   };
   auto store = [&](int f, const cpx<T>* v) {
     if (!active) return;
@@ -134,7 +176,7 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
 // in  [5][Nx][jp] fields (u0, u1, u2, w1, w2), rows iy (stride Kz)
 // out [6][Nx][Ny][Kz] fields (u0, u1, u2, w0, w1, w2)
 // ---------------------------------------------------------------------------
-template <int N, class T>
+template <int N, class T, bool PIN = false>
 __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(MinWaves<N, T>::value))) void k_y_inverse_curl(const cpx<T>* __restrict__ in,
                                                             int64_t in_fs,
                                                             cpx<T>* __restrict__ out,
@@ -142,6 +184,7 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
   using S = StridedShape<N>;
   using F = typename S::F;
   constexpr int R = S::R, Q = S::Q, TC = S::TC;
+  constexpr unsigned kZeroIn = PIN ? PrunedRows<F, N>::kIn : 0u;
   __shared__ cpx<T> lds[S::LDS];
   const int c = threadIdx.x % TC, q = threadIdx.x / TC;
   const int j = xcd_tile(blockIdx.x, gridDim.x) * TC + c;
@@ -164,7 +207,7 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
   auto load = [&](int f, cpx<T>* v) {
     const cpx<T>* p = src + f * in_fs;
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = irow[r] >= 0 ? p[irow[r] * g.kzs] : czero<T>();
+    for (int r = 0; r < R; ++r) v[r] = !((kZeroIn >> r) & 1u) && irow[r] >= 0 ? p[irow[r] * g.kzs] : czero<T>();
   };
   auto store = [&](int f, const cpx<T>* v) {
     if (!active) return;
@@ -229,38 +272,6 @@ template <class T>
 __device__ __forceinline__ void st_off(cpx<T>* base, uint32_t off, cpx<T> v) {
   *reinterpret_cast<cpx<T>*>(reinterpret_cast<char*>(base) + off) = v;
 }
-
-// Rows a pruned axis never stores: the 2/3 box keeps K = 2(N/3) + 1 rows,
-// the P = (K + 1)/2 non-negative frequencies first, so rows P .. N-(K-P)-1
-// are absent. Bit r of kIn: entry slot r (row q + Q r) is absent for every
-// thread q, so its value is a compile-time zero; bit s of kOut: exit slot s
-// (row out_index(q, s)) likewise, so its value is never stored or used. At
-// 1024 points 4 of a thread's 16 slots are absent either way, which frees
-// the registers the lean kernels need.
-template <class F, int N>
-struct PrunedRows {
-  static constexpr int K = 2 * (N / 3) + 1, P = (K + 1) / 2;
-  static constexpr bool absent(int row) { return row >= P && row < N - (K - P); }
-  static constexpr unsigned in_mask() {
-    unsigned m = 0;
-    for (int r = 0; r < F::R; ++r) {
-      bool all = true;
-      for (int q = 0; q < F::Q; ++q) all = all && absent(q + F::Q * r);
-      if (all) m |= 1u << r;
-    }
-    return m;
-  }
-  static constexpr unsigned out_mask() {
-    unsigned m = 0;
-    for (int s = 0; s < F::R; ++s) {
-      bool all = true;
-      for (int q = 0; q < F::Q; ++q) all = all && absent(F::out_index(q, s));
-      if (all) m |= 1u << s;
-    }
-    return m;
-  }
-  static constexpr unsigned kIn = in_mask(), kOut = out_mask();
-};
 
 // PIN / POUT: the input / output rows are the pruned layout's (compact index
 // = row of the retained set, as compact_of_row maps them), so the absent
@@ -583,6 +594,9 @@ __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_fused(cpx<T>* buf
     for (int s = 0; s < R; ++s)
       Bv[F::out_slot_as_input(s)] = {(z0[s].x * z1[s].y - z1[s].x * z0[s].y) * scale, T(0)};
   };
+  // nl2 keeps only k < kz; in the pruned layout kz = N/3 + 1 is known here,
+  // so the exit slots above it (5 of 8 at 256 points, 10 of 16 at 1024) are
+  // never computed
   auto store_nl2 = [&](const cpx<T>* Bv) {
     if (!active) return;
     cpx<T>* dst = buf + (kZOut + 2) * fs + base;
@@ -590,7 +604,7 @@ __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_fused(cpx<T>* buf
     for (int s = 0; s < R; ++s) {
       if ((kUnused >> s) & 1u) continue;
       const int k = F::out_index(q, s);
-      if (k < kz) dst[k] = Bv[s];
+      if (k < (PRUNED ? Z::kz : kz)) dst[k] = Bv[s];
     }
     for (int k = kz + q; k < kzs; k += Q) dst[k] = czero<T>();
   };
@@ -851,12 +865,12 @@ __device__ __forceinline__ T exp_neg(T x) {
 template <int N, bool LEAN>
 using StageShape = std::conditional_t<LEAN, LeanShape<N>, StridedShape<N>>;
 
-template <int N, int MODE, class T, bool LEAN = false>
+template <int N, int MODE, class T, bool LEAN = false, bool POUT = LEAN>
 __global__ __launch_bounds__((StageShape<N, LEAN>::THREADS)) __attribute__((amdgpu_waves_per_eu(LEAN ? 4 : MinWaves<N, T>::value))) void k_x_forward_stage(StageArgs<T> a) {
   using S = StageShape<N, LEAN>;
   using F = typename S::F;
   constexpr int R = S::R, Q = S::Q, TC = S::TC;
-  constexpr unsigned kDeadOut = LEAN ? PrunedRows<F, N>::kOut : 0u;
+  constexpr unsigned kDeadOut = POUT ? PrunedRows<F, N>::kOut : 0u;
   __shared__ cpx<T> lds[S::LDS];
   const Geo& g = a.g;
   const int c = threadIdx.x % TC, q = threadIdx.x / TC;
@@ -1077,10 +1091,19 @@ hipError_t launch_strided(int N, int dir, const StridedArgs<T>& a, hipStream_t s
       return hipGetLastError();
     }
   }
+  const bool pin = pruned_rows(N, a.in_k, a.in_p), pout = pruned_rows(N, a.out_k, a.out_p);
   H3_DISPATCH_N(N, {
-    const unsigned nb = ceil_div(a.J, StridedShape<NN>::TC);
-    if (dir < 0) hipLaunchKernelGGL((k_strided<NN, -1, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a);
-    else hipLaunchKernelGGL((k_strided<NN, +1, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a);
+    const dim3 grid(ceil_div(a.J, StridedShape<NN>::TC));
+    const dim3 blk(StridedShape<NN>::THREADS);
+    if constexpr (kPrunedN<NN>) {
+      if (pin && dir > 0) hipLaunchKernelGGL((k_strided<NN, +1, T, true, false>), grid, blk, 0, s, a);
+      else if (pout && dir < 0) hipLaunchKernelGGL((k_strided<NN, -1, T, false, true>), grid, blk, 0, s, a);
+      else if (dir < 0) hipLaunchKernelGGL((k_strided<NN, -1, T>), grid, blk, 0, s, a);
+      else hipLaunchKernelGGL((k_strided<NN, +1, T>), grid, blk, 0, s, a);
+    } else {
+      if (dir < 0) hipLaunchKernelGGL((k_strided<NN, -1, T>), grid, blk, 0, s, a);
+      else hipLaunchKernelGGL((k_strided<NN, +1, T>), grid, blk, 0, s, a);
+    }
   });
   return hipGetLastError();
 }
@@ -1105,10 +1128,17 @@ hipError_t launch_x_inverse_curl(int N, const cpx<T>* us, int64_t us_fs, cpx<T>*
       return hipGetLastError();
     }
   }
+  const bool pin = pruned_rows(N, g.kx, g.px);
   H3_DISPATCH_N(N, {
-    const unsigned nb = ceil_div(int64_t(g.ky) * g.kzs, StridedShape<NN>::TC);
-    hipLaunchKernelGGL((k_x_inverse_curl<NN, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, us, us_fs, out,
-                       out_fs, g, j);
+    const dim3 grid(ceil_div(int64_t(g.ky) * g.kzs, StridedShape<NN>::TC));
+    const dim3 blk(StridedShape<NN>::THREADS);
+    if constexpr (kPrunedN<NN>) {
+      if (pin) {
+        hipLaunchKernelGGL((k_x_inverse_curl<NN, T, true>), grid, blk, 0, s, us, us_fs, out, out_fs, g, j);
+        break;
+      }
+    }
+    hipLaunchKernelGGL((k_x_inverse_curl<NN, T>), grid, blk, 0, s, us, us_fs, out, out_fs, g, j);
   });
   return hipGetLastError();
 }
@@ -1130,10 +1160,17 @@ hipError_t launch_y_inverse_curl(int N, const cpx<T>* in, int64_t in_fs, cpx<T>*
       return hipGetLastError();
     }
   }
+  const bool pin = pruned_rows(N, g.ky, g.py);
   H3_DISPATCH_N(N, {
-    const unsigned nb = ceil_div(int64_t(g.nx) * g.kzs, StridedShape<NN>::TC);
-    hipLaunchKernelGGL((k_y_inverse_curl<NN, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, in, in_fs, out,
-                       out_fs, g);
+    const dim3 grid(ceil_div(int64_t(g.nx) * g.kzs, StridedShape<NN>::TC));
+    const dim3 blk(StridedShape<NN>::THREADS);
+    if constexpr (kPrunedN<NN>) {
+      if (pin) {
+        hipLaunchKernelGGL((k_y_inverse_curl<NN, T, true>), grid, blk, 0, s, in, in_fs, out, out_fs, g);
+        break;
+      }
+    }
+    hipLaunchKernelGGL((k_y_inverse_curl<NN, T>), grid, blk, 0, s, in, in_fs, out, out_fs, g);
   });
   return hipGetLastError();
 }
@@ -1220,19 +1257,39 @@ hipError_t launch_x_forward_stage(int N, int mode, const StageArgs<T>& a, hipStr
       return hipGetLastError();
     }
   }
+  const bool pout = pruned_rows(N, a.g.kx, a.g.px);
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(J, StridedShape<NN>::TC);
     if (nblocks_out) *nblocks_out = int(nb);
+    const dim3 grid(nb);
+    const dim3 blk(StridedShape<NN>::THREADS);
+    if constexpr (kPrunedN<NN>) {
+      if (pout) {
+        switch (mode) {
+          case kStageRhs: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageRhs, T, false, true>), grid, blk, 0, s, a); break;
+          case kStage1: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage1, T, false, true>), grid, blk, 0, s, a); break;
+          case kStage2: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage2, T, false, true>), grid, blk, 0, s, a); break;
+          case kStage3: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage3, T, false, true>), grid, blk, 0, s, a); break;
+          case kStage4: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage4, T, false, true>), grid, blk, 0, s, a); break;
+          case kStageIf1: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageIf1, T, false, true>), grid, blk, 0, s, a); break;
+          case kStageIf2: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageIf2, T, false, true>), grid, blk, 0, s, a); break;
+          case kStageIf3: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageIf3, T, false, true>), grid, blk, 0, s, a); break;
+          case kStageIf4: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageIf4, T, false, true>), grid, blk, 0, s, a); break;
+          default: return hipErrorInvalidValue;
+        }
+        break;
+      }
+    }
     switch (mode) {
-      case kStageRhs: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageRhs, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a); break;
-      case kStage1: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage1, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a); break;
-      case kStage2: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage2, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a); break;
-      case kStage3: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage3, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a); break;
-      case kStage4: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage4, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a); break;
-      case kStageIf1: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageIf1, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a); break;
-      case kStageIf2: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageIf2, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a); break;
-      case kStageIf3: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageIf3, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a); break;
-      case kStageIf4: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageIf4, T>), dim3(nb), dim3(StridedShape<NN>::THREADS), 0, s, a); break;
+      case kStageRhs: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageRhs, T>), grid, blk, 0, s, a); break;
+      case kStage1: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage1, T>), grid, blk, 0, s, a); break;
+      case kStage2: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage2, T>), grid, blk, 0, s, a); break;
+      case kStage3: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage3, T>), grid, blk, 0, s, a); break;
+      case kStage4: hipLaunchKernelGGL((k_x_forward_stage<NN, kStage4, T>), grid, blk, 0, s, a); break;
+      case kStageIf1: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageIf1, T>), grid, blk, 0, s, a); break;
+      case kStageIf2: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageIf2, T>), grid, blk, 0, s, a); break;
+      case kStageIf3: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageIf3, T>), grid, blk, 0, s, a); break;
+      case kStageIf4: hipLaunchKernelGGL((k_x_forward_stage<NN, kStageIf4, T>), grid, blk, 0, s, a); break;
       default: return hipErrorInvalidValue;
     }
   });
