@@ -865,12 +865,24 @@ __device__ __forceinline__ T exp_neg(T x) {
 template <int N, bool LEAN>
 using StageShape = std::conditional_t<LEAN, LeanShape<N>, StridedShape<N>>;
 
+// number of slots below s not set in DEAD
+template <unsigned DEAD>
+__host__ __device__ constexpr int live_rank(int s) {
+  int r = 0;
+  for (int i = 0; i < s; ++i) r += ((DEAD >> i) & 1u) ? 0 : 1;
+  return r;
+}
+
 template <int N, int MODE, class T, bool LEAN = false, bool POUT = LEAN>
 __global__ __launch_bounds__((StageShape<N, LEAN>::THREADS)) __attribute__((amdgpu_waves_per_eu(LEAN ? 4 : MinWaves<N, T>::value))) void k_x_forward_stage(StageArgs<T> a) {
   using S = StageShape<N, LEAN>;
   using F = typename S::F;
   constexpr int R = S::R, Q = S::Q, TC = S::TC;
   constexpr unsigned kDeadOut = POUT ? PrunedRows<F, N>::kOut : 0u;
+  // lean: live slots per field, and how many of the second field's fit in
+  // LDS beside the first's
+  constexpr int kLive = live_rank<kDeadOut>(R);
+  constexpr int kPark1 = LEAN ? (S::LDS / S::THREADS - kLive < kLive ? S::LDS / S::THREADS - kLive : kLive) : 0;
   __shared__ cpx<T> lds[S::LDS];
   const Geo& g = a.g;
   const int c = threadIdx.x % TC, q = threadIdx.x / TC;
@@ -903,13 +915,18 @@ __global__ __launch_bounds__((StageShape<N, LEAN>::THREADS)) __attribute__((amdg
     load(2);
     fft(1);
     fft(2);
-    // the transformed first field waits in LDS (its live slots, thread-major
-    // rows: conflict-free) while the epilogue runs, which frees its registers
-    static_assert(S::LDS >= R * S::THREADS, "parking area");
+    // the transformed first field (its live slots) and as many of the
+    // second's as fit wait in LDS while the epilogue runs, which frees their
+    // registers; rows of THREADS values, thread-major: conflict-free
+    static_assert(S::LDS >= (kLive + kPark1) * S::THREADS, "parking area");
     __syncthreads();
 #pragma unroll
-    for (int s = 0; s < R; ++s)
-      if (!((kDeadOut >> s) & 1u)) lds[s * S::THREADS + threadIdx.x] = v0[s];
+    for (int s = 0; s < R; ++s) {
+      if ((kDeadOut >> s) & 1u) continue;
+      const int rk = live_rank<kDeadOut>(s);
+      lds[rk * S::THREADS + threadIdx.x] = v0[s];
+      if (rk < kPark1) lds[(kLive + rk) * S::THREADS + threadIdx.x] = v1[s];
+    }
   } else {
     cpx<T> tw[R];
     F::load_tw(tw, q);
@@ -955,9 +972,11 @@ __global__ __launch_bounds__((StageShape<N, LEAN>::THREADS)) __attribute__((amdg
     T fc = T(0);
     if (a.fterm == kTermUniform) fc = coef;
     if (a.fterm == kTermBand) fc = (a.kmin2 <= k2 && k2 <= a.kmax2) ? coef : T(0);
-    const cpx<T> x0 = LEAN ? lds[s * S::THREADS + threadIdx.x] : v0[s];
+    const int rk = live_rank<kDeadOut>(s);
+    const cpx<T> x0 = LEAN ? lds[rk * S::THREADS + threadIdx.x] : v0[s];
+    const cpx<T> x1 = LEAN && rk < kPark1 ? lds[(kLive + rk) * S::THREADS + threadIdx.x] : v1[s];
     cpx<T> d0 = keep ? x0 + fc * s0 : czero<T>();
-    cpx<T> d1 = keep ? v1[s] + fc * s1 : czero<T>();
+    cpx<T> d1 = keep ? x1 + fc * s1 : czero<T>();
     cpx<T> d2 = keep ? v2[s] + fc * s2 : czero<T>();
     const cpx<T> div = ik2 * (kxv * d0 + kyv * d1 + kzv * d2);
     const T nk2 = a.nu * k2;

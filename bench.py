@@ -21,6 +21,7 @@ import argparse
 import ctypes
 import json
 import os
+import re
 import sys
 import time
 
@@ -239,6 +240,18 @@ def adjoint_timing(torch, h3, N=128, steps=200, checkpoint=20):
             "backward_over_forward": round((el - fw) / fw, 2)}
 
 
+def _kernel_pattern(N, kernel_key):
+    """Regex of a pass's kernel name in the rocprofv3 summaries (the 1024-point
+    strided passes run their register-lean forms, named *_lean)."""
+    name = {"z_c2r_cross_r2c": "k_z_fused", "y_inverse_curl": "k_y_inverse_curl",
+            "x_inverse_curl": "k_x_inverse_curl", "y_forward": "k_strided",
+            "x_forward_stage": "k_x_forward_stage"}.get(kernel_key)
+    if name is None:
+        return None
+    tail = r", -1," if kernel_key == "y_forward" else ","
+    return r"`" + name + r"(_lean)?<" + str(N) + tail
+
+
 def profiled_traffic(N, kernel_key):
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 FETCH_SIZE + WRITE_SIZE summary of this workload (counters
@@ -247,12 +260,11 @@ def profiled_traffic(N, kernel_key):
     path = os.path.join(ROOT, "profiles", f"rk4_{N}_hbm_traffic.md")
     if not os.path.exists(path):
         return None, None
-    pat = {"z_c2r_cross_r2c": f"k_z_fused<{N},", "y_inverse_curl": f"k_y_inverse_curl<{N},",
-           "x_inverse_curl": f"k_x_inverse_curl<{N},", "y_forward": f"k_strided<{N}, -1,"}.get(kernel_key)
+    pat = _kernel_pattern(N, kernel_key)
     if pat is None:
         return None, None
     for line in open(path):
-        if pat in line and "|" in line:
+        if "|" in line and re.search(pat, line):
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
             try:   # kernel | launches | fetch MB | write MB | total MB | algorithmic MB | ratio
                 return int(float(cells[4]) * 1e6), os.path.relpath(path, ROOT)
@@ -266,13 +278,11 @@ def profiled_kernel_ms(N, kernel_key):
     rocprofv3 kernel-trace summary of this workload, for comparison with the
     live HIP-event figure; (None, None) when there is no summary."""
     path = os.path.join(ROOT, "profiles", f"rk4_{N}_kernel_trace.md")
-    pat = {"z_c2r_cross_r2c": f"k_z_fused<{N},", "y_inverse_curl": f"k_y_inverse_curl<{N},",
-           "x_inverse_curl": f"k_x_inverse_curl<{N},", "y_forward": f"k_strided<{N}, -1,",
-           "x_forward_stage": f"k_x_forward_stage<{N},"}.get(kernel_key)
+    pat = _kernel_pattern(N, kernel_key)
     if pat is None or not os.path.exists(path):
         return None, None
     for line in open(path):
-        if pat in line and "|" in line:
+        if "|" in line and re.search(pat, line):
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
             try:   # kernel | calls | average us | min us | max us | ...
                 return float(cells[2]) / 1000.0, os.path.relpath(path, ROOT)
