@@ -578,14 +578,45 @@ __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_fused(cpx<T>* buf
   F::load_tw(tw, q);
   using Z = ZeroBand<F, N>;
   constexpr unsigned kZeroIn = PRUNED ? Z::kEntry : 0u, kUnused = PRUNED ? Z::kExit : 0u;
+  // Pruned layout (the launcher guarantees padding, kzs > kz, whose slots hold
+  // zeros): each element of the Hermitian-extended pencil is read through one
+  // 32-bit byte offset per slot from the wave-uniform field bases, shared by
+  // the six fields: the stored source index, or the last padding slot (a
+  // zero) when the mode is not stored, so no select is needed except for the
+  // kz = 0 mode's imaginary part (the Nyquist mode is never stored here).
+  // Whether a slot lies on the mirrored half is a compile-time fact except in
+  // the slot that holds N/2 (per lane there).
+  // Dense layout: the index work per slot, then selects (herm_pair_at).
+  auto mirrored = [&](int r) {
+    return (2 * Q * r > N) ? true : ((2 * (Q * r + Q - 1) <= N) ? false : (2 * (q + Q * r) > N));
+  };
   HermIdx hi[R];
+  uint32_t off[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) hi[r] = herm_idx<N>(q + Q * r, kzl);
+  for (int r = 0; r < R; ++r) {
+    if constexpr (PRUNED) {
+      if ((kZeroIn >> r) & 1u) continue;
+      const int k = q + Q * r;
+      const int src = mirrored(r) ? N - k : k;
+      const int idx = src < kzl ? src : kzs - 1;
+      off[r] = (uint32_t(base) + uint32_t(idx)) * uint32_t(sizeof(cpx<T>));
+    } else {
+      hi[r] = herm_idx<N>(q + Q * r, kzl);
+    }
+  }
   auto load = [&](int c, cpx<T>* z) {
 #pragma unroll
-    for (int r = 0; r < R; ++r)
-      z[r] = ((kZeroIn >> r) & 1u) ? czero<T>()
-                                   : herm_pair_at<T>(buf + c * fs + base, buf + (3 + c) * fs + base, hi[r]);
+    for (int r = 0; r < R; ++r) {
+      if ((kZeroIn >> r) & 1u) {
+        z[r] = czero<T>();
+      } else if constexpr (PRUNED) {
+        cpx<T> u = ld_off(buf + c * fs, off[r]), w = ld_off(buf + (3 + c) * fs, off[r]);
+        if (r == 0 && q == 0) u.y = w.y = T(0);   // kz = 0: c2r keeps the real part
+        z[r] = mirrored(r) ? cpx<T>{u.x + w.y, w.x - u.y} : cpx<T>{u.x - w.y, u.y + w.x};
+      } else {
+        z[r] = herm_pair_at<T>(buf + c * fs + base, buf + (3 + c) * fs + base, hi[r]);
+      }
+    }
   };
   // nl2 (real) at exit slot s of the inverse transforms, placed at the entry
   // slot of the forward transform; then its retained modes and the padding
@@ -597,16 +628,21 @@ __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_fused(cpx<T>* buf
   // nl2 keeps only k < kz; in the pruned layout kz = N/3 + 1 is known here,
   // so the exit slots above it (5 of 8 at 256 points, 10 of 16 at 1024) are
   // never computed
+  // output element k of this pencil in field f (pruned: uniform field base
+  // and a 32-bit offset, as for the loads)
+  auto put = [&](int f, int k, cpx<T> v) {
+    if constexpr (PRUNED) st_off(buf + f * fs, (uint32_t(base) + uint32_t(k)) * uint32_t(sizeof(cpx<T>)), v);
+    else buf[f * fs + base + k] = v;
+  };
   auto store_nl2 = [&](const cpx<T>* Bv) {
     if (!active) return;
-    cpx<T>* dst = buf + (kZOut + 2) * fs + base;
 #pragma unroll
     for (int s = 0; s < R; ++s) {
       if ((kUnused >> s) & 1u) continue;
       const int k = F::out_index(q, s);
-      if (k < (PRUNED ? Z::kz : kz)) dst[k] = Bv[s];
+      if (k < (PRUNED ? Z::kz : kz)) put(kZOut + 2, k, Bv[s]);
     }
-    for (int k = kz + q; k < kzs; k += Q) dst[k] = czero<T>();
+    for (int k = kz + q; k < kzs; k += Q) put(kZOut + 2, k, czero<T>());
   };
   // physical space: z_c = u_c + i w_c at position out_index(q, s)
   cpx<T> A[R];
@@ -657,13 +693,11 @@ __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_fused(cpx<T>* buf
     if (!((kUnused >> s) & 1u)) lp[F::out_index(q, s)] = A[s];
   WaveSync::sync();
   if (active) {
-    cpx<T>* d0 = buf + kZOut * fs + base;
-    cpx<T>* d1 = buf + (kZOut + 1) * fs + base;
     for (int k = q; k < kzs; k += Q) {
       cpx<T> X = czero<T>(), Y = czero<T>();
       if (k < kz) split_pair(lp[k], lp[(N - k) & (N - 1)], X, Y);
-      d0[k] = X;
-      d1[k] = Y;
+      put(kZOut, k, X);
+      put(kZOut + 1, k, Y);
     }
   }
 }
@@ -1199,7 +1233,9 @@ hipError_t launch_z_fused(int N, cpx<T>* buf, int64_t fs, int64_t npencils, int 
                           T scale, hipStream_t s) {
   // the pruned layout stores exactly kz = N/3 + 1 modes; anything else (the
   // dense layout, kz = N/2 + 1) takes the kernel without compile-time zeros
-  const bool pruned = kz == N / 3 + 1 && kz < N / 2 + 1;
+  // (and padding slots beyond kz, read as zeros, with 32-bit offsets inside
+  // a field)
+  const bool pruned = kz == N / 3 + 1 && kz < N / 2 + 1 && kzs > kz && fits_u32(npencils * kzs, sizeof(cpx<T>));
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(npencils, PencilBlock<NN>::PW);
     if (pruned)
