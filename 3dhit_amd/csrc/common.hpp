@@ -283,6 +283,11 @@ template <>
 struct FftSel<256, false> {
   using type = Fft3<256, 32>;
 };
+// 128 points: the three-level transform over 16 threads (8 values per thread)
+template <>
+struct FftSel<128, false> {
+  using type = Fft3<128, 16>;
+};
 template <int N>
 using FftFor = typename FftSel<N>::type;
 

@@ -219,7 +219,7 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
   // the prefetched field), so the barriers inside run() hide their latency
   // The three-level transforms (N >= 256) need more registers, so field 0
   // is fetched after w's transform (three arrays live instead of four).
-  constexpr bool kEarly = N < 256;
+  constexpr bool kEarly = N < 256 || (PIN && N == 256);
   cpx<T> a[R], b[R], pf[R], w[R];
   load(1, a);
   load(2, b);
