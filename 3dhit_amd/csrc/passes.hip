@@ -361,7 +361,10 @@ __global__ __launch_bounds__(LeanShape<N>::THREADS) __attribute__((amdgpu_waves_
     for (int s = 0; s < R; ++s) st_off(dst, cj + uint32_t(F::out_index(qv, s)) * rs, v[s]);
   };
   auto fft = [&](cpx<T>* v) { const int qv = opaque(q); F::template run<+1>(v, qv, L, Twiddles<F, T, true>(qv)); };
-  cpx<T> a[R], b[R], w[R];
+  // pruned inputs (12 live values per field) leave room to issue u1's loads
+  // one transform ahead of its use (before u2's transform); otherwise after
+  constexpr bool kEarlyU1 = PIN;
+  cpx<T> a[R], b[R], c1[R], w[R];
   load(0, a);
   load(2, b);
   {
@@ -371,18 +374,19 @@ __global__ __launch_bounds__(LeanShape<N>::THREADS) __attribute__((amdgpu_waves_
   }
   fft(w);
   store(3, w);
+  if constexpr (kEarlyU1) load(1, c1);
   fft(b);
   store(2, b);
-  load(1, b);
+  if constexpr (!kEarlyU1) load(1, c1);
   {
     const int qv = opaque(q);
 #pragma unroll
-    for (int r = 0; r < R; ++r) w[r] = mul_i(kx(qv, r) * b[r] - kyv * a[r]);   // w2
+    for (int r = 0; r < R; ++r) w[r] = mul_i(kx(qv, r) * c1[r] - kyv * a[r]);   // w2
   }
   fft(w);
   store(4, w);
-  fft(b);
-  store(1, b);
+  fft(c1);
+  store(1, c1);
   fft(a);
   store(0, a);
 }
