@@ -1389,8 +1389,7 @@ int hit3d_conv_forward(const hit3d_conv* conv, const void* x, const void* wp, co
   H3_TRY(check_ptr(wp, "wp"));
   H3_TRY(check_ptr(bias, "bias"));
   if (gelu) {
-    H3_TRY(check_ptr(pre, "pre"));
-    H3_TRY(check_ptr(act, "act"));
+    H3_TRY(check_ptr(act, "act"));   // pre may be NULL: not kept
   } else {
     H3_TRY(check_ptr(out, "out"));
   }
@@ -1557,6 +1556,31 @@ int hit3d_stress_divergence(const hit3d_grid* grid, const void* in, void* out, i
   else
     H3_HIP(h3::launch_stress_div<double>(static_cast<const cpx<double>*>(in), static_cast<cpx<double>*>(out),
                                          g, adjoint ? 1 : 0, s));
+  return HIT3D_OK;
+}
+
+int hit3d_rk4_combine(const hit3d_grid* grid, int32_t stage, double dt, const void* u_hat,
+                      const void* k_hat, void* acc, void* out, void* stream) {
+  H3_TRY(check_grid(grid));
+  if (stage < 1 || stage > 4) return fail(HIT3D_ERR_INVALID, "stage must be 1, 2, 3 or 4");
+  if (!std::isfinite(dt)) return fail(HIT3D_ERR_INVALID, "dt must be finite");
+  if (stage < 4) H3_TRY(check_ptr(u_hat, "u_hat"));
+  H3_TRY(check_ptr(k_hat, "k_hat"));
+  H3_TRY(check_ptr(acc, "acc"));
+  H3_TRY(check_ptr(out, "out"));
+  if (acc == out) return fail(HIT3D_ERR_INVALID, "acc and out must be different arrays");
+  const int64_t n = 3 * int64_t(grid->nx) * grid->ny * (grid->nz / 2 + 1);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (grid->precision == 4)
+    H3_HIP(h3::launch_rk4_combine<float>(static_cast<const cpx<float>*>(u_hat),
+                                         static_cast<const cpx<float>*>(k_hat),
+                                         static_cast<cpx<float>*>(acc), static_cast<cpx<float>*>(out),
+                                         n, stage, dt, s));
+  else
+    H3_HIP(h3::launch_rk4_combine<double>(static_cast<const cpx<double>*>(u_hat),
+                                          static_cast<const cpx<double>*>(k_hat),
+                                          static_cast<cpx<double>*>(acc),
+                                          static_cast<cpx<double>*>(out), n, stage, dt, s));
   return HIT3D_OK;
 }
 

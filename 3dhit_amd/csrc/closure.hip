@@ -120,7 +120,8 @@ __global__ __launch_bounds__(64) void k_conv_fwd(const __bf16* __restrict__ X,
 #pragma unroll
       for (int e = 0; e < 4; ++e) y[e] = acc[r][b][e] + (co0 + e < cout_real ? bias[co0 + e] : 0.f);
       if (mode == 0) {
-        *reinterpret_cast<float4*>(pre + int64_t(v) * COUT + co0) = make_float4(y[0], y[1], y[2], y[3]);
+        if (pre != nullptr)   // kept for a backward pass only
+          *reinterpret_cast<float4*>(pre + int64_t(v) * COUT + co0) = make_float4(y[0], y[1], y[2], y[3]);
         __bf16 h[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) h[e] = (__bf16)(co0 + e < cout_real ? gelu(y[e]) : 0.f);
@@ -142,7 +143,10 @@ __global__ __launch_bounds__(64) void k_conv_fwd(const __bf16* __restrict__ X,
 // is fetched from memory by one workgroup (plus the halo overlap) instead of
 // by the waves of its 27 neighbours, and the weights once per workgroup
 // instead of once per wave. Weight rows are padded by 16 bytes so that the
-// 16 rows one A read touches fall in distinct banks. Needs D % 4 == 0,
+// 16 rows one A read touches fall in distinct banks, and so are the halo's
+// voxel rows (CIN/8 + 1 chunks of 16 bytes): the 16 consecutive voxels of a
+// B read then start 80 (48) bytes apart instead of 64 (32), which puts any 8
+// of them in distinct banks. Needs D % 4 == 0,
 // H % 8 == 0, W % 16 == 0. Same products and summation order as k_conv_fwd.
 // ---------------------------------------------------------------------------
 constexpr int kTileA = 4, kTileB = 8, kTileC = 16, kTileWaves = 8;
@@ -160,13 +164,14 @@ __global__ __launch_bounds__(64 * kTileWaves) void k_conv_fwd_tiled(const __bf16
   constexpr int RB = COUT / 16;
   constexpr int KS = (27 * CIN + 31) / 32;
   constexpr int CHUNKS = CIN / 8;                    // 16-byte chunks per voxel row
+  constexpr int CSTR = CHUNKS + 1;                   // padded voxel row in LDS
   constexpr int HALO = kHaloA * kHaloB * kHaloC;
   constexpr int WROW = 4 * KS;                       // 16-byte chunks per weight row
   constexpr int WROWP = WROW + 1;                    // padded row
   constexpr int NHALO = HALO * CHUNKS, NW = COUT * WROW;
   constexpr int NLOAD = (NHALO + NW + NT - 1) / NT;
   static_assert(CIN == 16 || CIN == 32, "CIN must be 16 or 32");
-  __shared__ bf16x8 tile[NHALO];
+  __shared__ bf16x8 tile[HALO * CSTR];
   __shared__ bf16x8 wts[COUT * WROWP];
   const int tid = threadIdx.x, wave = tid / 64, l = tid % 64, li = l % 16, lg = l / 16;
   const int nc = g.W / kTileC, nbk = g.H / kTileB;
@@ -192,7 +197,7 @@ __global__ __launch_bounds__(64 * kTileWaves) void k_conv_fwd_tiled(const __bf16
   for (int j = 0; j < NLOAD; ++j) {
     const int i = tid + NT * j;
     if (i < NHALO) {
-      tile[i] = stage[j];
+      tile[(i / CHUNKS) * CSTR + i % CHUNKS] = stage[j];
     } else if (i < NHALO + NW) {
       const int w = i - NHALO;
       wts[(w / WROW) * WROWP + w % WROW] = stage[j];
@@ -215,7 +220,7 @@ __global__ __launch_bounds__(64 * kTileWaves) void k_conv_fwd_tiled(const __bf16
     for (int b = 0; b < 4; ++b) {
       if (o < 27) {
         const int t = ((wa + da) * kHaloB + (wb + b + db)) * kHaloC + (li + dc);
-        B[b] = tile[t * CHUNKS + ci / 8];
+        B[b] = tile[t * CSTR + ci / 8];
       } else {
         B[b] = bf16x8{};
       }
@@ -243,7 +248,8 @@ __global__ __launch_bounds__(64 * kTileWaves) void k_conv_fwd_tiled(const __bf16
 #pragma unroll
       for (int e = 0; e < 4; ++e) y[e] = acc[r][b][e] + (co0 + e < cout_real ? bias[co0 + e] : 0.f);
       if (mode == 0) {
-        *reinterpret_cast<float4*>(pre + int64_t(v) * COUT + co0) = make_float4(y[0], y[1], y[2], y[3]);
+        if (pre != nullptr)   // kept for a backward pass only
+          *reinterpret_cast<float4*>(pre + int64_t(v) * COUT + co0) = make_float4(y[0], y[1], y[2], y[3]);
         __bf16 h[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) h[e] = (__bf16)(co0 + e < cout_real ? gelu(y[e]) : 0.f);
@@ -468,12 +474,39 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ gr, floa
   }
 }
 
+// The pack (dir 0) runs one voxel per thread with 16-byte stores when the
+// channel groups of 8 start on 16-byte boundaries.
+__host__ __device__ inline bool pack_by_voxel(int dir, int CP, const __bf16* aos_bf) {
+  return dir == 0 && (CP & 7) == 0 && (reinterpret_cast<uintptr_t>(aos_bf) & 15) == 0;
+}
+
 // SoA fp32 [C][V] (scaled by s[c]) <-> channels-last [V][CP]: dir 0 writes
 // bf16 channels-last (zero channels >= C), dir 1 reads fp32 channels-last
 // (ld = its channel stride) into SoA.
+
 __global__ void k_soa_aos(const float* __restrict__ soa, int C, int64_t V, const float* __restrict__ s,
                           int CP, int dir, __bf16* __restrict__ aos_bf, const float* __restrict__ aos_f,
                           float* __restrict__ soa_out) {
+  if (pack_by_voxel(dir, CP, aos_bf)) {
+    // one voxel per thread: channel reads coalesce across the voxels, and
+    // the voxel's CP bf16 values leave as 16-byte stores
+    for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V;
+         v += int64_t(gridDim.x) * blockDim.x) {
+      for (int c8 = 0; c8 < CP; c8 += 8) {
+        union {
+          __bf16 h[8];
+          uint4 u;
+        } pk;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int ch = c8 + j;
+          pk.h[j] = (__bf16)(ch < C ? soa[ch * V + v] * s[ch] : 0.f);
+        }
+        *reinterpret_cast<uint4*>(aos_bf + v * CP + c8) = pk.u;
+      }
+    }
+    return;
+  }
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < V * CP;
        i += int64_t(gridDim.x) * blockDim.x) {
     const int64_t v = i / CP;
@@ -699,7 +732,8 @@ hipError_t launch_adam(float* p, const float* gr, float* m, float* v, int64_t n,
 
 hipError_t launch_soa_aos(const float* soa, int C, int64_t V, const float* scale, int CP, int dir,
                           __bf16* aos_bf, const float* aos_f, float* soa_out, hipStream_t s) {
-  hipLaunchKernelGGL(k_soa_aos, dim3(grid_for(V * CP)), dim3(256), 0, s, soa, C, V, scale, CP, dir, aos_bf,
+  const int64_t n = pack_by_voxel(dir, CP, aos_bf) ? V : V * CP;   // threads the kernel's loop takes
+  hipLaunchKernelGGL(k_soa_aos, dim3(grid_for(n)), dim3(256), 0, s, soa, C, V, scale, CP, dir, aos_bf,
                      aos_f, soa_out);
   return hipGetLastError();
 }

@@ -673,6 +673,31 @@ __global__ __launch_bounds__(kBlock) void k_hold_apply(const cpx<T>* __restrict_
   out[i] = r;
 }
 
+// One RK4 combination step over n complex values (integrators.py:32-38's
+// stage sums, with the stage derivative k read once):
+//   stage 1: out = u + dt/2 k, acc = u + dt/6 k
+//   stage 2: out = u + dt/2 k, acc += dt/3 k
+//   stage 3: out = u + dt k,   acc += dt/3 k
+//   stage 4: out = acc + dt/6 k (u unused)
+// c_out and c_acc are the stage's coefficients in the state's precision.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_rk4_combine(const cpx<T>* u, const cpx<T>* k,
+                                                        cpx<T>* __restrict__ acc, cpx<T>* out,
+                                                        int64_t n, int stage, T c_out, T c_acc) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const cpx<T> kv = k[i];
+  if (stage == 4) {
+    const cpx<T> a = acc[i];
+    out[i] = {a.x + c_acc * kv.x, a.y + c_acc * kv.y};
+    return;
+  }
+  const cpx<T> x = u[i];
+  const cpx<T> a = stage == 1 ? x : acc[i];
+  acc[i] = {a.x + c_acc * kv.x, a.y + c_acc * kv.y};
+  out[i] = {x.x + c_out * kv.x, x.y + c_out * kv.y};
+}
+
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_adj_combine(const cpx<T>* __restrict__ AB,
                                                         const cpx<T>* v,
@@ -912,6 +937,16 @@ hipError_t launch_hold_apply(const cpx<T>* u, const cpx<T>* du, cpx<T>* out, int
 }
 
 template <class T>
+hipError_t launch_rk4_combine(const cpx<T>* u, const cpx<T>* k, cpx<T>* acc, cpx<T>* out, int64_t n,
+                              int stage, double dt, hipStream_t s) {
+  const double co = stage == 3 ? dt : 0.5 * dt;
+  const double ca = (stage == 1 || stage == 4) ? dt / 6.0 : dt / 3.0;
+  hipLaunchKernelGGL((k_rk4_combine<T>), dim3(nblocks_for(n)), dim3(kBlock), 0, s, u, k, acc, out, n,
+                     stage, T(co), T(ca));
+  return hipGetLastError();
+}
+
+template <class T>
 hipError_t launch_adj_combine(const cpx<T>* AB, const cpx<T>* v, const cpx<T>* u, cpx<T>* out,
                               int64_t fs, const Geo& g, T kmin2, T kmax2, const AdjCombine& c,
                               const AdjEpilogue<T>& ep, hipStream_t s) {
@@ -957,7 +992,9 @@ hipError_t launch_axpby(cpx<T>* out, T a, const cpx<T>* x, T b, const cpx<T>* y,
                                            int, const double*, double, double*, double*,          \
                                            hipStream_t);                                           \
   template hipError_t launch_hold_apply<T>(const cpx<T>*, const cpx<T>*, cpx<T>*, int64_t,        \
-                                           const double*, hipStream_t);
+                                           const double*, hipStream_t);                            \
+  template hipError_t launch_rk4_combine<T>(const cpx<T>*, const cpx<T>*, cpx<T>*, cpx<T>*,        \
+                                            int64_t, int, double, hipStream_t);
 
 H3_INSTANTIATE_ADJ(float)
 H3_INSTANTIATE_ADJ(double)

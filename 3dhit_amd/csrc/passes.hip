@@ -10,6 +10,7 @@
 // z c2r / r2c kernels.
 #include "passes.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace h3 {
@@ -80,15 +81,20 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
     for (int r = 0; r < R; ++r)
       v[r] = !((kZeroIn >> r) & 1u) && irow[r] >= 0 ? src[int64_t(irow[r]) * a.B] : czero<T>();
   };
+  // this block's fields [f0, f1): the launcher splits the fields over
+  // gridDim.y when the column tiles alone would leave most CUs idle
+  const int fpb = (a.nf + int(gridDim.y) - 1) / int(gridDim.y);
+  const int f0 = int(blockIdx.y) * fpb, f1 = min(a.nf, f0 + fpb);
+  if (f0 >= f1) return;   // uniform per block
   // field f+1 is loaded before field f's transform, whose barriers would
   // otherwise hold the loads back
   cpx<T> nxt[R];
-  load(0, nxt);
-  for (int f = 0; f < a.nf; ++f) {
+  load(f0, nxt);
+  for (int f = f0; f < f1; ++f) {
     cpx<T> v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = nxt[r];
-    if (f + 1 < a.nf) load(f + 1, nxt);
+    if (f + 1 < f1) load(f + 1, nxt);
     F::template run<DIR>(v, q, L, tw);
     cpx<T>* dst = out0 + f * a.out_fs;
 #pragma unroll
@@ -709,6 +715,18 @@ __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_fused(cpx<T>* buf
   }
 }
 
+// Fields [f0, f1) of the standalone z passes' block, whole pairs of fields
+// (one complex transform each): the launchers split the pairs over
+// gridDim.y when the pencils alone would leave most CUs idle.
+struct FieldPairs {
+  int f0, f1;
+  __device__ __forceinline__ explicit FieldPairs(int nf) {
+    const int pairs = (nf + 1) / 2, ppb = (pairs + int(gridDim.y) - 1) / int(gridDim.y);
+    f0 = 2 * ppb * int(blockIdx.y);
+    f1 = min(nf, f0 + 2 * ppb);
+  }
+};
+
 template <int N, class T>
 __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_c2r(const cpx<T>* __restrict__ in, int64_t in_fs,
                                                    T* __restrict__ out, int64_t out_fs, int nf,
@@ -726,7 +744,8 @@ __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_c2r(const cpx<T>*
   const int kzl = active ? kz : 0;
   cpx<T> tw[R];
   F::load_tw(tw, q);
-  for (int f = 0; f < nf; f += 2) {
+  const FieldPairs fp(nf);
+  for (int f = fp.f0; f < fp.f1; f += 2) {
     const bool two = f + 1 < nf;
     cpx<T> z[R];
 #pragma unroll
@@ -763,7 +782,8 @@ __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_r2c(const T* __re
   SlotPencil<T> L{lp};
   cpx<T> tw[R];
   F::load_tw(tw, q);
-  for (int f = 0; f < nf; f += 2) {
+  const FieldPairs fp(nf);
+  for (int f = fp.f0; f < fp.f1; f += 2) {
     const bool two = f + 1 < nf;
     cpx<T> z[R];
 #pragma unroll
@@ -1115,6 +1135,16 @@ __global__ __launch_bounds__((StageShape<N, LEAN>::THREADS)) __attribute__((amdg
 
 static inline unsigned ceil_div(int64_t a, int64_t b) { return unsigned((a + b - 1) / b); }
 
+// Blocks along y for a pass whose nb column (pencil) blocks each loop over
+// `items` fields (field pairs): enough to reach about `want` blocks when nb
+// is under half of it, with no y-block left empty.
+static inline unsigned split_count(unsigned nb, int items, unsigned want) {
+  if (items <= 1 || nb == 0 || nb >= want / 2) return 1;
+  const unsigned k = std::min<unsigned>(unsigned(items), (want + nb - 1) / nb);
+  const unsigned per = (unsigned(items) + k - 1) / k;
+  return (unsigned(items) + per - 1) / per;
+}
+
 // The lean kernels address with 32-bit byte offsets inside a field.
 static inline bool fits_u32(int64_t max_elem, size_t esize) {
   return max_elem >= 0 && uint64_t(max_elem + 1) * esize <= (uint64_t(1) << 32);
@@ -1153,7 +1183,8 @@ hipError_t launch_strided(int N, int dir, const StridedArgs<T>& a, hipStream_t s
   }
   const bool pin = pruned_rows(N, a.in_k, a.in_p), pout = pruned_rows(N, a.out_k, a.out_p);
   H3_DISPATCH_N(N, {
-    const dim3 grid(ceil_div(a.J, StridedShape<NN>::TC));
+    const unsigned nb = ceil_div(a.J, StridedShape<NN>::TC);
+    const dim3 grid(nb, split_count(nb, a.nf, 1024));
     const dim3 blk(StridedShape<NN>::THREADS);
     if constexpr (kPrunedN<NN>) {
       if (pin && dir > 0) hipLaunchKernelGGL((k_strided<NN, +1, T, true, false>), grid, blk, 0, s, a);
@@ -1260,7 +1291,7 @@ hipError_t launch_z_c2r(int N, const cpx<T>* in, int64_t in_fs, T* out, int64_t 
                         int64_t npencils, int kz, T scale, hipStream_t s) {
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(npencils, PencilBlock<NN>::PW);
-    hipLaunchKernelGGL((k_z_c2r<NN, T>), dim3(nb), dim3(PencilBlock<NN>::THREADS), 0, s, in, in_fs, out, out_fs,
+    hipLaunchKernelGGL((k_z_c2r<NN, T>), dim3(nb, split_count(nb, (nf + 1) / 2, 2048)), dim3(PencilBlock<NN>::THREADS), 0, s, in, in_fs, out, out_fs,
                        nf, npencils, kz, scale);
   });
   return hipGetLastError();
@@ -1271,7 +1302,7 @@ hipError_t launch_z_r2c(int N, const T* in, int64_t in_fs, cpx<T>* out, int64_t 
                         int64_t npencils, int kz, hipStream_t s) {
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(npencils, PencilBlock<NN>::PW);
-    hipLaunchKernelGGL((k_z_r2c<NN, T>), dim3(nb), dim3(PencilBlock<NN>::THREADS), 0, s, in, in_fs, out, out_fs,
+    hipLaunchKernelGGL((k_z_r2c<NN, T>), dim3(nb, split_count(nb, (nf + 1) / 2, 2048)), dim3(PencilBlock<NN>::THREADS), 0, s, in, in_fs, out, out_fs,
                        nf, npencils, kz);
   });
   return hipGetLastError();

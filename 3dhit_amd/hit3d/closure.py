@@ -64,15 +64,17 @@ class ConvLayer:
         _lib.call("hit3d_conv_pack", self.d, self.w.data_ptr(), self.wp.data_ptr(),
                   self.wt.data_ptr(), stream())
 
-    def forward(self, x):
+    def forward(self, x, keep_pre=True):
         """x bf16 [V][cin_pad] -> (pre fp32 [V][cout_pad], act bf16 [V][cout_pad])
-        with GELU, else out fp32 [V][cout]."""
+        with GELU (pre None unless keep_pre: only a backward pass reads it),
+        else out fp32 [V][cout]."""
         dev = x.device
         if self.gelu:
-            pre = torch.empty(self.V, self.desc.cout_pad, device=dev)
+            pre = torch.empty(self.V, self.desc.cout_pad, device=dev) if keep_pre else None
             act = torch.empty(self.V, self.desc.cout_pad, dtype=torch.bfloat16, device=dev)
             _lib.call("hit3d_conv_forward", self.d, x.data_ptr(), self.wp.data_ptr(),
-                      self.b.data_ptr(), 1, pre.data_ptr(), act.data_ptr(), None, stream())
+                      self.b.data_ptr(), 1, None if pre is None else pre.data_ptr(), act.data_ptr(),
+                      None, stream())
             return pre, act
         out = torch.empty(self.V, self.cout, device=dev)
         _lib.call("hit3d_conv_forward", self.d, x.data_ptr(), self.wp.data_ptr(), self.b.data_ptr(),
@@ -168,14 +170,15 @@ class ClosureNet:
         self.in_scale = in_scale.float().to(self.in_scale.device).contiguous()
         self.out_scale = out_scale.float().to(self.out_scale.device).contiguous()
 
-    def forward_saved(self, g):
-        """(tau fp32 [6][V], saved tensors for `backward_saved`)."""
+    def forward_saved(self, g, keep=True):
+        """(tau fp32 [6][V], saved tensors for `backward_saved`); keep=False
+        skips the pre-activations that only the backward pass reads."""
         g = g.contiguous()
         x0 = torch.empty(self.V, 16, dtype=torch.bfloat16, device=g.device)
         _lib.call("hit3d_channels_pack", g.data_ptr(), 9, self.V, self.in_scale.data_ptr(), 16,
                   x0.data_ptr(), stream())
-        p1, a1 = self.layers[0].forward(x0)
-        p2, a2 = self.layers[1].forward(a1)
+        p1, a1 = self.layers[0].forward(x0, keep)
+        p2, a2 = self.layers[1].forward(a1, keep)
         out = self.layers[2].forward(a2)
         tau = torch.empty(6, self.V, device=g.device)
         keep = None
@@ -189,7 +192,8 @@ class ClosureNet:
         return tau, (x0, p1, a1, p2, a2, keep)
 
     def __call__(self, g):
-        return self.forward_saved(g)[0]
+        """tau for inference (nothing kept for a backward pass)."""
+        return self.forward_saved(g, keep=False)[0]
 
     def backward_saved(self, saved, grad_tau, need_grad_g=True):
         """Accumulates the parameter gradients; returns d/dg (fp32 [9][V])."""

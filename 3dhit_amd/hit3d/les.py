@@ -351,8 +351,15 @@ def les_rhs(u, rhs, params, net, t=0.0, mask=None):
     return du
 
 
-def les_step(u, rhs, params, net, dt, t=0.0, mask=None):
-    """Classic RK4 (integrators.py:32-38 structure) on the LES RHS."""
+def les_step(u, rhs, params, net, dt, t=0.0, mask=None, out=None):
+    """Classic RK4 (integrators.py:32-38 structure) on the LES RHS. Without
+    gradients each stage's sums are one library pass (hit3d_rk4_combine), and
+    `out` may name the array that receives the new state (u itself
+    included); with gradients they are tensor ops and `out` must be None."""
+    if not torch.is_grad_enabled():
+        return _les_step_fused(u, rhs, params, net, dt, t, mask, out)
+    if out is not None:
+        raise ValueError("out= is only taken by steps without gradients")
     k1 = les_rhs(u, rhs, params, net, t, mask)
     k2 = les_rhs(torch.add(u, k1, alpha=0.5 * dt), rhs, params, net, t + 0.5 * dt, mask)
     acc = torch.add(u, k1, alpha=dt / 6.0).add(k2, alpha=dt / 3.0)
@@ -360,6 +367,26 @@ def les_step(u, rhs, params, net, dt, t=0.0, mask=None):
     acc = acc.add(k3, alpha=dt / 3.0)
     k4 = les_rhs(torch.add(u, k3, alpha=dt), rhs, params, net, t + dt, mask)
     return acc.add(k4, alpha=dt / 6.0)
+
+
+def _les_step_fused(u, rhs, params, net, dt, t, mask, out):
+    grid = rhs.grid
+    u = check_spectral(u, grid, 3).contiguous()
+    if out is not None and (out.shape != u.shape or out.dtype != u.dtype or out.device != u.device
+                            or not out.is_contiguous()):
+        raise ValueError("out must be a contiguous array shaped and typed like u")
+    us, acc = torch.empty_like(u), torch.empty_like(u)
+    new = torch.empty_like(u) if out is None else out
+
+    def combine(stage, k, dst):
+        _lib.call("hit3d_rk4_combine", grid.desc, stage, float(dt), u.data_ptr(), k.data_ptr(),
+                  acc.data_ptr(), dst.data_ptr(), stream())
+
+    combine(1, les_rhs(u, rhs, params, net, t, mask), us)
+    combine(2, les_rhs(us, rhs, params, net, t + 0.5 * dt, mask), us)
+    combine(3, les_rhs(us, rhs, params, net, t + 0.5 * dt, mask), us)
+    combine(4, les_rhs(us, rhs, params, net, t + dt, mask), new)   # reads acc and k only
+    return new
 
 
 class LesStepGraph:
@@ -380,11 +407,10 @@ class LesStepGraph:
         torch.cuda.current_stream().wait_stream(side)
         self.graph = torch.cuda.CUDAGraph()
         with torch.no_grad(), torch.cuda.graph(self.graph):
-            self.out = les_step(self.u, rhs, params, net, dt, 0.0, mask)
+            les_step(self.u, rhs, params, net, dt, 0.0, mask, out=self.u)   # new state over the old
 
     def step(self):
         self.graph.replay()
-        self.u.copy_(self.out)
         return self.u
 
 

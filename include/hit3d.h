@@ -276,7 +276,9 @@ int hit3d_conv_sizes(const hit3d_conv* conv, int64_t* kp, int64_t* kt, size_t* w
 int hit3d_conv_pack(const hit3d_conv* conv, const float* w, void* wp, void* wt, void* stream);
 
 /* gelu = 1: pre fp32 [V][cout_pad] = conv(x) + bias, act bf16 [V][cout_pad] =
-   GELU(pre) (exact erf form; zero beyond cout). gelu = 0: out fp32 [V][cout]. */
+   GELU(pre) (exact erf form; zero beyond cout); pre may be NULL when no
+   backward pass follows (it is then not written). gelu = 0: out fp32
+   [V][cout]. */
 int hit3d_conv_forward(const hit3d_conv* conv, const void* x, const void* wp, const float* bias,
                        int32_t gelu, float* pre, void* act, float* out, void* stream);
 
@@ -344,6 +346,19 @@ int hit3d_les_combine(const hit3d_grid* grid, const void* du_hat, const void* ta
 int hit3d_hold_energy(const hit3d_grid* grid, const void* u_hat, const void* du_hat, void* out,
                       double energy, double tau, double* coef, void* work, size_t work_bytes,
                       void* stream);
+
+/* One combination step of classic RK4 over full-layout states [3]
+   (integrators.py:32-38, whose stage sums the Python layer otherwise forms
+   with separate tensor adds):
+     stage 1: out = u + dt/2 k, acc = u + dt/6 k
+     stage 2: out = u + dt/2 k, acc = acc + dt/3 k
+     stage 3: out = u + dt k,   acc = acc + dt/3 k
+     stage 4: out = acc + dt/6 k (u_hat unused, may be NULL)
+   k_hat is the stage's derivative, out the next stage's input (stage 4: the
+   new state). Elementwise, so out may alias u_hat or k_hat; acc must not
+   alias out. */
+int hit3d_rk4_combine(const hit3d_grid* grid, int32_t stage, double dt, const void* u_hat,
+                      const void* k_hat, void* acc, void* out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -88,6 +88,10 @@ def test_conv_forward_tiled_and_fallback_shapes(dhw, cin, cout):
     ref = torch_conv(x.float(), bf(L.w), L.b, dhw)
     got = L.forward(x)[0][:, :cout] if cout == 32 else L.forward(x)
     assert rel(got, ref) < 1e-5
+    if cout == 32:   # inference form: no pre-activation kept, the same activations
+        pre, act = L.forward(x)
+        none, act2 = L.forward(x, keep_pre=False)
+        assert none is None and torch.equal(act, act2)
 
 
 def _torch_grads(x, w, b, gout_pre):

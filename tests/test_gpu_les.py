@@ -12,8 +12,9 @@ import hit3d_oracle as O
 pytestmark = pytest.mark.gpu
 
 h3 = pytest.importorskip("hit3d")
-from hit3d import les
+from hit3d import _lib, les
 from hit3d.closure import ClosureNet
+from hit3d.grid import stream
 
 
 def rdot(a, b):
@@ -384,3 +385,72 @@ def test_les_with_the_committed_closure_reproduces_the_filtered_dns_spectrum():
     # measured on an MI355X: 0.113 (k = 3..21) and 0.120 (k = 1, 2)
     assert err[2:].max() < 0.17 and err[:2].max() < 0.18
     assert err_bare[-1] > 1.0 and err_bare[-1] > 5 * err[-1]
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_rk4_combine_matches_the_stage_sums(dtype):
+    """hit3d_rk4_combine against classic RK4's stage sums (integrators.py:32-38)
+    evaluated in float64 from the same inputs; a wrong stage coefficient, a
+    stage 2/3 that restarts acc, or a stage 4 that reads u fails."""
+    g = h3.make_grid(16, dtype=dtype)
+    gen = torch.Generator().manual_seed(5)
+    shape = (3, 16, 16, 9)
+    cd = g.complex_dtype
+    u, k, acc0 = (crand(shape, gen, cd) for _ in range(3))
+    dt = 0.37
+    tol = 1e-6 if dtype == torch.float32 else 1e-15
+    want = {1: (u + dt / 2 * k, u + dt / 6 * k), 2: (u + dt / 2 * k, acc0 + dt / 3 * k),
+            3: (u + dt * k, acc0 + dt / 3 * k), 4: (acc0 + dt / 6 * k, acc0)}
+    for stage, (w_out, w_acc) in want.items():
+        acc = acc0.clone()
+        out = torch.empty_like(u)
+        _lib.call("hit3d_rk4_combine", g.desc, stage, dt, None if stage == 4 else u.data_ptr(),
+                  k.data_ptr(), acc.data_ptr(), out.data_ptr(), stream())
+        for got, ref in ((out, w_out), (acc, w_acc)):
+            err = float((got.to(torch.complex128) - ref.to(torch.complex128)).abs().max())
+            assert err <= tol * float(ref.abs().max()), (stage, err)
+    with pytest.raises(ValueError):
+        _lib.call("hit3d_rk4_combine", g.desc, 5, dt, u.data_ptr(), k.data_ptr(), acc.data_ptr(),
+                  out.data_ptr(), stream())
+    with pytest.raises(ValueError):
+        _lib.call("hit3d_rk4_combine", g.desc, 2, dt, u.data_ptr(), k.data_ptr(), acc.data_ptr(),
+                  acc.data_ptr(), stream())
+
+
+def test_les_step_without_gradients_matches_the_tensor_op_step():
+    """The no-gradient LES step (fused right-hand side, one combination pass
+    per stage, new state written over u) against the differentiable step built
+    from tensor ops: three steps without the closure, and one step with it.
+    With the closure the two forms differ by more than round-off: their stage
+    inputs differ in the last bit (one rounding in the combination pass, two
+    in the tensor ops), and the BF16 rounding of the closure's input turns
+    that into a difference of one BF16 unit (2^-8) at some voxels of an
+    untrained net's force (7.9e-5 relative after one step on an MI355X); a
+    wrong stage coefficient or a dropped stage moves the state by O(dt)."""
+    g = h3.make_grid(32, dtype=torch.float32)
+    fine = h3.make_grid(64, dtype=torch.float32)
+    u0 = les.filter_state(h3.random_field(4, fine, h3.config_spectrum), fine, g, 10.0)
+    rhs = h3.make_ns_rhs(g)
+    net = ClosureNet(g, channels=16, seed=2)
+    mask = les.sphere_mask(g, 10.0)
+    params = h3.NSParams(nu=1 / 500, forcing=les.HoldResolvedEnergy(0.3, 1.0))
+
+    def rel(a, b):
+        return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+    ref, got = u0.clone(), u0.clone()
+    for n in range(3):
+        with torch.enable_grad():
+            ref = les.les_step(ref, rhs, params, None, 4e-3, n * 4e-3, mask).detach()
+        with torch.no_grad():
+            same = les.les_step(got, rhs, params, None, 4e-3, n * 4e-3, mask, out=got)
+        assert same.data_ptr() == got.data_ptr()
+    assert rel(got, ref) < 5e-6, rel(got, ref)
+    assert rel(got, u0) > 1e-3                     # the steps moved the state
+    with torch.enable_grad():
+        ref = les.les_step(u0, rhs, params, net, 4e-3, 0.0, mask).detach()
+    with torch.no_grad():
+        got = les.les_step(u0, rhs, params, net, 4e-3, 0.0, mask)
+    assert rel(got, ref) < 3e-4, rel(got, ref)
+    with torch.enable_grad(), pytest.raises(ValueError):
+        les.les_step(got, rhs, params, net, 4e-3, 0.0, mask, out=got)
