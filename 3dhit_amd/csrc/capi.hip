@@ -288,18 +288,23 @@ int rhs_passes(const hit3d_grid& gd, const Geo& g, const cpx<T>* us, cpx<T>* buf
   const int64_t b_fs = int64_t(g.nx) * g.ny * g.kzs;
   H3_HIP(h3::launch_x_inverse_curl<T>(g.nx, us, st_fs, bufA, a_fs, g, s, job));
   mark(tm, kP1);
-  H3_HIP(h3::launch_y_inverse_curl<T>(g.ny, bufA, a_fs, bufB, b_fs, g, s));
+  const bool ymajor = h3::pencil_ymajor<T>(g);
+  H3_HIP(h3::launch_y_inverse_curl<T>(g.ny, bufA, a_fs, bufB, b_fs, g, ymajor, s));
   mark(tm, kP2);
   const double n = double(gd.nx) * double(gd.ny) * double(gd.nz);
-  H3_HIP(h3::launch_z_fused<T>(g.nz, bufB, b_fs, int64_t(g.nx) * g.ny, g.kz, g.kzs,
-                               T(1.0 / (n * n)), s));
+  H3_HIP(h3::launch_z_fused<T>(g.nz, bufB, b_fs, g.nx, g.ny, g.kz, g.kzs, ymajor, T(1.0 / (n * n)), s));
   mark(tm, kP3);
   h3::StridedArgs<T> y{};
   y.in = bufB + h3::kZOut * b_fs;
   y.out = bufA;
   y.in_fs = b_fs;
   y.out_fs = a_fs;
-  y.in_slab = int64_t(g.ny) * g.kzs;
+  if (ymajor) {   // column j = x kzs + iz, rows nx kzs apart
+    y.in_slab = g.kzs;
+    y.in_rs = int64_t(g.nx) * g.kzs;
+  } else {
+    y.in_slab = int64_t(g.ny) * g.kzs;
+  }
   y.out_slab = g.jp;
   y.B = g.kzs;
   y.J = g.nx * g.kzs;

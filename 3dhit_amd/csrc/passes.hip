@@ -75,11 +75,12 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
   int irow[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) irow[r] = active ? compact_of_row(q + Q * r, N, a.in_k, a.in_p) : -1;
+  const int64_t irs = a.in_row(), ors = a.out_row();
   auto load = [&](int f, cpx<T>* v) {
     const cpx<T>* src = in0 + f * a.in_fs;
 #pragma unroll
     for (int r = 0; r < R; ++r)
-      v[r] = !((kZeroIn >> r) & 1u) && irow[r] >= 0 ? src[int64_t(irow[r]) * a.B] : czero<T>();
+      v[r] = !((kZeroIn >> r) & 1u) && irow[r] >= 0 ? src[int64_t(irow[r]) * irs] : czero<T>();
   };
   // this block's fields [f0, f1): the launcher splits the fields over
   // gridDim.y when the column tiles alone would leave most CUs idle
@@ -101,7 +102,7 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
     for (int s = 0; s < R; ++s) {
       if ((kDeadOut >> s) & 1u) continue;
       const int o = compact_of_row(F::out_index(q, s), N, a.out_k, a.out_p);
-      if (active && o >= 0) dst[int64_t(o) * a.B] = v[s];
+      if (active && o >= 0) dst[int64_t(o) * ors] = v[s];
     }
   }
 }
@@ -186,7 +187,7 @@ template <int N, class T, bool PIN = false>
 __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(MinWaves<N, T>::value))) void k_y_inverse_curl(const cpx<T>* __restrict__ in,
                                                             int64_t in_fs,
                                                             cpx<T>* __restrict__ out,
-                                                            int64_t out_fs, Geo g) {
+                                                            int64_t out_fs, Geo g, int ymajor) {
   using S = StridedShape<N>;
   using F = typename S::F;
   constexpr int R = S::R, Q = S::Q, TC = S::TC;
@@ -209,7 +210,10 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
     kyv[r] = wavenumber<T>(g.k0y, irow[r] >= 0 ? freq_of(irow[r], g.ky, g.py) : 0);
   }
   const cpx<T>* src = in + int64_t(x) * g.jp + iz;
-  cpx<T>* dst0 = out + int64_t(x) * g.ny * g.kzs + iz;
+  // pencil buffer rows: y-major (row y at y nx kzs, column j) or x-major
+  // (pencil (x, y) at (x ny + y) kzs)
+  cpx<T>* dst0 = out + (ymajor ? int64_t(jj) : int64_t(x) * g.ny * g.kzs + iz);
+  const int prs = ymajor ? g.nx * g.kzs : g.kzs;
   auto load = [&](int f, cpx<T>* v) {
     const cpx<T>* p = src + f * in_fs;
 #pragma unroll
@@ -219,7 +223,7 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
     if (!active) return;
     cpx<T>* p = dst0 + f * out_fs;
 #pragma unroll
-    for (int s = 0; s < R; ++s) p[F::out_index(q, s) * g.kzs] = v[s];
+    for (int s = 0; s < R; ++s) p[F::out_index(q, s) * prs] = v[s];
   };
   // each field's loads are issued one transform ahead of its use (pf holds
   // the prefetched field), so the barriers inside run() hide their latency
@@ -298,7 +302,7 @@ __global__ __launch_bounds__(LeanShape<N>::THREADS) __attribute__((amdgpu_waves_
   const int ao = jj / a.B, bo = jj - ao * a.B;
   const uint32_t cin = uint32_t(int64_t(ao) * a.in_slab + bo) * E;
   const uint32_t cout = uint32_t(int64_t(ao) * a.out_slab + bo) * E;
-  const uint32_t rs = uint32_t(a.B) * E;
+  const uint32_t rs_in = uint32_t(a.in_row()) * E, rs_out = uint32_t(a.out_row()) * E;
   SlotStrided<TC, T> L{lds + c};
   const Twiddles<F, T> tw(q);
   for (int f = 0; f < a.nf; ++f) {
@@ -308,7 +312,7 @@ __global__ __launch_bounds__(LeanShape<N>::THREADS) __attribute__((amdgpu_waves_
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int ir = compact_of_row(qv + Q * r, N, a.in_k, a.in_p);
-      v[r] = ((kZeroIn >> r) & 1u) || !(active && ir >= 0) ? czero<T>() : ld_off(src, cin + uint32_t(ir) * rs);
+      v[r] = ((kZeroIn >> r) & 1u) || !(active && ir >= 0) ? czero<T>() : ld_off(src, cin + uint32_t(ir) * rs_in);
     }
     F::template run<DIR>(v, qv, L, tw);
     cpx<T>* dst = a.out + f * a.out_fs;
@@ -316,7 +320,7 @@ __global__ __launch_bounds__(LeanShape<N>::THREADS) __attribute__((amdgpu_waves_
     for (int s = 0; s < R; ++s) {
       if ((kDeadOut >> s) & 1u) continue;
       const int o = compact_of_row(F::out_index(qv, s), N, a.out_k, a.out_p);
-      if (active && o >= 0) st_off(dst, cout + uint32_t(o) * rs, v[s]);
+      if (active && o >= 0) st_off(dst, cout + uint32_t(o) * rs_out, v[s]);
     }
   }
 }
@@ -415,7 +419,7 @@ __global__ __launch_bounds__(LeanShape<N>::THREADS) __attribute__((amdgpu_waves_
   const int x = jj / g.kzs, iz = jj - x * g.kzs;
   const T kzv = wavenumber<T>(g.k0z, iz);
   const uint32_t cin = uint32_t(int64_t(x) * g.jp + iz) * E;
-  const uint32_t cout = uint32_t(int64_t(x) * g.ny * g.kzs + iz) * E;
+  const uint32_t cout = uint32_t(int64_t(x) * g.ny * g.kzs + iz) * E;   // x-major pencil buffer
   const uint32_t rs = uint32_t(g.kzs) * E;
   SlotStrided<TC, T> L{lds + c};
   // every step takes its own opaque copy of q (see opaque())
@@ -572,8 +576,8 @@ __device__ __forceinline__ void split_pair(cpx<T> ak, cpx<T> bk, cpx<T>& X, cpx<
 // launch at 1024^3 against 9.6 ms in the order above).
 // ---------------------------------------------------------------------------
 template <int N, class T, bool PRUNED>
-__global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_fused(cpx<T>* buf, int64_t fs, int64_t npencils,
-                                                     int kz, int kzs, T scale) {
+__global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_fused(cpx<T>* buf, int64_t fs, int nx, int ny,
+                                                     int kz, int kzs, T scale, int ymajor) {
   using B = PencilBlock<N>;
   using F = typename B::F;
   static_assert(B::THREADS == 64, "WaveSync exchanges need one-wave blocks");
@@ -581,9 +585,26 @@ __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_fused(cpx<T>* buf
   constexpr int R = B::R, Q = B::Q, PW = B::PW;
   __shared__ cpx<T> lds[B::LDS];
   const int q = threadIdx.x % Q, p = threadIdx.x / Q;
-  const int64_t pp = int64_t(xcd_tile_rev(blockIdx.x, gridDim.x)) * PW + p;
-  const bool active = pp < npencils;
-  const int64_t base = (active ? pp : 0) * kzs;
+  // Workgroups walk the pencils x-major (the order in which the y-inverse
+  // pass wrote them), each XCD's run from its end. x-major storage: pencil
+  // pp = x ny + y at pp kzs. y-major storage (ymajor, nx % PW == 0): the
+  // workgroup takes the PW x-adjacent pencils (x0 + p, y), whose rows are
+  // contiguous, so a group of pencils covers whole lines.
+  const int64_t gi = xcd_tile_rev(blockIdx.x, gridDim.x);
+  int64_t base;
+  bool active;
+  if (ymajor) {
+    active = gi < int64_t(nx / PW) * ny;
+    // (walking the groups y-major, or in blocks of 4-16 x-groups per y, to
+    // keep consecutive workgroups on adjacent memory measured slower: the
+    // order the y-inverse pass wrote in matters more)
+    const int ga = active ? int(gi) : 0, gx = ga / ny, gy = ga - gx * ny;
+    base = (int64_t(gy) * nx + gx * PW + p) * kzs;
+  } else {
+    const int64_t pp = gi * PW + p;
+    active = pp < int64_t(nx) * ny;
+    base = (active ? pp : 0) * kzs;
+  }
   cpx<T>* lp = lds + p * F::kSlots;
   SlotPencil<T> L{lp};
   const int kzl = active ? kz : 0;  // inactive pencils load nothing
@@ -1167,8 +1188,8 @@ hipError_t launch_strided(int N, int dir, const StridedArgs<T>& a, hipStream_t s
   if (a.J <= 0 || a.nf <= 0) return hipSuccess;
   if constexpr (lean_type<T>()) {
     const int64_t ao = (a.J - 1) / a.B;
-    const bool fit = fits_u32(ao * a.in_slab + (a.B - 1) + int64_t(N - 1) * a.B, sizeof(cpx<T>)) &&
-                     fits_u32(ao * a.out_slab + (a.B - 1) + int64_t(N - 1) * a.B, sizeof(cpx<T>));
+    const bool fit = fits_u32(ao * a.in_slab + (a.B - 1) + int64_t(N - 1) * a.in_row(), sizeof(cpx<T>)) &&
+                     fits_u32(ao * a.out_slab + (a.B - 1) + int64_t(N - 1) * a.out_row(), sizeof(cpx<T>));
     if (N == 1024 && fit) {
       const bool pin = pruned_rows(N, a.in_k, a.in_p), pout = pruned_rows(N, a.out_k, a.out_p);
       if (dir < 0) {
@@ -1235,10 +1256,26 @@ hipError_t launch_x_inverse_curl(int N, const cpx<T>* us, int64_t us_fs, cpx<T>*
 }
 
 template <class T>
+bool pencil_ymajor(const Geo& g) {
+  // rows off 128-byte lines, a field small enough that rows nx kzs apart
+  // stay within few 2 MB pages, and whole groups of x-adjacent pencils
+  const int64_t row_bytes = int64_t(g.kzs) * int64_t(sizeof(cpx<T>));
+  const int64_t field_bytes = int64_t(g.nx) * g.ny * row_bytes;
+  int pw = 0;
+  switch (g.nz) {
+#define H3_PW(NN) case NN: pw = PencilBlock<NN>::PW; break;
+    H3_PW(4) H3_PW(8) H3_PW(16) H3_PW(32) H3_PW(64) H3_PW(128) H3_PW(256) H3_PW(512) H3_PW(1024)
+#undef H3_PW
+    default: return false;
+  }
+  return row_bytes % 128 != 0 && field_bytes <= (int64_t(256) << 20) && g.nx % pw == 0;
+}
+
+template <class T>
 hipError_t launch_y_inverse_curl(int N, const cpx<T>* in, int64_t in_fs, cpx<T>* out,
-                                 int64_t out_fs, const Geo& g, hipStream_t s) {
+                                 int64_t out_fs, const Geo& g, bool ymajor, hipStream_t s) {
   if constexpr (lean_type<T>()) {
-    if (N == 1024 && fits_u32(int64_t(g.nx) * g.jp, sizeof(cpx<T>)) &&
+    if (N == 1024 && !ymajor && fits_u32(int64_t(g.nx) * g.jp, sizeof(cpx<T>)) &&
         fits_u32(int64_t(g.nx) * g.ny * g.kzs, sizeof(cpx<T>))) {
       constexpr int NN = 1024;
       const unsigned nb = ceil_div(int64_t(g.nx) * g.kzs, LeanShape<NN>::TC);
@@ -1257,18 +1294,20 @@ hipError_t launch_y_inverse_curl(int N, const cpx<T>* in, int64_t in_fs, cpx<T>*
     const dim3 blk(StridedShape<NN>::THREADS);
     if constexpr (kPrunedN<NN>) {
       if (pin) {
-        hipLaunchKernelGGL((k_y_inverse_curl<NN, T, true>), grid, blk, 0, s, in, in_fs, out, out_fs, g);
+        hipLaunchKernelGGL((k_y_inverse_curl<NN, T, true>), grid, blk, 0, s, in, in_fs, out, out_fs, g,
+                           ymajor ? 1 : 0);
         break;
       }
     }
-    hipLaunchKernelGGL((k_y_inverse_curl<NN, T>), grid, blk, 0, s, in, in_fs, out, out_fs, g);
+    hipLaunchKernelGGL((k_y_inverse_curl<NN, T>), grid, blk, 0, s, in, in_fs, out, out_fs, g, ymajor ? 1 : 0);
   });
   return hipGetLastError();
 }
 
 template <class T>
-hipError_t launch_z_fused(int N, cpx<T>* buf, int64_t fs, int64_t npencils, int kz, int kzs,
-                          T scale, hipStream_t s) {
+hipError_t launch_z_fused(int N, cpx<T>* buf, int64_t fs, int nx, int ny, int kz, int kzs,
+                          bool ymajor, T scale, hipStream_t s) {
+  const int64_t npencils = int64_t(nx) * ny;
   // the pruned layout stores exactly kz = N/3 + 1 modes; anything else (the
   // dense layout, kz = N/2 + 1) takes the kernel without compile-time zeros
   // (and padding slots beyond kz, read as zeros, with 32-bit offsets inside
@@ -1278,10 +1317,10 @@ hipError_t launch_z_fused(int N, cpx<T>* buf, int64_t fs, int64_t npencils, int 
     const unsigned nb = ceil_div(npencils, PencilBlock<NN>::PW);
     if (pruned)
       hipLaunchKernelGGL((k_z_fused<NN, T, true>), dim3(nb), dim3(PencilBlock<NN>::THREADS), 0, s, buf, fs,
-                         npencils, kz, kzs, scale);
+                         nx, ny, kz, kzs, scale, ymajor ? 1 : 0);
     else
       hipLaunchKernelGGL((k_z_fused<NN, T, false>), dim3(nb), dim3(PencilBlock<NN>::THREADS), 0, s, buf, fs,
-                         npencils, kz, kzs, scale);
+                         nx, ny, kz, kzs, scale, ymajor ? 1 : 0);
   });
   return hipGetLastError();
 }
@@ -1394,8 +1433,9 @@ hipError_t launch_x_forward_stage(int N, int mode, const StageArgs<T>& a, hipStr
   template hipError_t launch_x_inverse_curl<T>(int, const cpx<T>*, int64_t, cpx<T>*, int64_t,   \
                                                const Geo&, hipStream_t, const FinalizeJob*);     \
   template hipError_t launch_y_inverse_curl<T>(int, const cpx<T>*, int64_t, cpx<T>*, int64_t,   \
-                                               const Geo&, hipStream_t);                         \
-  template hipError_t launch_z_fused<T>(int, cpx<T>*, int64_t, int64_t, int, int, T, hipStream_t); \
+                                               const Geo&, bool, hipStream_t);                         \
+  template hipError_t launch_z_fused<T>(int, cpx<T>*, int64_t, int, int, int, int, bool, T, hipStream_t); \
+  template bool pencil_ymajor<T>(const Geo&); \
   template hipError_t launch_z_c2r<T>(int, const cpx<T>*, int64_t, T*, int64_t, int, int64_t,   \
                                       int, T, hipStream_t);                                      \
   template hipError_t launch_z_r2c<T>(int, const T*, int64_t, cpx<T>*, int64_t, int, int64_t,   \

@@ -72,9 +72,11 @@ inline int x_stage_blocks(int N, int J) {
 
 
 // Strided pencil pass over [F][outer a][rows][B]: column j -> (a = j / B,
-// b = j % B); element (f, a, row, b) at f*fs + a*slab + row*B + b.  Input
-// rows retained: in_k of N (compact map with in_p positives); output rows
-// retained: out_k of N.  in_k == N / out_k == N is the identity map.
+// b = j % B); element (f, a, row, b) at f*fs + a*slab + row*rs + b, the row
+// stride rs = B unless in_rs / out_rs say otherwise (the y-forward reads
+// y-major pencil buffers: slab B, row stride nx*B).  Input rows retained:
+// in_k of N (compact map with in_p positives); output rows retained: out_k
+// of N.  in_k == N / out_k == N is the identity map.
 template <class T>
 struct StridedArgs {
   const cpx<T>* in;
@@ -82,6 +84,9 @@ struct StridedArgs {
   int64_t in_fs, out_fs, in_slab, out_slab;
   int B, J, nf;
   int in_k, in_p, out_k, out_p;
+  int64_t in_rs, out_rs;   // row strides; 0: B
+  __host__ __device__ int64_t in_row() const { return in_rs ? in_rs : B; }
+  __host__ __device__ int64_t out_row() const { return out_rs ? out_rs : B; }
 };
 
 // Fused x-forward + spectral operator + RK4 stage (pass P5).
@@ -184,12 +189,21 @@ template <class T>
 hipError_t launch_x_inverse_curl(int N, const cpx<T>* us, int64_t us_fs, cpx<T>* out,
                                  int64_t out_fs, const Geo& g, hipStream_t s,
                                  const FinalizeJob* job = nullptr);
+// out: the pencil buffer in the layout launch_z_fused reads
 template <class T>
 hipError_t launch_y_inverse_curl(int N, const cpx<T>* in, int64_t in_fs, cpx<T>* out,
-                                 int64_t out_fs, const Geo& g, hipStream_t s);
+                                 int64_t out_fs, const Geo& g, bool ymajor, hipStream_t s);
+// The forward RHS's pencil buffer (y-inverse out, z pass in/out, y-forward
+// in) is x-major, pencil (x, y) at (x ny + y) kzs, or, where pencil_ymajor
+// says so, y-major, pencil (x, y) at (y nx + x) kzs: a y-pass column
+// j = x kzs + iz then has its rows nx kzs apart, and the 16 columns of a
+// workgroup tile cover whole 128-byte lines of every row even when kzs
+// complex values are not a whole number of lines (kzs = 88 at N = 256).
 template <class T>
-hipError_t launch_z_fused(int N, cpx<T>* buf, int64_t fs, int64_t npencils, int kz, int kzs,
-                          T scale, hipStream_t s);
+bool pencil_ymajor(const Geo& g);
+template <class T>
+hipError_t launch_z_fused(int N, cpx<T>* buf, int64_t fs, int nx, int ny, int kz, int kzs,
+                          bool ymajor, T scale, hipStream_t s);
 template <class T>
 hipError_t launch_z_c2r(int N, const cpx<T>* in, int64_t in_fs, T* out, int64_t out_fs, int nf,
                         int64_t npencils, int kz, T scale, hipStream_t s);
