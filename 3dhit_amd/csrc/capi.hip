@@ -153,7 +153,9 @@ StepBuffers carve_step(const hit3d_grid& gd, const Geo& g, bool pruned, bool rk4
   b.us = c.take(st);
   if (rk4) b.acc = c.take(st);
   b.bufA = c.take(size_t(5) * g.nx * g.jp * cb);
-  b.bufB = c.take(size_t(6) * g.nx * g.ny * g.kzs * cb);
+  // pencil buffer: six fields, x-major or y-major with padded rows
+  const int yrow = gd.precision == 4 ? h3::pencil_row<float>(g) : h3::pencil_row<double>(g);
+  b.bufB = c.take(size_t(6) * std::max<size_t>(size_t(g.nx) * g.ny * g.kzs, size_t(g.ny) * yrow) * cb);
   const int nb = std::max(h3::x_stage_blocks(g.nx, g.ky * g.kzs), h3::kReduceBlocks);
   b.partials = static_cast<double*>(c.take(size_t(3) * nb * sizeof(double)));
   b.scalars = static_cast<double*>(c.take(16 * sizeof(double)));
@@ -285,23 +287,23 @@ int rhs_passes(const hit3d_grid& gd, const Geo& g, const cpx<T>* us, cpx<T>* buf
                hipStream_t s, PassTimer* tm = nullptr, const h3::FinalizeJob* job = nullptr) {
   const int64_t st_fs = int64_t(g.kx) * g.jp;
   const int64_t a_fs = int64_t(g.nx) * g.jp;
-  const int64_t b_fs = int64_t(g.nx) * g.ny * g.kzs;
+  const int yrow = h3::pencil_row<T>(g);   // 0: x-major pencil buffer
+  const int64_t b_fs = yrow ? int64_t(g.ny) * yrow : int64_t(g.nx) * g.ny * g.kzs;
   H3_HIP(h3::launch_x_inverse_curl<T>(g.nx, us, st_fs, bufA, a_fs, g, s, job));
   mark(tm, kP1);
-  const bool ymajor = h3::pencil_ymajor<T>(g);
-  H3_HIP(h3::launch_y_inverse_curl<T>(g.ny, bufA, a_fs, bufB, b_fs, g, ymajor, s));
+  H3_HIP(h3::launch_y_inverse_curl<T>(g.ny, bufA, a_fs, bufB, b_fs, g, yrow, s));
   mark(tm, kP2);
   const double n = double(gd.nx) * double(gd.ny) * double(gd.nz);
-  H3_HIP(h3::launch_z_fused<T>(g.nz, bufB, b_fs, g.nx, g.ny, g.kz, g.kzs, ymajor, T(1.0 / (n * n)), s));
+  H3_HIP(h3::launch_z_fused<T>(g.nz, bufB, b_fs, g.nx, g.ny, g.kz, g.kzs, yrow, T(1.0 / (n * n)), s));
   mark(tm, kP3);
   h3::StridedArgs<T> y{};
   y.in = bufB + h3::kZOut * b_fs;
   y.out = bufA;
   y.in_fs = b_fs;
   y.out_fs = a_fs;
-  if (ymajor) {   // column j = x kzs + iz, rows nx kzs apart
+  if (yrow) {   // column j = x kzs + iz, rows yrow apart
     y.in_slab = g.kzs;
-    y.in_rs = int64_t(g.nx) * g.kzs;
+    y.in_rs = yrow;
   } else {
     y.in_slab = int64_t(g.ny) * g.kzs;
   }

@@ -187,7 +187,7 @@ template <int N, class T, bool PIN = false>
 __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(MinWaves<N, T>::value))) void k_y_inverse_curl(const cpx<T>* __restrict__ in,
                                                             int64_t in_fs,
                                                             cpx<T>* __restrict__ out,
-                                                            int64_t out_fs, Geo g, int ymajor) {
+                                                            int64_t out_fs, Geo g, int yrow) {
   using S = StridedShape<N>;
   using F = typename S::F;
   constexpr int R = S::R, Q = S::Q, TC = S::TC;
@@ -212,8 +212,8 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
   const cpx<T>* src = in + int64_t(x) * g.jp + iz;
   // pencil buffer rows: y-major (row y at y nx kzs, column j) or x-major
   // (pencil (x, y) at (x ny + y) kzs)
-  cpx<T>* dst0 = out + (ymajor ? int64_t(jj) : int64_t(x) * g.ny * g.kzs + iz);
-  const int prs = ymajor ? g.nx * g.kzs : g.kzs;
+  cpx<T>* dst0 = out + (yrow ? int64_t(jj) : int64_t(x) * g.ny * g.kzs + iz);
+  const int prs = yrow ? yrow : g.kzs;
   auto load = [&](int f, cpx<T>* v) {
     const cpx<T>* p = src + f * in_fs;
 #pragma unroll
@@ -577,7 +577,7 @@ __device__ __forceinline__ void split_pair(cpx<T> ak, cpx<T> bk, cpx<T>& X, cpx<
 // ---------------------------------------------------------------------------
 template <int N, class T, bool PRUNED>
 __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_fused(cpx<T>* buf, int64_t fs, int nx, int ny,
-                                                     int kz, int kzs, T scale, int ymajor) {
+                                                     int kz, int kzs, T scale, int yrow) {
   using B = PencilBlock<N>;
   using F = typename B::F;
   static_assert(B::THREADS == 64, "WaveSync exchanges need one-wave blocks");
@@ -587,19 +587,20 @@ __global__ __launch_bounds__(PencilBlock<N>::THREADS) void k_z_fused(cpx<T>* buf
   const int q = threadIdx.x % Q, p = threadIdx.x / Q;
   // Workgroups walk the pencils x-major (the order in which the y-inverse
   // pass wrote them), each XCD's run from its end. x-major storage: pencil
-  // pp = x ny + y at pp kzs. y-major storage (ymajor, nx % PW == 0): the
-  // workgroup takes the PW x-adjacent pencils (x0 + p, y), whose rows are
-  // contiguous, so a group of pencils covers whole lines.
+  // pp = x ny + y at pp kzs. y-major storage (yrow > 0, nx % PW == 0):
+  // pencil (x, y) at y yrow + x kzs; the workgroup takes the PW x-adjacent
+  // pencils (x0 + p, y), whose rows are contiguous, so a group of pencils
+  // covers whole lines.
   const int64_t gi = xcd_tile_rev(blockIdx.x, gridDim.x);
   int64_t base;
   bool active;
-  if (ymajor) {
+  if (yrow) {
     active = gi < int64_t(nx / PW) * ny;
     // (walking the groups y-major, or in blocks of 4-16 x-groups per y, to
     // keep consecutive workgroups on adjacent memory measured slower: the
     // order the y-inverse pass wrote in matters more)
     const int ga = active ? int(gi) : 0, gx = ga / ny, gy = ga - gx * ny;
-    base = (int64_t(gy) * nx + gx * PW + p) * kzs;
+    base = int64_t(gy) * yrow + int64_t(gx * PW + p) * kzs;
   } else {
     const int64_t pp = gi * PW + p;
     active = pp < int64_t(nx) * ny;
@@ -1255,8 +1256,14 @@ hipError_t launch_x_inverse_curl(int N, const cpx<T>* us, int64_t us_fs, cpx<T>*
   return hipGetLastError();
 }
 
+// y-major rows are padded by 32 complex values (256 bytes for complex64):
+// measured at 256^3, pads of 0 / 16 / 32 / 48 / 64 / 128 / 256 values gave
+// 698.6 / 702.8 / 705.4 / 697 / 700 / 700 / 698 RK4 steps/s
+#ifndef H3_PENCIL_PAD
+#define H3_PENCIL_PAD 32
+#endif
 template <class T>
-bool pencil_ymajor(const Geo& g) {
+int pencil_row(const Geo& g) {
   // rows off 128-byte lines, a field small enough that rows nx kzs apart
   // stay within few 2 MB pages, and whole groups of x-adjacent pencils
   const int64_t row_bytes = int64_t(g.kzs) * int64_t(sizeof(cpx<T>));
@@ -1266,16 +1273,17 @@ bool pencil_ymajor(const Geo& g) {
 #define H3_PW(NN) case NN: pw = PencilBlock<NN>::PW; break;
     H3_PW(4) H3_PW(8) H3_PW(16) H3_PW(32) H3_PW(64) H3_PW(128) H3_PW(256) H3_PW(512) H3_PW(1024)
 #undef H3_PW
-    default: return false;
+    default: return 0;
   }
-  return row_bytes % 128 != 0 && field_bytes <= (int64_t(256) << 20) && g.nx % pw == 0;
+  if (row_bytes % 128 == 0 || field_bytes > (int64_t(256) << 20) || g.nx % pw != 0) return 0;
+  return g.nx * g.kzs + H3_PENCIL_PAD;
 }
 
 template <class T>
 hipError_t launch_y_inverse_curl(int N, const cpx<T>* in, int64_t in_fs, cpx<T>* out,
-                                 int64_t out_fs, const Geo& g, bool ymajor, hipStream_t s) {
+                                 int64_t out_fs, const Geo& g, int yrow, hipStream_t s) {
   if constexpr (lean_type<T>()) {
-    if (N == 1024 && !ymajor && fits_u32(int64_t(g.nx) * g.jp, sizeof(cpx<T>)) &&
+    if (N == 1024 && !yrow && fits_u32(int64_t(g.nx) * g.jp, sizeof(cpx<T>)) &&
         fits_u32(int64_t(g.nx) * g.ny * g.kzs, sizeof(cpx<T>))) {
       constexpr int NN = 1024;
       const unsigned nb = ceil_div(int64_t(g.nx) * g.kzs, LeanShape<NN>::TC);
@@ -1295,32 +1303,33 @@ hipError_t launch_y_inverse_curl(int N, const cpx<T>* in, int64_t in_fs, cpx<T>*
     if constexpr (kPrunedN<NN>) {
       if (pin) {
         hipLaunchKernelGGL((k_y_inverse_curl<NN, T, true>), grid, blk, 0, s, in, in_fs, out, out_fs, g,
-                           ymajor ? 1 : 0);
+                           yrow);
         break;
       }
     }
-    hipLaunchKernelGGL((k_y_inverse_curl<NN, T>), grid, blk, 0, s, in, in_fs, out, out_fs, g, ymajor ? 1 : 0);
+    hipLaunchKernelGGL((k_y_inverse_curl<NN, T>), grid, blk, 0, s, in, in_fs, out, out_fs, g, yrow);
   });
   return hipGetLastError();
 }
 
 template <class T>
 hipError_t launch_z_fused(int N, cpx<T>* buf, int64_t fs, int nx, int ny, int kz, int kzs,
-                          bool ymajor, T scale, hipStream_t s) {
+                          int yrow, T scale, hipStream_t s) {
   const int64_t npencils = int64_t(nx) * ny;
   // the pruned layout stores exactly kz = N/3 + 1 modes; anything else (the
   // dense layout, kz = N/2 + 1) takes the kernel without compile-time zeros
   // (and padding slots beyond kz, read as zeros, with 32-bit offsets inside
   // a field)
-  const bool pruned = kz == N / 3 + 1 && kz < N / 2 + 1 && kzs > kz && fits_u32(npencils * kzs, sizeof(cpx<T>));
+  const int64_t span = yrow ? int64_t(ny) * yrow : npencils * kzs;   // elements of a field
+  const bool pruned = kz == N / 3 + 1 && kz < N / 2 + 1 && kzs > kz && fits_u32(span, sizeof(cpx<T>));
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(npencils, PencilBlock<NN>::PW);
     if (pruned)
       hipLaunchKernelGGL((k_z_fused<NN, T, true>), dim3(nb), dim3(PencilBlock<NN>::THREADS), 0, s, buf, fs,
-                         nx, ny, kz, kzs, scale, ymajor ? 1 : 0);
+                         nx, ny, kz, kzs, scale, yrow);
     else
       hipLaunchKernelGGL((k_z_fused<NN, T, false>), dim3(nb), dim3(PencilBlock<NN>::THREADS), 0, s, buf, fs,
-                         nx, ny, kz, kzs, scale, ymajor ? 1 : 0);
+                         nx, ny, kz, kzs, scale, yrow);
   });
   return hipGetLastError();
 }
@@ -1433,9 +1442,9 @@ hipError_t launch_x_forward_stage(int N, int mode, const StageArgs<T>& a, hipStr
   template hipError_t launch_x_inverse_curl<T>(int, const cpx<T>*, int64_t, cpx<T>*, int64_t,   \
                                                const Geo&, hipStream_t, const FinalizeJob*);     \
   template hipError_t launch_y_inverse_curl<T>(int, const cpx<T>*, int64_t, cpx<T>*, int64_t,   \
-                                               const Geo&, bool, hipStream_t);                         \
-  template hipError_t launch_z_fused<T>(int, cpx<T>*, int64_t, int, int, int, int, bool, T, hipStream_t); \
-  template bool pencil_ymajor<T>(const Geo&); \
+                                               const Geo&, int, hipStream_t);                         \
+  template hipError_t launch_z_fused<T>(int, cpx<T>*, int64_t, int, int, int, int, int, T, hipStream_t); \
+  template int pencil_row<T>(const Geo&); \
   template hipError_t launch_z_c2r<T>(int, const cpx<T>*, int64_t, T*, int64_t, int, int64_t,   \
                                       int, T, hipStream_t);                                      \
   template hipError_t launch_z_r2c<T>(int, const T*, int64_t, cpx<T>*, int64_t, int, int64_t,   \

@@ -192,18 +192,20 @@ hipError_t launch_x_inverse_curl(int N, const cpx<T>* us, int64_t us_fs, cpx<T>*
 // out: the pencil buffer in the layout launch_z_fused reads
 template <class T>
 hipError_t launch_y_inverse_curl(int N, const cpx<T>* in, int64_t in_fs, cpx<T>* out,
-                                 int64_t out_fs, const Geo& g, bool ymajor, hipStream_t s);
+                                 int64_t out_fs, const Geo& g, int yrow, hipStream_t s);
 // The forward RHS's pencil buffer (y-inverse out, z pass in/out, y-forward
-// in) is x-major, pencil (x, y) at (x ny + y) kzs, or, where pencil_ymajor
-// says so, y-major, pencil (x, y) at (y nx + x) kzs: a y-pass column
-// j = x kzs + iz then has its rows nx kzs apart, and the 16 columns of a
-// workgroup tile cover whole 128-byte lines of every row even when kzs
-// complex values are not a whole number of lines (kzs = 88 at N = 256).
+// in) is x-major, pencil (x, y) at (x ny + y) kzs, or, where pencil_row
+// returns a row stride yrow > 0, y-major, pencil (x, y) at y yrow + x kzs
+// (yrow = nx kzs + a pad of 32 values, so that rows yrow apart do not all
+// fall in the same few cache sets and memory channels): a y-pass column j = x kzs + iz then has its rows yrow
+// apart, and the 16 columns of a workgroup tile cover whole 128-byte lines of
+// every row even when kzs complex values are not a whole number of lines
+// (kzs = 88 at N = 256). A y-major field spans ny yrow elements.
 template <class T>
-bool pencil_ymajor(const Geo& g);
+int pencil_row(const Geo& g);
 template <class T>
 hipError_t launch_z_fused(int N, cpx<T>* buf, int64_t fs, int nx, int ny, int kz, int kzs,
-                          bool ymajor, T scale, hipStream_t s);
+                          int yrow, T scale, hipStream_t s);
 template <class T>
 hipError_t launch_z_c2r(int N, const cpx<T>* in, int64_t in_fs, T* out, int64_t out_fs, int nf,
                         int64_t npencils, int kz, T scale, hipStream_t s);
