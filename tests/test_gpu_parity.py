@@ -188,6 +188,26 @@ def test_rhs_large_axis_pruned(shape, dtype, tol):
     assert rel(h3.make_ns_rhs(g)(dev(uh, g), params[1], 0.0), ref) < tol
 
 
+@pytest.mark.parametrize("dtype,tol", [(np.float32, 5e-6), (np.float64, 1e-12)])
+@pytest.mark.parametrize("shape", [(32, 64, 64), (64, 32, 64), (16, 64, 64), (256, 16, 64)])
+def test_rhs_and_rollout_pencil_layouts(shape, dtype, tol):
+    """The forward RHS's pencil buffer is y-major when its kz rows are not
+    whole 128-byte lines (float32 at Nz = 64: 24 modes, 192 bytes) and
+    x-major otherwise (float64: 384 bytes); both against the oracle on boxes
+    with Nx != Ny, where an x/y mix-up of the y-major addressing (row stride,
+    pencil groups along x, padded rows) would show, for the RHS and a
+    3-step forced rollout."""
+    og, g = _grids(shape, dtype)
+    uh = O.project(random_spectral(14, og), og)
+    op = O.NSParams(nu=0.01, forcing=O.LundgrenForcing(nu=0.01, energy=0.5, tau=0.3))
+    hp = h3.NSParams(nu=0.01, forcing=h3.LundgrenForcing(nu=0.01, energy=0.5, tau=0.3))
+    ref = O.make_ns_rhs(og)(uh, op, 0.0)
+    assert rel(h3.make_ns_rhs(g)(dev(uh, g), hp, 0.0), ref) < tol
+    ref = O.rollout(O.make_ns_rhs(og), uh, op, 1e-3, 3)
+    got = h3.rollout(h3.make_ns_rhs(g), dev(uh, g), hp, 1e-3, 3)
+    assert rel(got, ref) < (2e-5 if dtype == np.float32 else 1e-12)
+
+
 @pytest.mark.parametrize("shape", [(1024, 8, 4), (8, 8, 256), (4, 8, 512), (8, 512, 4)])
 def test_rollout_large_axis_pruned(shape):
     """RK4 stage kernels and the per-stage FP64 coefficient at large N."""
