@@ -281,42 +281,64 @@ bool needs_sums(const hit3d_forcing* f) {
   return f && (f->kind == HIT3D_FORCING_BAND || f->kind == HIT3D_FORCING_LUNDGREN);
 }
 
+// P2-P4 run over nch chunks of the x range, each a sub-grid of Nx/nch
+// x-planes through a pencil buffer that the chunks reuse. Two chunks when the
+// six-field pencil buffer exceeds the 256 MB Infinity Cache but half of it
+// does not (the float32 256^3 pruned step: 277 MB): a chunk's intermediates
+// then stay in that cache between P2, P3 and P4 (256^3: 702 -> 742 RK4 steps/s
+// on one box; three chunks 714, four 657 as the passes' grids shrink).
+int yz_chunks(const hit3d_grid& gd, const Geo& g) {
+  const double pencil_bytes = 6.0 * g.nx * g.ny * g.kzs * double(cbytes(gd));
+  const bool halves = pencil_bytes > 192.0 * (1 << 20) && pencil_bytes <= 384.0 * (1 << 20);
+  return halves && g.nx % 4 == 0 && g.nx >= 32 ? 2 : 1;
+}
+
 // P1-P4 of one RHS evaluation on the compact stage input `us`.
 template <class T>
 int rhs_passes(const hit3d_grid& gd, const Geo& g, const cpx<T>* us, cpx<T>* bufA, cpx<T>* bufB,
                hipStream_t s, PassTimer* tm = nullptr, const h3::FinalizeJob* job = nullptr) {
   const int64_t st_fs = int64_t(g.kx) * g.jp;
   const int64_t a_fs = int64_t(g.nx) * g.jp;
-  const int yrow = h3::pencil_row<T>(g);   // 0: x-major pencil buffer
-  const int64_t b_fs = yrow ? int64_t(g.ny) * yrow : int64_t(g.nx) * g.ny * g.kzs;
+  const int nch = yz_chunks(gd, g);
+  auto x_at = [&](int ch) { return ch * g.nx / nch; };   // nch divides nx / 2
+  Geo gm = g;
+  gm.nx = g.nx / nch;
+  const int yrow = h3::pencil_row<T>(gm);   // 0: x-major pencil buffer
+  const int64_t b_fs = yrow ? int64_t(g.ny) * yrow : int64_t(gm.nx) * g.ny * g.kzs;
+  const double n = double(gd.nx) * double(gd.ny) * double(gd.nz);
   H3_HIP(h3::launch_x_inverse_curl<T>(g.nx, us, st_fs, bufA, a_fs, g, s, job));
   mark(tm, kP1);
-  H3_HIP(h3::launch_y_inverse_curl<T>(g.ny, bufA, a_fs, bufB, b_fs, g, yrow, s));
-  mark(tm, kP2);
-  const double n = double(gd.nx) * double(gd.ny) * double(gd.nz);
-  H3_HIP(h3::launch_z_fused<T>(g.nz, bufB, b_fs, g.nx, g.ny, g.kz, g.kzs, yrow, T(1.0 / (n * n)), s));
-  mark(tm, kP3);
-  h3::StridedArgs<T> y{};
-  y.in = bufB + h3::kZOut * b_fs;
-  y.out = bufA;
-  y.in_fs = b_fs;
-  y.out_fs = a_fs;
-  if (yrow) {   // column j = x kzs + iz, rows yrow apart
-    y.in_slab = g.kzs;
-    y.in_rs = yrow;
-  } else {
-    y.in_slab = int64_t(g.ny) * g.kzs;
+  for (int ch = 0; ch < nch; ++ch) {
+    const int x0 = x_at(ch);
+    Geo gc = g;
+    gc.nx = x_at(ch + 1) - x0;
+    cpx<T>* const bufAc = bufA + int64_t(x0) * g.jp;
+    H3_HIP(h3::launch_y_inverse_curl<T>(g.ny, bufAc, a_fs, bufB, b_fs, gc, yrow, s));
+    mark(tm, kP2);
+    H3_HIP(h3::launch_z_fused<T>(g.nz, bufB, b_fs, gc.nx, g.ny, g.kz, g.kzs, yrow, T(1.0 / (n * n)), s));
+    mark(tm, kP3);
+    h3::StridedArgs<T> y{};
+    y.in = bufB + h3::kZOut * b_fs;
+    y.out = bufAc;
+    y.in_fs = b_fs;
+    y.out_fs = a_fs;
+    if (yrow) {   // column j = x kzs + iz, rows yrow apart
+      y.in_slab = g.kzs;
+      y.in_rs = yrow;
+    } else {
+      y.in_slab = int64_t(g.ny) * g.kzs;
+    }
+    y.out_slab = g.jp;
+    y.B = g.kzs;
+    y.J = gc.nx * g.kzs;
+    y.nf = 3;
+    y.in_k = g.ny;
+    y.in_p = (g.ny + 1) / 2;
+    y.out_k = g.ky;
+    y.out_p = g.py;
+    H3_HIP(h3::launch_strided<T>(g.ny, -1, y, s));
+    mark(tm, kP4);
   }
-  y.out_slab = g.jp;
-  y.B = g.kzs;
-  y.J = g.nx * g.kzs;
-  y.nf = 3;
-  y.in_k = g.ny;
-  y.in_p = (g.ny + 1) / 2;
-  y.out_k = g.ky;
-  y.out_p = g.py;
-  H3_HIP(h3::launch_strided<T>(g.ny, -1, y, s));
-  mark(tm, kP4);
   return HIT3D_OK;
 }
 

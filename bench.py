@@ -140,6 +140,7 @@ def pass_profile(h3, lib, g, u, nu, dt, fdesc, ws, stream, nsteps):
     del up
     per_bytes, _ = pass_bytes(g.Nx)
     passes = {}
+    nrhs = max(int(cnt[0]), 1)   # x-inverse: one launch per RHS evaluation
     for i, name in enumerate(lib.PASS_NAMES):
         if cnt[i] == 0:
             continue
@@ -147,8 +148,12 @@ def pass_profile(h3, lib, g, u, nu, dt, fdesc, ws, stream, nsteps):
         d = {"avg_ms": round(avg, 5), "launches": int(cnt[i]),
              "share": round(ms[i] / sum(ms[j] for j in range(7)), 4)}
         if name in per_bytes:
-            d["bytes"] = per_bytes[name]
-            d["gbs"] = round(per_bytes[name] / (avg * 1e-3) / 1e9, 1)
+            # P2-P4 may run as equal chunks of the x range (several launches
+            # per RHS): bytes and times are per launch
+            per_rhs = cnt[i] / nrhs
+            d["launches_per_rhs"] = round(per_rhs, 3)
+            d["bytes"] = int(round(per_bytes[name] / per_rhs))
+            d["gbs"] = round(d["bytes"] / (avg * 1e-3) / 1e9, 1)
         passes[name] = d
     return passes
 
@@ -501,7 +506,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled)",
-                     "traffic_source": traffic_src, "algorithmic_bytes": per_bytes[dom],
+                     "traffic_source": traffic_src, "algorithmic_bytes": passes[dom]["bytes"],
                      "avg_ms_live": passes[dom]["avg_ms"], "avg_ms_rocprof": rocprof_ms,
                      "rocprof_source": rocprof_src,
                      "frac_of_stream_copy": round(achieved / ceiling["best"], 4)},
