@@ -208,6 +208,27 @@ def test_rhs_and_rollout_pencil_layouts(shape, dtype, tol):
     assert rel(got, ref) < (2e-5 if dtype == np.float32 else 1e-12)
 
 
+def test_rhs_and_rollout_in_two_x_halves_with_x_major_pencils():
+    """A float32 pencil buffer between 192 and 384 MB makes the forward RHS
+    run its y-inverse, z and y-forward passes over two halves of the x range
+    (256^3 covers the y-major form); at (256, 512, 128) the kz rows (48 modes,
+    384 bytes) are whole lines, so the halves go through the x-major buffer
+    (302 MB). Against the float64 path on the same grid, which runs whole (its
+    pencil buffer is 604 MB)."""
+    shape = (256, 512, 128)
+    og = O.make_grid(*shape, dtype=np.float64)
+    uh = O.project(random_spectral(15, og), og)
+    g32, g64 = h3.make_grid(*shape, dtype=torch.float32), h3.make_grid(*shape, dtype=torch.float64)
+    hp = h3.NSParams(nu=0.01, forcing=h3.LundgrenForcing(nu=0.01, energy=0.5, tau=0.3))
+    u32 = dev(uh.astype(np.complex64), g32)
+    r32 = h3.make_ns_rhs(g32)(u32, hp, 0.0)
+    r64 = h3.make_ns_rhs(g64)(dev(uh, g64), hp, 0.0)
+    assert rel(r32, r64.cpu().numpy()) < 5e-6
+    u32 = h3.rollout(h3.make_ns_rhs(g32), u32, hp, 1e-3, 2)
+    u64 = h3.rollout(h3.make_ns_rhs(g64), dev(uh, g64), hp, 1e-3, 2)
+    assert rel(u32, u64.cpu().numpy()) < 2e-6
+
+
 @pytest.mark.parametrize("shape", [(1024, 8, 4), (8, 8, 256), (4, 8, 512), (8, 512, 4)])
 def test_rollout_large_axis_pruned(shape):
     """RK4 stage kernels and the per-stage FP64 coefficient at large N."""
