@@ -282,15 +282,21 @@ bool needs_sums(const hit3d_forcing* f) {
 }
 
 // P2-P4 run over nch chunks of the x range, each a sub-grid of Nx/nch
-// x-planes through a pencil buffer that the chunks reuse. Two chunks when the
-// six-field pencil buffer exceeds the 256 MB Infinity Cache but half of it
-// does not (the float32 256^3 pruned step: 277 MB): a chunk's intermediates
-// then stay in that cache between P2, P3 and P4 (256^3: 702 -> 742 RK4 steps/s
-// on one box; three chunks 714, four 657 as the passes' grids shrink).
+// x-planes through a pencil buffer that the chunks reuse. When the six-field
+// pencil buffer exceeds 192 MB (no longer fits the 256 MB Infinity Cache
+// beside the pass's other streams), the fewest chunks (2, 4, 8 or 16) of at
+// most 160 MB each: a chunk's intermediates then stay in that cache between
+// P2, P3 and P4. Measured (RK4 steps/s, one box): 256^3 float32 (277 MB)
+// 702 -> 742 with 2 chunks (3: 714, 4: 657, as the passes' grids shrink);
+// 512^3 float32 (2.2 GB) 85.4 -> 87.7 with 16 (8: 82.1, 4: 84.9); 1024^3
+// would need ~110 chunks and runs whole.
 int yz_chunks(const hit3d_grid& gd, const Geo& g) {
+  const double mb = double(1 << 20);
   const double pencil_bytes = 6.0 * g.nx * g.ny * g.kzs * double(cbytes(gd));
-  const bool halves = pencil_bytes > 192.0 * (1 << 20) && pencil_bytes <= 384.0 * (1 << 20);
-  return halves && g.nx % 4 == 0 && g.nx >= 32 ? 2 : 1;
+  if (pencil_bytes <= 192.0 * mb) return 1;
+  for (int nch = 2; nch <= 16; nch *= 2)
+    if (g.nx % (2 * nch) == 0 && g.nx / nch >= 16 && pencil_bytes / nch <= 160.0 * mb) return nch;
+  return 1;
 }
 
 // P1-P4 of one RHS evaluation on the compact stage input `us`.
@@ -300,7 +306,7 @@ int rhs_passes(const hit3d_grid& gd, const Geo& g, const cpx<T>* us, cpx<T>* buf
   const int64_t st_fs = int64_t(g.kx) * g.jp;
   const int64_t a_fs = int64_t(g.nx) * g.jp;
   const int nch = yz_chunks(gd, g);
-  auto x_at = [&](int ch) { return ch * g.nx / nch; };   // nch divides nx / 2
+  auto x_at = [&](int ch) { return ch * g.nx / nch; };   // 2 nch divides nx
   Geo gm = g;
   gm.nx = g.nx / nch;
   const int yrow = h3::pencil_row<T>(gm);   // 0: x-major pencil buffer

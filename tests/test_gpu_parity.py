@@ -229,6 +229,27 @@ def test_rhs_and_rollout_in_two_x_halves_with_x_major_pencils():
     assert rel(u32, u64.cpu().numpy()) < 2e-6
 
 
+def test_rhs_and_rollout_in_sixteen_x_chunks_at_512():
+    """The float32 512^3 pencil buffer (2.2 GB) makes the forward RHS run its
+    y-inverse, z and y-forward passes over 16 chunks of 32 x-planes; against
+    the float64 path (4.4 GB, run whole) from the same seeded field."""
+    g32, g64 = h3.make_grid(512, dtype=torch.float32), h3.make_grid(512, dtype=torch.float64)
+    u64 = h3.random_field(5, g64, h3.config_spectrum)
+    u32 = u64.to(torch.complex64)
+    hp = h3.NSParams(nu=1 / 2000, forcing=h3.LundgrenForcing(nu=1 / 2000, energy=0.5))
+    r32 = h3.make_ns_rhs(g32)(u32, hp, 0.0)
+    r64 = h3.make_ns_rhs(g64)(u64, hp, 0.0)
+    err = float(torch.linalg.vector_norm((r32.to(torch.complex128) - r64).reshape(-1))
+                / torch.linalg.vector_norm(r64.reshape(-1)))
+    assert err < 5e-6, err
+    del r32, r64
+    u32 = h3.rollout(h3.make_ns_rhs(g32), u32, hp, 2e-3, 2)
+    u64 = h3.rollout(h3.make_ns_rhs(g64), u64, hp, 2e-3, 2)
+    err = float(torch.linalg.vector_norm((u32.to(torch.complex128) - u64).reshape(-1))
+                / torch.linalg.vector_norm(u64.reshape(-1)))
+    assert err < 2e-6, err
+
+
 @pytest.mark.parametrize("shape", [(1024, 8, 4), (8, 8, 256), (4, 8, 512), (8, 512, 4)])
 def test_rollout_large_axis_pruned(shape):
     """RK4 stage kernels and the per-stage FP64 coefficient at large N."""
