@@ -1595,7 +1595,8 @@ int hit3d_stress_divergence(const hit3d_grid* grid, const void* in, void* out, i
 }
 
 int hit3d_rk4_combine(const hit3d_grid* grid, int32_t stage, double dt, const void* u_hat,
-                      const void* k_hat, void* acc, void* out, void* stream) {
+                      const void* k_hat, const double* hold_coef, const void* hold_us, void* acc,
+                      void* out, void* stream) {
   H3_TRY(check_grid(grid));
   if (stage < 1 || stage > 4) return fail(HIT3D_ERR_INVALID, "stage must be 1, 2, 3 or 4");
   if (!std::isfinite(dt)) return fail(HIT3D_ERR_INVALID, "dt must be finite");
@@ -1604,18 +1605,22 @@ int hit3d_rk4_combine(const hit3d_grid* grid, int32_t stage, double dt, const vo
   H3_TRY(check_ptr(acc, "acc"));
   H3_TRY(check_ptr(out, "out"));
   if (acc == out) return fail(HIT3D_ERR_INVALID, "acc and out must be different arrays");
+  if ((hold_coef == nullptr) != (hold_us == nullptr))
+    return fail(HIT3D_ERR_INVALID, "hold_coef and hold_us must both be given or both be NULL");
   const int64_t n = 3 * int64_t(grid->nx) * grid->ny * (grid->nz / 2 + 1);
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (grid->precision == 4)
     H3_HIP(h3::launch_rk4_combine<float>(static_cast<const cpx<float>*>(u_hat),
                                          static_cast<const cpx<float>*>(k_hat),
                                          static_cast<cpx<float>*>(acc), static_cast<cpx<float>*>(out),
-                                         n, stage, dt, s));
+                                         n, stage, dt, hold_coef, static_cast<const cpx<float>*>(hold_us),
+                                         s));
   else
     H3_HIP(h3::launch_rk4_combine<double>(static_cast<const cpx<double>*>(u_hat),
                                           static_cast<const cpx<double>*>(k_hat),
                                           static_cast<cpx<double>*>(acc),
-                                          static_cast<cpx<double>*>(out), n, stage, dt, s));
+                                          static_cast<cpx<double>*>(out), n, stage, dt, hold_coef,
+                                          static_cast<const cpx<double>*>(hold_us), s));
   return HIT3D_OK;
 }
 
@@ -1625,7 +1630,6 @@ int hit3d_hold_energy(const hit3d_grid* grid, const void* u_hat, const void* du_
   H3_TRY(check_grid(grid));
   H3_TRY(check_ptr(u_hat, "u_hat"));
   H3_TRY(check_ptr(du_hat, "du_hat"));
-  H3_TRY(check_ptr(out, "out"));
   H3_TRY(check_ptr(coef, "coef"));
   if (!(energy >= 0.0) || std::isnan(tau))
     return fail(HIT3D_ERR_INVALID, "energy must be >= 0 and tau a number");
@@ -1647,6 +1651,7 @@ int hit3d_hold_energy(const hit3d_grid* grid, const void* u_hat, const void* du_
   hipLaunchKernelGGL(k_hold_coef, dim3(1), dim3(256), 0, s, partials, h3::kReduceBlocks,
                      ntot2_inv(*grid), energy, tau, tiny, coef);
   H3_HIP(hipGetLastError());
+  if (out == nullptr) return HIT3D_OK;   // coefficient only
   if (grid->precision == 4)
     H3_HIP(h3::launch_hold_apply<float>(static_cast<const cpx<float>*>(u_hat),
                                         static_cast<const cpx<float>*>(du_hat),

@@ -680,13 +680,22 @@ __global__ __launch_bounds__(kBlock) void k_hold_apply(const cpx<T>* __restrict_
 //   stage 3: out = u + dt k,   acc += dt/3 k
 //   stage 4: out = acc + dt/6 k (u unused)
 // c_out and c_acc are the stage's coefficients in the state's precision.
+// With hold_coef, the stage derivative is k + hold_coef[0] us (the energy
+// hold's term on the stage input us, added here instead of by its own pass).
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_rk4_combine(const cpx<T>* u, const cpx<T>* k,
                                                         cpx<T>* __restrict__ acc, cpx<T>* out,
-                                                        int64_t n, int stage, T c_out, T c_acc) {
+                                                        int64_t n, int stage, T c_out, T c_acc,
+                                                        const double* __restrict__ hold_coef,
+                                                        const cpx<T>* hold_us) {
   const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= n) return;
-  const cpx<T> kv = k[i];
+  cpx<T> kv = k[i];
+  if (hold_coef) {   // uniform
+    const T a = T(hold_coef[0]);
+    const cpx<T> x = hold_us[i];
+    kv = {kv.x + a * x.x, kv.y + a * x.y};
+  }
   if (stage == 4) {
     const cpx<T> a = acc[i];
     out[i] = {a.x + c_acc * kv.x, a.y + c_acc * kv.y};
@@ -938,11 +947,12 @@ hipError_t launch_hold_apply(const cpx<T>* u, const cpx<T>* du, cpx<T>* out, int
 
 template <class T>
 hipError_t launch_rk4_combine(const cpx<T>* u, const cpx<T>* k, cpx<T>* acc, cpx<T>* out, int64_t n,
-                              int stage, double dt, hipStream_t s) {
+                              int stage, double dt, const double* hold_coef, const cpx<T>* hold_us,
+                              hipStream_t s) {
   const double co = stage == 3 ? dt : 0.5 * dt;
   const double ca = (stage == 1 || stage == 4) ? dt / 6.0 : dt / 3.0;
   hipLaunchKernelGGL((k_rk4_combine<T>), dim3(nblocks_for(n)), dim3(kBlock), 0, s, u, k, acc, out, n,
-                     stage, T(co), T(ca));
+                     stage, T(co), T(ca), hold_coef, hold_us);
   return hipGetLastError();
 }
 
@@ -994,7 +1004,8 @@ hipError_t launch_axpby(cpx<T>* out, T a, const cpx<T>* x, T b, const cpx<T>* y,
   template hipError_t launch_hold_apply<T>(const cpx<T>*, const cpx<T>*, cpx<T>*, int64_t,        \
                                            const double*, hipStream_t);                            \
   template hipError_t launch_rk4_combine<T>(const cpx<T>*, const cpx<T>*, cpx<T>*, cpx<T>*,        \
-                                            int64_t, int, double, hipStream_t);
+                                            int64_t, int, double, const double*, const cpx<T>*,    \
+                                            hipStream_t);
 
 H3_INSTANTIATE_ADJ(float)
 H3_INSTANTIATE_ADJ(double)

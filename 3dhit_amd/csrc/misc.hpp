@@ -86,7 +86,8 @@ hipError_t launch_hold_apply(const cpx<T>* u, const cpx<T>* du, cpx<T>* out, int
 // one classic-RK4 combination step over n complex values (stages 1..4)
 template <class T>
 hipError_t launch_rk4_combine(const cpx<T>* u, const cpx<T>* k, cpx<T>* acc, cpx<T>* out, int64_t n,
-                              int stage, double dt, hipStream_t s);
+                              int stage, double dt, const double* hold_coef, const cpx<T>* hold_us,
+                              hipStream_t s);
 struct AdjCombine {
   int fkind;          // HIT3D_FORCING_*
   double a;           // LINEAR coefficient

@@ -88,7 +88,7 @@ SIGNATURES = {
     "hit3d_stress_divergence": [_G, _P, _P, _I32, _P],
     "hit3d_hold_energy": [_G, _P, _P, _P, _D, _D, _P, _P, _SZ, _P],
     "hit3d_les_combine": [_G, _P, _P, _P, _P, _D, _P, _P],
-    "hit3d_rk4_combine": [_G, _I32, _D, _P, _P, _P, _P, _P],
+    "hit3d_rk4_combine": [_G, _I32, _D, _P, _P, _P, _P, _P, _P, _P],
 }
 
 PASS_NAMES = ("x_inverse_curl", "y_inverse_curl", "z_c2r_cross_r2c", "y_forward",

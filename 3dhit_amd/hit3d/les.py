@@ -243,10 +243,11 @@ def sphere_mask(grid, kcut):
     return m
 
 
-def _les_rhs_fused(u, rhs, p, hold, net, t, kcut):
+def _les_rhs_fused(u, rhs, p, hold, net, t, kcut, defer_hold=False):
     """les_rhs without autograd: the NS RHS, then the closure force, the
     sum and the spherical mask in one library pass (hit3d_les_combine),
-    then the energy hold."""
+    then the energy hold. defer_hold: return (RHS without the hold's term,
+    its device coefficients) for hit3d_rk4_combine to add A u."""
     grid = rhs.grid
     du = rhs(u, p, t, dealiased=True)
     out = torch.empty_like(du)
@@ -259,6 +260,8 @@ def _les_rhs_fused(u, rhs, p, hold, net, t, kcut):
     else:
         _lib.call("hit3d_les_combine", grid.desc, du.data_ptr(), None, None, None, kcut, out.data_ptr(),
                   stream())
+    if hold is not None and defer_hold:
+        return out, _hold_raw(u, out, grid, hold.energy, hold.tau, apply=False)[1]
     if hold is not None:
         out = _hold_raw(u, out, grid, hold.energy, hold.tau)[0]
     return out
@@ -282,16 +285,16 @@ class HoldResolvedEnergy:
         return f"HoldResolvedEnergy_E{float(self.energy):g}_tau{self.tau}"
 
 
-def _hold_raw(u, du, grid, energy, tau):
+def _hold_raw(u, du, grid, energy, tau, apply=True):
     u = check_spectral(u, grid, 3)
     if du.shape != u.shape or du.dtype != u.dtype or du.device != u.device:
         raise ValueError("du must match u in shape, dtype and device")
-    out = torch.empty_like(du)
+    out = torch.empty_like(du) if apply else None
     coef = torch.empty(3, dtype=torch.float64, device=du.device)
     ws = grid.workspace(_lib.OP_REDUCE)
     _lib.call("hit3d_hold_energy", grid.desc, u.contiguous().data_ptr(), du.contiguous().data_ptr(),
-              out.data_ptr(), float(energy), 0.0 if tau is None else float(tau), coef.data_ptr(),
-              ws.data_ptr(), ws.numel(), stream())
+              None if out is None else out.data_ptr(), float(energy), 0.0 if tau is None else float(tau),
+              coef.data_ptr(), ws.data_ptr(), ws.numel(), stream())
     return out, coef
 
 
@@ -377,15 +380,27 @@ def _les_step_fused(u, rhs, params, net, dt, t, mask, out):
         raise ValueError("out must be a contiguous array shaped and typed like u")
     us, acc = torch.empty_like(u), torch.empty_like(u)
     new = torch.empty_like(u) if out is None else out
+    hold = params.forcing if isinstance(params.forcing, HoldResolvedEnergy) else None
+    fused = grid.dtype == torch.float32 and (mask is None or getattr(mask, "kcut", None) is not None)
 
-    def combine(stage, k, dst):
+    def rhs_at(x, tt):
+        """(derivative, hold coefficients or None) at the stage input x"""
+        if hold is not None and fused:
+            p = NSParams(nu=params.nu, forcing=NoForcing())
+            return _les_rhs_fused(x, rhs, p, hold, net, tt, math.inf if mask is None else mask.kcut,
+                                  defer_hold=True)
+        return les_rhs(x, rhs, params, net, tt, mask), None
+
+    def combine(stage, x, tt, dst):
+        k, coef = rhs_at(x, tt)
         _lib.call("hit3d_rk4_combine", grid.desc, stage, float(dt), u.data_ptr(), k.data_ptr(),
+                  None if coef is None else coef.data_ptr(), None if coef is None else x.data_ptr(),
                   acc.data_ptr(), dst.data_ptr(), stream())
 
-    combine(1, les_rhs(u, rhs, params, net, t, mask), us)
-    combine(2, les_rhs(us, rhs, params, net, t + 0.5 * dt, mask), us)
-    combine(3, les_rhs(us, rhs, params, net, t + 0.5 * dt, mask), us)
-    combine(4, les_rhs(us, rhs, params, net, t + dt, mask), new)   # reads acc and k only
+    combine(1, u, t, us)
+    combine(2, us, t + 0.5 * dt, us)
+    combine(3, us, t + 0.5 * dt, us)
+    combine(4, us, t + dt, new)   # reads acc, k and us only
     return new
 
 

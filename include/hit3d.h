@@ -342,7 +342,8 @@ int hit3d_les_combine(const hit3d_grid* grid, const void* du_hat, const void* ta
    dE/dt = -(E - energy)/tau; tau <= 0 or infinite drops the relaxation term
    (dE/dt = 0), A = 0 when 2E <= finfo(dtype).eps. Full-layout states [3];
    coef (device, 3 doubles) receives [A, D, 2E]; FP64 fixed-order sums;
-   workspace of HIT3D_OP_REDUCE. out may alias du_hat or u_hat. */
+   workspace of HIT3D_OP_REDUCE. out may alias du_hat or u_hat; out NULL:
+   only coef is written (hit3d_rk4_combine can apply the term). */
 int hit3d_hold_energy(const hit3d_grid* grid, const void* u_hat, const void* du_hat, void* out,
                       double energy, double tau, double* coef, void* work, size_t work_bytes,
                       void* stream);
@@ -355,10 +356,14 @@ int hit3d_hold_energy(const hit3d_grid* grid, const void* u_hat, const void* du_
      stage 3: out = u + dt k,   acc = acc + dt/3 k
      stage 4: out = acc + dt/6 k (u_hat unused, may be NULL)
    k_hat is the stage's derivative, out the next stage's input (stage 4: the
-   new state). Elementwise, so out may alias u_hat or k_hat; acc must not
-   alias out. */
+   new state). With hold_coef (device, as hit3d_hold_energy writes it) and
+   hold_us (the stage's input state) the derivative is k_hat + hold_coef[0]
+   hold_us: the energy hold's term applied here instead of by its own pass
+   (both NULL: k_hat alone). Elementwise, so out may alias u_hat, k_hat or
+   hold_us; acc must not alias out. */
 int hit3d_rk4_combine(const hit3d_grid* grid, int32_t stage, double dt, const void* u_hat,
-                      const void* k_hat, void* acc, void* out, void* stream);
+                      const void* k_hat, const double* hold_coef, const void* hold_us, void* acc,
+                      void* out, void* stream);
 
 #ifdef __cplusplus
 }

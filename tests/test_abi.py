@@ -108,15 +108,17 @@ def test_adjoint_record_and_forcing_term_argument_checks():
 
 def test_rk4_combine_argument_checks():
     """hit3d_rk4_combine rejects a stage outside 1..4, a non-finite dt, a
-    missing u for stages 1-3 and acc aliasing out, before any device work."""
+    missing u for stages 1-3, acc aliasing out and a hold coefficient without
+    its stage state, before any device work."""
     from hit3d import _lib as L
     lib = _lib()
     g = _grid(16, 16, 16)
     P = ctypes.c_void_p
-    for stage, dt, u, acc, out, word in ((0, 0.1, P(4096), P(8192), P(12288), "stage"),
-                                         (5, 0.1, P(4096), P(8192), P(12288), "stage"),
-                                         (1, float("nan"), P(4096), P(8192), P(12288), "dt"),
-                                         (2, 0.1, None, P(8192), P(12288), "u_hat"),
-                                         (3, 0.1, P(4096), P(8192), P(8192), "different")):
-        rc = lib.hit3d_rk4_combine(ctypes.byref(g), stage, dt, u, P(16384), acc, out, None)
+    for stage, dt, u, hc, acc, out, word in ((0, 0.1, P(4096), None, P(8192), P(12288), "stage"),
+                                             (5, 0.1, P(4096), None, P(8192), P(12288), "stage"),
+                                             (1, float("nan"), P(4096), None, P(8192), P(12288), "dt"),
+                                             (2, 0.1, None, None, P(8192), P(12288), "u_hat"),
+                                             (3, 0.1, P(4096), None, P(8192), P(8192), "different"),
+                                             (3, 0.1, P(4096), P(20480), P(8192), P(12288), "both")):
+        rc = lib.hit3d_rk4_combine(ctypes.byref(g), stage, dt, u, P(16384), hc, None, acc, out, None)
         assert rc == L.HIT3D_ERR_INVALID and word in L.last_error(), (stage, L.last_error())

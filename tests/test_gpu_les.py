@@ -405,16 +405,27 @@ def test_rk4_combine_matches_the_stage_sums(dtype):
         acc = acc0.clone()
         out = torch.empty_like(u)
         _lib.call("hit3d_rk4_combine", g.desc, stage, dt, None if stage == 4 else u.data_ptr(),
-                  k.data_ptr(), acc.data_ptr(), out.data_ptr(), stream())
+                  k.data_ptr(), None, None, acc.data_ptr(), out.data_ptr(), stream())
         for got, ref in ((out, w_out), (acc, w_acc)):
             err = float((got.to(torch.complex128) - ref.to(torch.complex128)).abs().max())
             assert err <= tol * float(ref.abs().max()), (stage, err)
+    # with the energy hold's term: the derivative is k + coef[0] us
+    us = crand(shape, gen, cd)
+    coef = torch.tensor([0.7, 0.0, 0.0], dtype=torch.float64, device="cuda")
+    acc = acc0.clone()
+    out = torch.empty_like(u)
+    _lib.call("hit3d_rk4_combine", g.desc, 2, dt, u.data_ptr(), k.data_ptr(), coef.data_ptr(),
+              us.data_ptr(), acc.data_ptr(), out.data_ptr(), stream())
+    kh = k.to(torch.complex128) + 0.7 * us.to(torch.complex128)
+    for got, ref in ((out, u.to(torch.complex128) + dt / 2 * kh), (acc, acc0.to(torch.complex128) + dt / 3 * kh)):
+        err = float((got.to(torch.complex128) - ref).abs().max())
+        assert err <= 2 * tol * float(ref.abs().max()), err
     with pytest.raises(ValueError):
-        _lib.call("hit3d_rk4_combine", g.desc, 5, dt, u.data_ptr(), k.data_ptr(), acc.data_ptr(),
-                  out.data_ptr(), stream())
+        _lib.call("hit3d_rk4_combine", g.desc, 5, dt, u.data_ptr(), k.data_ptr(), None, None,
+                  acc.data_ptr(), out.data_ptr(), stream())
     with pytest.raises(ValueError):
-        _lib.call("hit3d_rk4_combine", g.desc, 2, dt, u.data_ptr(), k.data_ptr(), acc.data_ptr(),
-                  acc.data_ptr(), stream())
+        _lib.call("hit3d_rk4_combine", g.desc, 2, dt, u.data_ptr(), k.data_ptr(), None, None,
+                  acc.data_ptr(), acc.data_ptr(), stream())
 
 
 def test_les_step_without_gradients_matches_the_tensor_op_step():
