@@ -1084,7 +1084,10 @@ __global__ __launch_bounds__((StageShape<N, LEAN>::THREADS)) __attribute__((amdg
         o1 = LD(a.acc, 1, o) + sdt * d1;
         o2 = LD(a.acc, 2, o) + sdt * d2;
       } else {
-        const cpx<T> b0 = LD(a.u, 0, o), b1 = LD(a.u, 1, o), b2 = LD(a.u, 2, o);
+        // stage 1's input is the step-start state itself: its values are in hand
+        const bool same = MODE == kStageIf1 && !LEAN && a.u == a.us;   // uniform (lean: more spill)
+        const cpx<T> b0 = same ? s0 : LD(a.u, 0, o), b1 = same ? s1 : LD(a.u, 1, o),
+                     b2 = same ? s2 : LD(a.u, 2, o);
         if constexpr (MODE == kStageIf1) {
           o0 = eh * (b0 + hdt * d0); o1 = eh * (b1 + hdt * d1); o2 = eh * (b2 + hdt * d2);
           ST(a.acc, 0, o, ef * (b0 + sdt * d0));
@@ -1110,7 +1113,10 @@ __global__ __launch_bounds__((StageShape<N, LEAN>::THREADS)) __attribute__((amdg
       o1 = LD(a.acc, 1, o) + sdt * d1;
       o2 = LD(a.acc, 2, o) + sdt * d2;
     } else {
-      const cpx<T> b0 = LD(a.u, 0, o), b1 = LD(a.u, 1, o), b2 = LD(a.u, 2, o);
+      // stage 1's input is the step-start state itself: its values are in hand
+      const bool same = MODE == kStage1 && !LEAN && a.u == a.us;   // uniform (lean: more spill)
+      const cpx<T> b0 = same ? s0 : LD(a.u, 0, o), b1 = same ? s1 : LD(a.u, 1, o),
+                   b2 = same ? s2 : LD(a.u, 2, o);
       const T h = MODE == kStage3 ? a.dt : hdt;
       o0 = b0 + h * d0; o1 = b1 + h * d1; o2 = b2 + h * d2;
       if constexpr (MODE == kStage1) {
