@@ -123,7 +123,7 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
   constexpr int R = S::R, Q = S::Q, TC = S::TC;
   constexpr unsigned kZeroIn = PIN ? PrunedRows<F, N>::kIn : 0u;
   static_assert(S::THREADS % 64 == 0 && S::THREADS <= 1024, "finalize_in_block needs whole waves");
-  if (job.partials != nullptr && blockIdx.x == 0) finalize_in_block(job);   // uniform per block
+  if (job.partials != nullptr && blockIdx.x == 0 && blockIdx.y == 0) finalize_in_block(job);   // uniform per block
   __shared__ cpx<T> lds[S::LDS];
   const int c = threadIdx.x % TC, q = threadIdx.x / TC;
   const int j = xcd_tile(blockIdx.x, gridDim.x) * TC + c;
@@ -156,9 +156,32 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
 #pragma unroll
     for (int s = 0; s < R; ++s) dst[int64_t(F::out_index(q, s)) * g.jp] = v[s];
   };
+  cpx<T> a[R], b[R], c1[R], w[R];
+  if (gridDim.y == 2) {   // uniform: the five outputs in two groups (u0 read by both)
+    load(u0, a);
+    if (blockIdx.y == 0) {   // w1 = i(kz u0 - kx u2), u2, u0
+      load(u2, b);
+#pragma unroll
+      for (int r = 0; r < R; ++r) w[r] = mul_i(kzv * a[r] - kxv[r] * b[r]);
+      F::template run<+1>(w, q, L, tw);
+      store(3, w);
+      F::template run<+1>(b, q, L, tw);
+      store(2, b);
+      F::template run<+1>(a, q, L, tw);
+      store(0, a);
+    } else {   // w2 = i(kx u1 - ky u0), u1
+      load(u1, c1);
+#pragma unroll
+      for (int r = 0; r < R; ++r) w[r] = mul_i(kxv[r] * c1[r] - kyv * a[r]);
+      F::template run<+1>(w, q, L, tw);
+      store(4, w);
+      F::template run<+1>(c1, q, L, tw);
+      store(1, c1);
+    }
+    return;
+  }
   // all three components are loaded up front (the barriers inside run()
   // would otherwise delay every later load until the previous transform ends)
-  cpx<T> a[R], b[R], c1[R], w[R];
   load(u0, a);
   load(u2, b);
   load(u1, c1);
@@ -1274,7 +1297,11 @@ hipError_t launch_x_inverse_curl(int N, const cpx<T>* us, int64_t us_fs, cpx<T>*
   }
   const bool pin = pruned_rows(N, g.kx, g.px);
   H3_DISPATCH_N(N, {
-    const dim3 grid(ceil_div(int64_t(g.ky) * g.kzs, StridedShape<NN>::TC));
+    const unsigned nb = ceil_div(int64_t(g.ky) * g.kzs, StridedShape<NN>::TC);
+    // under 512 column tiles (one round of two per CU): the outputs in two
+    // groups over gridDim.y (128^3: 255 tiles, P1 12.9 -> 11.9 us, 4168 ->
+    // 4255 RK4 steps/s; at 256^3's 941 tiles the groups measured slower)
+    const dim3 grid(nb, nb < 512 ? 2 : 1);
     const dim3 blk(StridedShape<NN>::THREADS);
     if constexpr (kPrunedN<NN>) {
       if (pin) {
@@ -1330,10 +1357,10 @@ hipError_t launch_y_inverse_curl(int N, const cpx<T>* in, int64_t in_fs, cpx<T>*
   const bool pin = pruned_rows(N, g.ky, g.py);
   H3_DISPATCH_N(N, {
     const unsigned nb = ceil_div(int64_t(g.nx) * g.kzs, StridedShape<NN>::TC);
-    // under 1024 column tiles (two per CU per round): the outputs in two
+    // under 512 column tiles (one round of two per CU): the outputs in two
     // groups of three over gridDim.y (128^3: 384 tiles, P2 16.8 -> 14.9 us,
-    // 4068 -> 4195 RK4 steps/s; the 256^3 halves' 704 tiles: unchanged)
-    const dim3 grid(nb, nb < 1024 ? 2 : 1);
+    // 4068 -> 4195 RK4 steps/s; the 256^3 halves' 704 tiles: no change)
+    const dim3 grid(nb, nb < 512 ? 2 : 1);
     const dim3 blk(StridedShape<NN>::THREADS);
     if constexpr (kPrunedN<NN>) {
       if (pin) {
