@@ -113,12 +113,12 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
 // out fields: [u0, u1, u2, w1, w2]
 // ---------------------------------------------------------------------------
 template <int N, class T, bool PIN = false>
-__global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(MinWaves<N, T>::value))) void k_x_inverse_curl(const cpx<T>* __restrict__ us,
+__global__ __launch_bounds__(CurlShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(MinWaves<N, T>::value))) void k_x_inverse_curl(const cpx<T>* __restrict__ us,
                                                             int64_t us_fs,
                                                             cpx<T>* __restrict__ out,
                                                             int64_t out_fs, Geo g,
                                                             FinalizeJob job) {
-  using S = StridedShape<N>;
+  using S = CurlShape<N>;
   using F = typename S::F;
   constexpr int R = S::R, Q = S::Q, TC = S::TC;
   constexpr unsigned kZeroIn = PIN ? PrunedRows<F, N>::kIn : 0u;
@@ -207,11 +207,11 @@ __global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_wav
 // out [6][Nx][Ny][Kz] fields (u0, u1, u2, w0, w1, w2)
 // ---------------------------------------------------------------------------
 template <int N, class T, bool PIN = false>
-__global__ __launch_bounds__(StridedShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(MinWaves<N, T>::value))) void k_y_inverse_curl(const cpx<T>* __restrict__ in,
+__global__ __launch_bounds__(CurlShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(MinWaves<N, T>::value))) void k_y_inverse_curl(const cpx<T>* __restrict__ in,
                                                             int64_t in_fs,
                                                             cpx<T>* __restrict__ out,
                                                             int64_t out_fs, Geo g, int yrow) {
-  using S = StridedShape<N>;
+  using S = CurlShape<N>;
   using F = typename S::F;
   constexpr int R = S::R, Q = S::Q, TC = S::TC;
   constexpr unsigned kZeroIn = PIN ? PrunedRows<F, N>::kIn : 0u;
@@ -1297,12 +1297,12 @@ hipError_t launch_x_inverse_curl(int N, const cpx<T>* us, int64_t us_fs, cpx<T>*
   }
   const bool pin = pruned_rows(N, g.kx, g.px);
   H3_DISPATCH_N(N, {
-    const unsigned nb = ceil_div(int64_t(g.ky) * g.kzs, StridedShape<NN>::TC);
+    const unsigned nb = ceil_div(int64_t(g.ky) * g.kzs, CurlShape<NN>::TC);
     // under 512 column tiles (one round of two per CU): the outputs in two
     // groups over gridDim.y (128^3: 255 tiles, P1 12.9 -> 11.9 us, 4168 ->
     // 4255 RK4 steps/s; at 256^3's 941 tiles the groups measured slower)
     const dim3 grid(nb, nb < 512 ? 2 : 1);
-    const dim3 blk(StridedShape<NN>::THREADS);
+    const dim3 blk(CurlShape<NN>::THREADS);
     if constexpr (kPrunedN<NN>) {
       if (pin) {
         hipLaunchKernelGGL((k_x_inverse_curl<NN, T, true>), grid, blk, 0, s, us, us_fs, out, out_fs, g, j);
@@ -1356,12 +1356,12 @@ hipError_t launch_y_inverse_curl(int N, const cpx<T>* in, int64_t in_fs, cpx<T>*
   }
   const bool pin = pruned_rows(N, g.ky, g.py);
   H3_DISPATCH_N(N, {
-    const unsigned nb = ceil_div(int64_t(g.nx) * g.kzs, StridedShape<NN>::TC);
+    const unsigned nb = ceil_div(int64_t(g.nx) * g.kzs, CurlShape<NN>::TC);
     // under 512 column tiles (one round of two per CU): the outputs in two
     // groups of three over gridDim.y (128^3: 384 tiles, P2 16.8 -> 14.9 us,
     // 4068 -> 4195 RK4 steps/s; the 256^3 halves' 704 tiles: no change)
     const dim3 grid(nb, nb < 512 ? 2 : 1);
-    const dim3 blk(StridedShape<NN>::THREADS);
+    const dim3 blk(CurlShape<NN>::THREADS);
     if constexpr (kPrunedN<NN>) {
       if (pin) {
         hipLaunchKernelGGL((k_y_inverse_curl<NN, T, true>), grid, blk, 0, s, in, in_fs, out, out_fs, g,

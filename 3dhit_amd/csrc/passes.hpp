@@ -26,14 +26,20 @@ struct MinWaves {
   static constexpr int value = (N == 256 && sizeof(T) == 4) ? 4 : 1;
 };
 
-template <int N>
+// Column-tiled passes: TC columns of Q threads per workgroup. At 128 points
+// the plain strided and stage kernels take 8 columns (twice the workgroups
+// of a 128^3 grid's few tiles), the x/y-inverse passes 16 (they split their
+// outputs over gridDim.y instead).
+template <int N, int TCW = (N == 256 ? 16 : N == 128 ? 8 : 0)>
 struct StridedShape {
   using F = FftFor<N>;
   static constexpr int R = F::R, Q = F::Q;
-  static constexpr int TC = N == 256 ? 16 : ((kThreads / Q) > 8 ? kThreads / Q : 8);  // columns per workgroup
+  static constexpr int TC = TCW ? TCW : ((kThreads / Q) > 8 ? kThreads / Q : 8);  // columns per workgroup
   static constexpr int THREADS = TC * Q;
   static constexpr int LDS = TC * F::kSlots;
 };
+template <int N>
+using CurlShape = StridedShape<N, (N == 256 || N == 128) ? 16 : 0>;
 // Workgroup shape of the register-lean 1024-point passes (passes.hip): 16
 // columns (one 128-byte line of complex64 per row) of Q threads.
 template <int N>
